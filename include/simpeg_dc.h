@@ -1,0 +1,161 @@
+/*
+ * simpeg_dc.h - C ABI of the MI355X (gfx950) nodal DC-resistivity library (libsimpeg_dc.so).
+ *
+ * The library replaces, for SimPEG's Simulation3DNodal, the assembled sparse operator and the
+ * direct solver behind it (pymatsolver's factorisation: `self.solver(A)` and `Ainv * RHS`) with a
+ * matrix-free 7-point stencil, a geometric-multigrid-preconditioned block CG (fp32 inside fp64
+ * iterative refinement, or plain fp64), and hand-written kernels for the model derivatives.
+ * File:line citations are relative to the simpeg repository root.
+ *
+ * Conventions
+ *  - All array arguments named *_dev are device pointers (HIP global memory); the library never
+ *    frees caller memory.  Host arrays are only read during the call.
+ *  - A batch of node vectors ("columns", one per source or per adjoint right-hand side) is stored
+ *    column-major with column stride sdc_ld(h) doubles/floats: column c, node n lives at
+ *    base[c * ld + n], node n = i + (nx+1) * (j + (ny+1) * k) (x fastest, the discretize ordering).
+ *  - Every call is enqueued on `stream` (a hipStream_t; NULL = default stream).  Calls that
+ *    return host-visible results (sdc_solve statistics) synchronise that stream.
+ *  - Every function returns 0 on success or a nonzero SDC_ERR_* code; sdc_last_error() gives the
+ *    message for the calling thread.  A handle must not be used from two threads at once, and
+ *    calls that overlap on one handle race on its workspace.
+ */
+#ifndef SIMPEG_DC_H
+#define SIMPEG_DC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDC_OK 0
+#define SDC_ERR_ARG 1      /* invalid argument (shape, size, null pointer, bad enum) */
+#define SDC_ERR_HIP 2      /* HIP runtime error */
+#define SDC_ERR_NOCONV 3   /* solver did not reach the requested tolerance */
+#define SDC_ERR_STATE 4    /* call order error (e.g. solve before sdc_set_sigma) */
+#define SDC_ERR_ALLOC 5    /* device allocation failed */
+
+#define SDC_BC_ROBIN 0     /* Robin/Mixed boundary (reference default) */
+#define SDC_BC_NEUMANN 1   /* Neumann: row 0 of A replaced by e_0 */
+
+#define SDC_F32 0
+#define SDC_F64 1
+
+#define SDC_MODE_MIXED 0   /* fp32 GMG-PCG wrapped in fp64 iterative refinement */
+#define SDC_MODE_FP64 1    /* plain fp64 GMG-PCG */
+
+typedef struct sdc_handle_s* sdc_handle;
+
+/* Mesh and boundary description (host arrays).  Replaces the discretize TensorMesh operators
+ * used by Simulation3DNodal.getA and setBC (electromagnetics/static/resistivity/simulation.py:
+ * 523-555, 573-644): widths of the cells along each axis and, for bc_type Robin, the boundary
+ * face weights beta_f = alpha_f * |f| / 4 of the term _AvgBC = AvgN2Fb^T diag(alpha |f|) AvgCC2Fb
+ * on each of the six boundary planes (0 on surface faces). */
+typedef struct {
+  int32_t nx, ny, nz;          /* cells per axis (>= 1) */
+  const double* hx;            /* nx widths */
+  const double* hy;            /* ny widths */
+  const double* hz;            /* nz widths */
+  int32_t bc_type;             /* SDC_BC_ROBIN or SDC_BC_NEUMANN */
+  const double* beta_xmin;     /* ny*nz, index j + ny*k  (ignored for Neumann) */
+  const double* beta_xmax;
+  const double* beta_ymin;     /* nx*nz, index i + nx*k */
+  const double* beta_ymax;
+  const double* beta_zmin;     /* nx*ny, index i + nx*j */
+  const double* beta_zmax;
+  int32_t coarse_max;          /* multigrid: coarsen until at most this many nodes (0 = 512) */
+  int32_t device;              /* HIP device ordinal */
+} sdc_mesh_desc;
+
+/* Multigrid / Krylov tuning (defaults from sdc_default_params). */
+typedef struct {
+  int32_t nu_pre;              /* Chebyshev pre-smoothing steps (>= 1) */
+  int32_t nu_post;             /* Chebyshev post-smoothing steps (>= 1) */
+  double cheb_lmax;            /* upper bound of the spectrum of D^-1 A (Gershgorin: 2) */
+  double cheb_ratio;           /* lower bound = cheb_lmax / cheb_ratio */
+  int32_t rhs_per_thread;      /* 0 = automatic, else 1, 2, 4 or 8 columns per work-item */
+  int32_t check_every;         /* PCG iterations between host convergence polls */
+} sdc_params;
+
+typedef struct {
+  int32_t mode;                /* SDC_MODE_MIXED or SDC_MODE_FP64 */
+  double tol;                  /* per column target |q - A u|_2 / |q|_2 (fp64 residual) */
+  double inner_tol;            /* mixed: relative residual reduction of each fp32 PCG solve */
+  int32_t max_outer;           /* mixed: maximum refinement steps */
+  int32_t max_iter;            /* maximum PCG iterations per (inner) solve */
+} sdc_solve_opts;
+
+typedef struct {
+  int32_t outer;               /* refinement steps taken (1 for fp64 mode) */
+  int32_t iters;               /* PCG iterations summed over refinement steps */
+  double max_rel_res;          /* max over columns of |q - A u|/|q| after the solve */
+  int32_t converged;           /* 1 if every column reached tol */
+} sdc_solve_stats;
+
+/* Library identification and the last error message of the calling thread. */
+const char* sdc_version(void);
+const char* sdc_last_error(void);
+
+/* Create / destroy a handle holding the mesh, the multigrid hierarchy and the solver workspace
+ * (device memory owned by the handle).  Replaces Simulation3DNodal.__init__ + setBC
+ * (resistivity/simulation.py:483-501, 573-644). */
+int sdc_create(const sdc_mesh_desc* desc, sdc_handle* out);
+int sdc_destroy(sdc_handle h);
+
+/* Column stride (elements) of level-0 node vectors, and node/cell counts. */
+int64_t sdc_ld(sdc_handle h);
+int64_t sdc_num_nodes(sdc_handle h);
+int64_t sdc_num_cells(sdc_handle h);
+int32_t sdc_num_levels(sdc_handle h);
+
+int sdc_default_params(sdc_params* p);
+int sdc_set_params(sdc_handle h, const sdc_params* p);
+
+/* Set the cell conductivity (device fp64, nC, Fortran cell order) and rebuild the edge
+ * conductances of every level and the coarsest dense inverse.  Replaces MeSigma + getA
+ * (base/pde_simulation.py:150-162, resistivity/simulation.py:523-555) and the factorisation
+ * `self.Ainv = self.solver(A, **self.solver_opts)` (resistivity/simulation.py:108). */
+int sdc_set_sigma(sdc_handle h, const double* sigma_dev, void* stream);
+
+/* y = A x for ncol columns (dtype SDC_F32 or SDC_F64), with A exactly the reference operator
+ * getA() (Neumann: identity row 0).  resistivity/simulation.py:523-555. */
+int sdc_apply(sdc_handle h, int dtype, const void* x_dev, void* y_dev, int ncol, void* stream);
+
+/* z = M^-1 b: one multigrid V-cycle of the preconditioner (testing / external Krylov use). */
+int sdc_vcycle(sdc_handle h, int dtype, const void* b_dev, void* z_dev, int ncol, void* stream);
+
+/* Solve A u = q for ncol columns (fp64 q and u, stride sdc_ld).  Replaces `Ainv * RHS`
+ * (resistivity/simulation.py:111, 181, 245).  stats may be NULL. */
+int sdc_solve(sdc_handle h, const double* q_dev, double* u_dev, int ncol,
+              const sdc_solve_opts* opts, sdc_solve_stats* stats, void* stream);
+
+/* out = -A(dsig) u  (no Neumann row replacement): the Jvec right-hand side -getADeriv(u, v)
+ * with dsig = sigmaDeriv @ v (resistivity/simulation.py:557-565, 177-181). */
+int sdc_jvec_rhs(sdc_handle h, const double* dsig_dev, const double* u_dev, double* out_dev,
+                 int ncol, void* stream);
+
+/* Adjoint contraction of getADeriv(u, lam, adjoint=True) before the map derivative
+ * (resistivity/simulation.py:566-570, base/pde_simulation.py:35-36):
+ *   g_c = d(u^T A(sigma) lam)/d sigma_c = A_v^T (G u . G lam) + AvgBC^T (u . lam)
+ * Column j of lam is paired with column ucol[j] of u (ucol_dev NULL: j).  per_column = 0 sums
+ * over columns into out_dev[nC]; per_column = 1 writes out_dev[j*ldo + c] (getJ rows). */
+int sdc_jtvec_contract(sdc_handle h, const double* u_dev, const double* lam_dev, int ncol,
+                       const int32_t* ucol_dev, int per_column, double* out_dev, int64_t ldo,
+                       void* stream);
+
+/* d[i] = sum_{e in [rowptr[i], rowptr[i+1])} w[e] * u[idx[e]]: the receiver projection
+ * P u of BaseRx.eval (receivers.py:156-192), idx = flat column*ld + node. */
+int sdc_gather(sdc_handle h, int nrow, const int32_t* rowptr_dev, const int64_t* idx_dev,
+               const double* w_dev, const double* u_dev, double* d_dev, void* stream);
+
+/* out[dst[i]] = sum_{e in row i} w[e] * v[src[e]] after zeroing ncol columns of out: the adjoint
+ * projection P^T v (receivers.py:241-244) and the source term q (sources.py:64-88). */
+int sdc_scatter(sdc_handle h, int nrow, const int64_t* dst_dev, const int32_t* rowptr_dev,
+                const int32_t* src_dev, const double* w_dev, const double* v_dev,
+                double* out_dev, int ncol, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMPEG_DC_H */

@@ -1,0 +1,916 @@
+// HIP kernels for the matrix-free 3D nodal DC operator A(sigma) = G^T M_e(sigma) G + diag(AvgBC sigma),
+// its geometric-multigrid preconditioner, block-PCG vector updates, fp64 iterative
+// refinement, and the adjoint (Jtvec) edge-gradient contraction.
+//
+// Reference arithmetic restated here (paths relative to the simpeg repository):
+//   A(sigma):            electromagnetics/static/resistivity/simulation.py:523-555
+//   Robin term AvgBC:    electromagnetics/static/resistivity/simulation.py:579-644
+//   dA/dm (adjoint):     electromagnetics/static/resistivity/simulation.py:557-571,
+//                        base/pde_simulation.py:15-83, 296-323
+#pragma once
+#include "sdc_device.h"
+
+// ------------------------------------------------------------------------------------------
+// generic launch geometry: blockIdx.x -> 256 nodes (XCD-remapped), blockIdx.y -> R columns
+
+#define SDC_NODE_PROLOGUE(NN)                                  \
+  const uint32_t nbx = gridDim.x;                              \
+  const uint32_t bx = xcd_remap(blockIdx.x, nbx);              \
+  const uint32_t n = bx * SDC_BLOCK + threadIdx.x;             \
+  const int c0 = blockIdx.y * R;                               \
+  const bool live = n < (NN);
+
+// ------------------------------------------------------------------------------------------
+// operator application: y = A x, optional per-column partial sums of x.y
+
+template <typename T, int R, bool DOT>
+__global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> s, const T* __restrict__ x,
+                                                     T* __restrict__ y, size_t ld, int ncol,
+                                                     double* __restrict__ partial,
+                                                     const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(s.nN)
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  if (live) {
+    const NodeC<T> c = node_coef(s, n);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (c0 + r < ncol) {
+        const T* xc = x + (size_t)(c0 + r) * ld;
+        const T v = stencil(c, xc, n, s.n1x, s.plane);
+        y[(size_t)(c0 + r) * ld + n] = v;
+        if (DOT) acc[r] += (double)xc[n] * (double)v;
+      }
+    }
+  }
+  if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+}
+
+// r = b - A x
+template <typename T, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const T* __restrict__ b,
+                                                        const T* __restrict__ x,
+                                                        T* __restrict__ res, size_t ld, int ncol,
+                                                        const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(s.nN)
+  if (!live) return;
+  const NodeC<T> c = node_coef(s, n);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (c0 + r < ncol) {
+      const size_t o = (size_t)(c0 + r) * ld;
+      res[o + n] = b[o + n] - stencil(c, x + o, n, s.n1x, s.plane);
+    }
+  }
+}
+
+// First Chebyshev step from a zero guess: x = c0 * D^-1 b
+template <typename T, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const T* __restrict__ b,
+                                                       T* __restrict__ x, size_t ld, int ncol,
+                                                       T coef, const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(s.nN)
+  if (!live) return;
+  const T w = coef * s.idg[n];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (c0 + r < ncol) {
+      const size_t o = (size_t)(c0 + r) * ld;
+      x[o + n] = w * b[o + n];
+    }
+  }
+}
+
+// Chebyshev step: res = b - A xin; dn = c1*d + c2*D^-1 res; xout = xin + dn; dout = dn.
+// USE_D = false drops the c1*d term (first step from a non-zero guess).
+// DOT adds per-column partial sums of b.xout (the r.z product of PCG).
+template <typename T, int R, bool USE_D, bool DOT>
+__global__ __launch_bounds__(SDC_BLOCK) void k_smooth(Sten<T> s, const T* __restrict__ b,
+                                                      const T* __restrict__ xin,
+                                                      T* __restrict__ xout, const T* dsrc,
+                                                      T* ddst, size_t ld, int ncol, T c1, T c2,
+                                                      double* __restrict__ partial,
+                                                      const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(s.nN)
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  if (live) {
+    const NodeC<T> c = node_coef(s, n);
+    const T w = c2 * s.idg[n];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (c0 + r < ncol) {
+        const size_t o = (size_t)(c0 + r) * ld;
+        const T bv = b[o + n];
+        const T res = bv - stencil(c, xin + o, n, s.n1x, s.plane);
+        T dn = w * res;
+        if (USE_D) dn += c1 * dsrc[o + n];
+        const T xn = xin[o + n] + dn;
+        xout[o + n] = xn;
+        ddst[o + n] = dn;
+        if (DOT) acc[r] += (double)bv * (double)xn;
+      }
+    }
+  }
+  if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+}
+
+// ------------------------------------------------------------------------------------------
+// inter-grid transfer (vertex-based coarsening of each tensor axis, linear interpolation
+// in physical coordinates, restriction = transpose of interpolation)
+
+template <typename T>
+struct Xfer {
+  // restriction tables indexed by coarse index: fine index of the coarse node and the
+  // weights of its fine neighbours (0 when that neighbour is itself a coarse node)
+  const int* rix;
+  const int* riy;
+  const int* riz;
+  const T* rlx;
+  const T* rly;
+  const T* rlz;
+  const T* rhx;
+  const T* rhy;
+  const T* rhz;
+  // interpolation tables indexed by fine index: left coarse node and weight of the right one
+  const int* pix;
+  const int* piy;
+  const int* piz;
+  const T* pwx;
+  const T* pwy;
+  const T* pwz;
+  uint32_t fn1x, fn1y, fplane, fN;
+  uint32_t cn1x, cn1y, cplane, cN;
+  FastDiv ffx, ffy, cfx, cfy;
+};
+
+template <typename T, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __restrict__ rf,
+                                                        T* __restrict__ rc, size_t ldf, size_t ldc,
+                                                        int ncol, const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(t.cN)
+  if (!live) return;
+  const uint32_t rr = fdiv(n, t.cfx);
+  const uint32_t I = n - rr * t.cn1x;
+  const uint32_t K = fdiv(rr, t.cfy);
+  const uint32_t J = rr - K * t.cn1y;
+  const int fi = t.rix[I], fj = t.riy[J], fk = t.riz[K];
+  const T wx[3] = {t.rlx[I], T(1), t.rhx[I]};
+  const T wy[3] = {t.rly[J], T(1), t.rhy[J]};
+  const T wz[3] = {t.rlz[K], T(1), t.rhz[K]};
+  T acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = T(0);
+  for (int c = 0; c < 3; ++c) {
+    if (wz[c] == T(0)) continue;
+    for (int b = 0; b < 3; ++b) {
+      if (wy[b] == T(0)) continue;
+      const T wyz = wy[b] * wz[c];
+      const size_t rowb = (size_t)(fj + b - 1) * t.fn1x + (size_t)(fk + c - 1) * t.fplane;
+      for (int a = 0; a < 3; ++a) {
+        if (wx[a] == T(0)) continue;
+        const T w = wx[a] * wyz;
+        const size_t f = rowb + (size_t)(fi + a - 1);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (c0 + r < ncol) acc[r] += w * rf[(size_t)(c0 + r) * ldf + f];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (c0 + r < ncol) rc[(size_t)(c0 + r) * ldc + n] = acc[r];
+}
+
+// xf += P xc
+template <typename T, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const T* __restrict__ xc,
+                                                           T* __restrict__ xf, size_t ldf,
+                                                           size_t ldc, int ncol,
+                                                           const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(t.fN)
+  if (!live) return;
+  const uint32_t rr = fdiv(n, t.ffx);
+  const uint32_t i = n - rr * t.fn1x;
+  const uint32_t k = fdiv(rr, t.ffy);
+  const uint32_t j = rr - k * t.fn1y;
+  const int I = t.pix[i], J = t.piy[j], K = t.piz[k];
+  const T ux = t.pwx[i], uy = t.pwy[j], uz = t.pwz[k];
+  const T wx[2] = {T(1) - ux, ux};
+  const T wy[2] = {T(1) - uy, uy};
+  const T wz[2] = {T(1) - uz, uz};
+  T acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = T(0);
+  for (int c = 0; c < 2; ++c) {
+    if (wz[c] == T(0)) continue;
+    for (int b = 0; b < 2; ++b) {
+      if (wy[b] == T(0)) continue;
+      for (int a = 0; a < 2; ++a) {
+        if (wx[a] == T(0)) continue;
+        const T w = wx[a] * wy[b] * wz[c];
+        const size_t cc = (size_t)(I + a) + (size_t)(J + b) * t.cn1x + (size_t)(K + c) * t.cplane;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (c0 + r < ncol) acc[r] += w * xc[(size_t)(c0 + r) * ldc + cc];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (c0 + r < ncol) xf[(size_t)(c0 + r) * ldf + n] += acc[r];
+}
+
+// coarsest level: x = Ainv b (dense inverse, n x n; symmetric, so column m is read as row m
+// and consecutive work-items read consecutive addresses)
+template <typename T>
+__global__ __launch_bounds__(SDC_BLOCK) void k_dense_solve(const T* __restrict__ ainv, int nn,
+                                                           const T* __restrict__ b,
+                                                           T* __restrict__ x, size_t ld, int ncol,
+                                                           const int* __restrict__ done) {
+  if (done && *done) return;
+  const int row = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  const int col = blockIdx.y;
+  if (row >= nn || col >= ncol) return;
+  const T* bc = b + (size_t)col * ld;
+  T acc = T(0);
+  for (int m = 0; m < nn; ++m) acc += ainv[(size_t)m * nn + row] * bc[m];
+  x[(size_t)col * ld + row] = acc;
+}
+
+// ------------------------------------------------------------------------------------------
+// PCG vector updates and reductions
+
+template <typename T, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_dot(const T* __restrict__ a, const T* __restrict__ b,
+                                                   size_t ld, uint32_t nN, int ncol,
+                                                   double* __restrict__ partial,
+                                                   const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(nN)
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  if (live) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (c0 + r < ncol) {
+        const size_t o = (size_t)(c0 + r) * ld + n;
+        acc[r] = (double)a[o] * (double)b[o];
+      }
+  }
+  block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+}
+
+// x += alpha p ; r -= alpha q ; partial |r|^2
+template <typename T, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_update_xr(T* __restrict__ x, const T* __restrict__ p,
+                                                         T* __restrict__ r_,
+                                                         const T* __restrict__ q,
+                                                         const double* __restrict__ alpha,
+                                                         size_t ld, uint32_t nN, int ncol,
+                                                         double* __restrict__ partial,
+                                                         const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(nN)
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  if (live) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (c0 + r < ncol) {
+        const T al = (T)alpha[c0 + r];
+        const size_t o = (size_t)(c0 + r) * ld + n;
+        x[o] += al * p[o];
+        const T rn = r_[o] - al * q[o];
+        r_[o] = rn;
+        acc[r] = (double)rn * (double)rn;
+      }
+  }
+  block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+}
+
+// p = z + beta p  (FIRST: p = z)
+template <typename T, int R, bool FIRST>
+__global__ __launch_bounds__(SDC_BLOCK) void k_update_p(const T* __restrict__ z, T* __restrict__ p,
+                                                        const double* __restrict__ beta, size_t ld,
+                                                        uint32_t nN, int ncol,
+                                                        const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(nN)
+  if (!live) return;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (c0 + r < ncol) {
+      const size_t o = (size_t)(c0 + r) * ld + n;
+      p[o] = FIRST ? z[o] : z[o] + (T)beta[c0 + r] * p[o];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(SDC_BLOCK) void k_zero(T* __restrict__ x, size_t len) {
+  const size_t i = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (i < len) x[i] = T(0);
+}
+
+// ------------------------------------------------------------------------------------------
+// per-column scalar finalisation (one work-group per column, sums the block partials)
+
+__device__ __forceinline__ double block_sum(const double* __restrict__ part, uint32_t nbx) {
+  __shared__ double red[SDC_BLOCK / 64];
+  double v = 0.0;
+  for (uint32_t b = threadIdx.x; b < nbx; b += SDC_BLOCK) v += part[b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < SDC_BLOCK / 64; ++w) t += red[w];
+  return t;
+}
+
+// out[c] = sum of partials; mode 1 additionally initialises the PCG state
+__global__ __launch_bounds__(SDC_BLOCK) void k_fin_norm(const double* __restrict__ partial,
+                                                        uint32_t nbx, double* __restrict__ out,
+                                                        int* __restrict__ active, int init) {
+  const int c = blockIdx.x;
+  const double v = block_sum(partial + (size_t)c * nbx, nbx);
+  if (threadIdx.x == 0) {
+    out[c] = v;
+    if (init) active[c] = (v > 0.0) ? 1 : 0;
+  }
+}
+
+// alpha = rz / (p.Ap) on active columns, 0 elsewhere
+__global__ __launch_bounds__(SDC_BLOCK) void k_fin_alpha(const double* __restrict__ partial,
+                                                         uint32_t nbx,
+                                                         const double* __restrict__ rz,
+                                                         double* __restrict__ alpha,
+                                                         const int* __restrict__ active,
+                                                         const int* __restrict__ done) {
+  if (*done) return;
+  const int c = blockIdx.x;
+  const double pq = block_sum(partial + (size_t)c * nbx, nbx);
+  if (threadIdx.x == 0) alpha[c] = (active[c] && pq > 0.0) ? rz[c] / pq : 0.0;
+}
+
+// convergence test |r| <= tol |b| per column; the last work-group publishes the number of
+// still-active columns and the all-converged flag (device copy and host-mapped copy)
+__global__ __launch_bounds__(SDC_BLOCK) void k_fin_rr(const double* __restrict__ partial,
+                                                      uint32_t nbx,
+                                                      const double* __restrict__ bnorm2,
+                                                      double tol2, double* __restrict__ rr,
+                                                      int* __restrict__ active,
+                                                      int* __restrict__ counters, int* done,
+                                                      int* host_flag, int maxit_reached) {
+  if (*done) return;
+  const int c = blockIdx.x;
+  const double v = block_sum(partial + (size_t)c * nbx, nbx);
+  if (threadIdx.x == 0) {
+    rr[c] = v;
+    int a = active[c];
+    if (a && v <= tol2 * bnorm2[c]) a = 0;
+    active[c] = a;
+    if (a) atomicAdd(&counters[0], 1);
+    __threadfence();
+    const int ticket = atomicAdd(&counters[1], 1);
+    if (ticket == (int)gridDim.x - 1) {
+      const int nact = atomicAdd(&counters[0], 0);
+      const int fin = (nact == 0 || maxit_reached) ? 1 : 0;
+      *done = fin;
+      counters[2] = nact;
+      counters[0] = 0;
+      counters[1] = 0;
+      if (host_flag) {
+        host_flag[0] = fin;
+        host_flag[1] = nact;
+        __threadfence_system();
+      }
+    }
+  }
+}
+
+// beta = rz_new / rz_old on active columns
+__global__ __launch_bounds__(SDC_BLOCK) void k_fin_beta(const double* __restrict__ partial,
+                                                        uint32_t nbx, double* __restrict__ rz,
+                                                        double* __restrict__ beta,
+                                                        const int* __restrict__ active, int first,
+                                                        const int* __restrict__ done) {
+  if (*done) return;
+  const int c = blockIdx.x;
+  const double v = block_sum(partial + (size_t)c * nbx, nbx);
+  if (threadIdx.x == 0) {
+    const double old = rz[c];
+    beta[c] = (!first && active[c] && old != 0.0) ? v / old : 0.0;
+    rz[c] = v;
+  }
+}
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+// active[c] &= mask[c]
+__global__ __launch_bounds__(SDC_BLOCK) void k_and_active(int* __restrict__ active,
+                                                          const int* __restrict__ mask, int ncol) {
+  const int c = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (c < ncol && !mask[c]) active[c] = 0;
+}
+
+// refinement convergence: mask[c] = |r_c|^2 > tol^2 |q_c|^2; counters[0] += number still open
+__global__ __launch_bounds__(SDC_BLOCK) void k_outer_check(const double* __restrict__ rr,
+                                                           const double* __restrict__ qn2,
+                                                           double tol2, int* __restrict__ mask,
+                                                           int* __restrict__ nopen, int ncol) {
+  const int c = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (c >= ncol) return;
+  const int open = rr[c] > tol2 * qn2[c] ? 1 : 0;
+  mask[c] = open;
+  if (open) atomicAdd(nopen, 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// fp64 iterative refinement
+
+struct PinFix {
+  int on;      // Neumann gauge: rows next to node 0 get the column-0 term moved to the rhs
+  uint32_t n1x, plane;
+  const double* cx;  // level-0 conductances; the node-0 edges are cx[0], cy[0], cz[0]
+  const double* cy;
+  const double* cz;
+};
+
+// r = q - A' u (A' = reference operator: identity row 0 for Neumann), written in type TO;
+// with U_ZERO the u loads are skipped (first refinement step).  The rows next to node 0 are
+// shifted into the symmetric (identity row and column 0) form the inner PCG solves.
+// Partial sums of |r|^2 (before the shift) per column.
+template <typename TO, int R, bool U_ZERO>
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const double* __restrict__ q,
+                                                           const double* __restrict__ u,
+                                                           TO* __restrict__ rout, size_t ld,
+                                                           int ncol, PinFix pf,
+                                                           double* __restrict__ partial) {
+  SDC_NODE_PROLOGUE(s.nN)
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  if (live) {
+    NodeC<double> c;
+    if (!U_ZERO) c = node_coef(s, n);
+    double wfix = 0.0;
+    if (pf.on) {
+      if (n == 1) wfix = pf.cx[0];
+      else if (n == pf.n1x) wfix = pf.cy[0];
+      else if (n == pf.plane) wfix = pf.cz[0];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (c0 + r < ncol) {
+        const size_t o = (size_t)(c0 + r) * ld;
+        double rv = q[o + n];
+        if (!U_ZERO) rv -= stencil(c, u + o, n, s.n1x, s.plane);
+        acc[r] = rv * rv;
+        if (wfix != 0.0) {
+          const double r0 = U_ZERO ? q[o] : q[o] - u[o];
+          rv += wfix * r0;
+        }
+        rout[o + n] = (TO)rv;
+      }
+  }
+  block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+}
+
+// u += e
+template <int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_update(double* __restrict__ u,
+                                                         const float* __restrict__ e, size_t ld,
+                                                         uint32_t nN, int ncol) {
+  SDC_NODE_PROLOGUE(nN)
+  if (!live) return;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (c0 + r < ncol) {
+      const size_t o = (size_t)(c0 + r) * ld + n;
+      u[o] += (double)e[o];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// coefficient construction
+
+struct Geo {
+  const double* hx;
+  const double* hy;
+  const double* hz;
+  // Robin face weights beta_f = alpha_f |f| / 4 on the six boundary planes
+  const double* bxm;  // x = xmin faces, index j + ny*k
+  const double* bxp;
+  const double* bym;  // index i + nx*k
+  const double* byp;
+  const double* bzm;  // index i + nx*j
+  const double* bzp;
+  int robin;
+  uint32_t nx, ny, nz, n1x, n1y, nN, plane;
+  FastDiv fx, fy;     // by n1x, n1y (nodes)
+  FastDiv cfx, cfy;   // by nx, ny (cells)
+};
+
+// Edge conductances of the +x/+y/+z edges of every node and the Robin diagonal, from a cell
+// property sig (sigma, or a perturbation of it; A is linear in sigma):
+//   x edge (i,j,k)-(i+1,j,k): sum over the <=4 cells (i, j', k') touching it of
+//   sig * hy_j' * hz_k' / (4 hx_i), i.e. M_e = diag(A_v sig) with A_v[e,c] = V_c/4 divided by hx_i^2
+//   Robin: b_n = sum over boundary faces f containing n of beta_f * sig(cell of f)
+__global__ __launch_bounds__(SDC_BLOCK) void k_coef_from_sigma(Geo g, const double* __restrict__ sig,
+                                                               double* __restrict__ cx,
+                                                               double* __restrict__ cy,
+                                                               double* __restrict__ cz,
+                                                               double* __restrict__ bd,
+                                                               double scale) {
+  const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (n >= g.nN) return;
+  const uint32_t r = fdiv(n, g.fx);
+  const int i = (int)(n - r * g.n1x);
+  const int k = (int)fdiv(r, g.fy);
+  const int j = (int)(r - (uint32_t)k * g.n1y);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  // s[a][b][c] = sig of cell (i-1+a, j-1+b, k-1+c), 0 outside the mesh
+  double s[2][2][2];
+  double hxa[2], hyb[2], hzc[2];
+  for (int a = 0; a < 2; ++a) {
+    const int ii = i - 1 + a;
+    hxa[a] = (ii >= 0 && ii < nx) ? g.hx[ii] : 0.0;
+  }
+  for (int b = 0; b < 2; ++b) {
+    const int jj = j - 1 + b;
+    hyb[b] = (jj >= 0 && jj < ny) ? g.hy[jj] : 0.0;
+  }
+  for (int c = 0; c < 2; ++c) {
+    const int kk = k - 1 + c;
+    hzc[c] = (kk >= 0 && kk < nz) ? g.hz[kk] : 0.0;
+  }
+  for (int c = 0; c < 2; ++c)
+    for (int b = 0; b < 2; ++b)
+      for (int a = 0; a < 2; ++a) {
+        const int ii = i - 1 + a, jj = j - 1 + b, kk = k - 1 + c;
+        const bool ok = ii >= 0 && ii < nx && jj >= 0 && jj < ny && kk >= 0 && kk < nz;
+        s[a][b][c] = ok ? sig[(size_t)ii + (size_t)nx * ((size_t)jj + (size_t)ny * kk)] : 0.0;
+      }
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  if (i < nx) {
+    double t = 0.0;
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c) t += s[1][b][c] * hyb[b] * hzc[c];
+    wx = t / (4.0 * hxa[1]);
+  }
+  if (j < ny) {
+    double t = 0.0;
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c) t += s[a][1][c] * hxa[a] * hzc[c];
+    wy = t / (4.0 * hyb[1]);
+  }
+  if (k < nz) {
+    double t = 0.0;
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) t += s[a][b][1] * hxa[a] * hyb[b];
+    wz = t / (4.0 * hzc[1]);
+  }
+  double rb = 0.0;
+  if (g.robin) {
+    if (i == 0 || i == nx) {
+      const double* bf = (i == 0) ? g.bxm : g.bxp;
+      const int a = (i == 0) ? 1 : 0;
+      for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 2; ++c) {
+          const int jj = j - 1 + b, kk = k - 1 + c;
+          if (jj >= 0 && jj < ny && kk >= 0 && kk < nz) rb += bf[jj + ny * kk] * s[a][b][c];
+        }
+    }
+    if (j == 0 || j == ny) {
+      const double* bf = (j == 0) ? g.bym : g.byp;
+      const int b = (j == 0) ? 1 : 0;
+      for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < 2; ++c) {
+          const int ii = i - 1 + a, kk = k - 1 + c;
+          if (ii >= 0 && ii < nx && kk >= 0 && kk < nz) rb += bf[ii + nx * kk] * s[a][b][c];
+        }
+    }
+    if (k == 0 || k == nz) {
+      const double* bf = (k == 0) ? g.bzm : g.bzp;
+      const int c = (k == 0) ? 1 : 0;
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          const int ii = i - 1 + a, jj = j - 1 + b;
+          if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) rb += bf[ii + nx * jj] * s[a][b][c];
+        }
+    }
+  }
+  cx[n] = scale * wx;
+  cy[n] = scale * wy;
+  cz[n] = scale * wz;
+  bd[n] = scale * rb;
+}
+
+// full diagonal and its inverse; pin = 1 makes node 0 an identity row
+__global__ __launch_bounds__(SDC_BLOCK) void k_coef_diag(uint32_t n1x, uint32_t n1y, uint32_t nN,
+                                                         FastDiv fx, FastDiv fy,
+                                                         const double* __restrict__ cx,
+                                                         const double* __restrict__ cy,
+                                                         const double* __restrict__ cz,
+                                                         const double* __restrict__ bd,
+                                                         double* __restrict__ dg,
+                                                         double* __restrict__ idg, int pin) {
+  const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (n >= nN) return;
+  const uint32_t r = fdiv(n, fx);
+  const uint32_t i = n - r * n1x;
+  const uint32_t k = fdiv(r, fy);
+  const uint32_t j = r - k * n1y;
+  const uint32_t plane = n1x * n1y;
+  double d = cx[n] + cy[n] + cz[n] + bd[n];
+  if (i > 0) d += cx[n - 1];
+  if (j > 0) d += cy[n - n1x];
+  if (k > 0) d += cz[n - plane];
+  if (pin && n == 0) d = 1.0;
+  dg[n] = d;
+  idg[n] = 1.0 / d;
+}
+
+// Coarse-level conductances from fine ones: along the edge direction the (one or two)
+// fine edges between the two coarse nodes act in series; across it the fine edge lines
+// are summed with the interpolation weights of the transverse axes.  The Robin diagonal is
+// restricted like a vector (lumped Galerkin P^T b), so A_c 1 = P^T A_f 1.
+__global__ __launch_bounds__(SDC_BLOCK) void k_coarsen_coef(Xfer<double> t,
+                                                            const double* __restrict__ fcx,
+                                                            const double* __restrict__ fcy,
+                                                            const double* __restrict__ fcz,
+                                                            const double* __restrict__ fbd,
+                                                            double* __restrict__ ccx,
+                                                            double* __restrict__ ccy,
+                                                            double* __restrict__ ccz,
+                                                            double* __restrict__ cbd,
+                                                            uint32_t cn1z) {
+  const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (n >= t.cN) return;
+  const uint32_t rr = fdiv(n, t.cfx);
+  const uint32_t I = n - rr * t.cn1x;
+  const uint32_t K = fdiv(rr, t.cfy);
+  const uint32_t J = rr - K * t.cn1y;
+  const int fi = t.rix[I], fj = t.riy[J], fk = t.riz[K];
+  const double wx[3] = {t.rlx[I], 1.0, t.rhx[I]};
+  const double wy[3] = {t.rly[J], 1.0, t.rhy[J]};
+  const double wz[3] = {t.rlz[K], 1.0, t.rhz[K]};
+  auto fidx = [&](int a, int b, int c) -> size_t {
+    return (size_t)a + (size_t)b * t.fn1x + (size_t)c * t.fplane;
+  };
+  auto series = [](double c1, double c2) -> double {
+    const double s = c1 + c2;
+    return s > 0.0 ? c1 * c2 / s : 0.0;
+  };
+  // x edges
+  double sx = 0.0;
+  if (I + 1 < t.cn1x) {
+    const int span = t.rix[I + 1] - fi;
+    for (int c = 0; c < 3; ++c) {
+      if (wz[c] == 0.0) continue;
+      for (int b = 0; b < 3; ++b) {
+        if (wy[b] == 0.0) continue;
+        const size_t f = fidx(fi, fj + b - 1, fk + c - 1);
+        const double e = (span == 2) ? series(fcx[f], fcx[f + 1]) : fcx[f];
+        sx += wy[b] * wz[c] * e;
+      }
+    }
+  }
+  double sy = 0.0;
+  if (J + 1 < t.cn1y) {
+    const int span = t.riy[J + 1] - fj;
+    for (int c = 0; c < 3; ++c) {
+      if (wz[c] == 0.0) continue;
+      for (int a = 0; a < 3; ++a) {
+        if (wx[a] == 0.0) continue;
+        const size_t f = fidx(fi + a - 1, fj, fk + c - 1);
+        const double e = (span == 2) ? series(fcy[f], fcy[f + t.fn1x]) : fcy[f];
+        sy += wx[a] * wz[c] * e;
+      }
+    }
+  }
+  double sz = 0.0;
+  if (K + 1 < cn1z) {
+    const int span = t.riz[K + 1] - fk;
+    for (int b = 0; b < 3; ++b) {
+      if (wy[b] == 0.0) continue;
+      for (int a = 0; a < 3; ++a) {
+        if (wx[a] == 0.0) continue;
+        const size_t f = fidx(fi + a - 1, fj + b - 1, fk);
+        const double e = (span == 2) ? series(fcz[f], fcz[f + t.fplane]) : fcz[f];
+        sz += wx[a] * wy[b] * e;
+      }
+    }
+  }
+  double sb = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    if (wz[c] == 0.0) continue;
+    for (int b = 0; b < 3; ++b) {
+      if (wy[b] == 0.0) continue;
+      for (int a = 0; a < 3; ++a) {
+        if (wx[a] == 0.0) continue;
+        sb += wx[a] * wy[b] * wz[c] * fbd[fidx(fi + a - 1, fj + b - 1, fk + c - 1)];
+      }
+    }
+  }
+  ccx[n] = sx;
+  ccy[n] = sy;
+  ccz[n] = sz;
+  cbd[n] = sb;
+}
+
+template <typename TO>
+__global__ __launch_bounds__(SDC_BLOCK) void k_cast(const double* __restrict__ a, TO* __restrict__ b,
+                                                    size_t len) {
+  const size_t i = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (i < len) b[i] = (TO)a[i];
+}
+
+// dense operator of the coarsest level (row-major nn x nn), pin = 1: identity row/column 0
+__global__ __launch_bounds__(SDC_BLOCK) void k_dense_build(Sten<double> s, double* __restrict__ a) {
+  const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (n >= s.nN) return;
+  const size_t N = s.nN;
+  double* row = a + (size_t)n * N;
+  for (size_t m = 0; m < N; ++m) row[m] = 0.0;
+  const NodeC<double> c = node_coef(s, n);
+  row[n] = c.d;
+  if (c.hxm) row[n - 1] = -c.wxm;
+  if (c.hxp) row[n + 1] = -c.wxp;
+  if (c.hym) row[n - s.n1x] = -c.wym;
+  if (c.hyp) row[n + s.n1x] = -c.wyp;
+  if (c.hzm) row[n - s.plane] = -c.wzm;
+  if (c.hzp) row[n + s.plane] = -c.wzp;
+}
+
+// ainv64 <- (ainv64 + ainv64^T)/2 and its fp32 copy
+__global__ __launch_bounds__(SDC_BLOCK) void k_symmetrize(double* __restrict__ a,
+                                                          float* __restrict__ a32, int nn) {
+  const size_t e = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
+  const size_t N = nn;
+  if (e >= N * N) return;
+  const size_t r = e / N, c = e % N;
+  if (c < r) return;
+  const double v = 0.5 * (a[r * N + c] + a[c * N + r]);
+  a[r * N + c] = v;
+  a[c * N + r] = v;
+  a32[r * N + c] = (float)v;
+  a32[c * N + r] = (float)v;
+}
+
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one work-group of 1024.
+// a: nn x nn (destroyed), inv: nn x nn output, f: nn scratch for the elimination factors.
+__global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ a, double* __restrict__ inv,
+                                                       double* __restrict__ f, int nn) {
+  const int tid = threadIdx.x;
+  const size_t N = nn;
+  for (size_t e = tid; e < N * N; e += 1024) inv[e] = ((e / N) == (e % N)) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int p = 0; p < nn; ++p) {
+    const double piv = a[(size_t)p * N + p];
+    __syncthreads();
+    const double ip = 1.0 / piv;
+    for (int m = tid; m < nn; m += 1024) {
+      a[(size_t)p * N + m] *= ip;
+      inv[(size_t)p * N + m] *= ip;
+    }
+    __syncthreads();
+    for (int r = tid; r < nn; r += 1024) f[r] = (r == p) ? 0.0 : a[(size_t)r * N + p];
+    __syncthreads();
+    for (int m = tid; m < nn; m += 1024) {
+      const double ap = a[(size_t)p * N + m], vp = inv[(size_t)p * N + m];
+      for (int r = 0; r < nn; ++r) {
+        const double fr = f[r];
+        if (fr != 0.0) {
+          a[(size_t)r * N + m] -= fr * ap;
+          inv[(size_t)r * N + m] -= fr * vp;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// adjoint contraction (Jtvec):
+//   g_c = sum_j  (hy hz / 4hx) sum_{4 x-edges} du dl + (hx hz / 4hy) sum_{4 y-edges} du dl
+//              + (hx hy / 4hz) sum_{4 z-edges} du dl
+//              + sum_{boundary faces f of c} beta_f sum_{4 nodes of f} u l
+// (the derivative of u^T A(sigma) lambda with respect to the cell value sigma_c).
+// PERCOL writes one column per lambda column (getJ), otherwise the sum over columns.
+
+template <bool PERCOL>
+__global__ __launch_bounds__(SDC_BLOCK) void k_jtvec(Geo g, const double* __restrict__ u,
+                                                     const double* __restrict__ lam, size_t ld,
+                                                     int ncol, const int* __restrict__ ucol,
+                                                     double* __restrict__ out, size_t ldo) {
+  const uint32_t nC = g.nx * g.ny * g.nz;
+  const uint32_t nbx = gridDim.x;
+  const uint32_t c = xcd_remap(blockIdx.x, nbx) * SDC_BLOCK + threadIdx.x;
+  if (c >= nC) return;
+  const uint32_t r = fdiv(c, g.cfx);
+  const uint32_t i = c - r * g.nx;
+  const uint32_t k = fdiv(r, g.cfy);
+  const uint32_t j = r - k * g.ny;
+  const double hx = g.hx[i], hy = g.hy[j], hz = g.hz[k];
+  const double fxw = hy * hz / (4.0 * hx), fyw = hx * hz / (4.0 * hy), fzw = hx * hy / (4.0 * hz);
+  double bx0 = 0.0, bx1 = 0.0, by0 = 0.0, by1 = 0.0, bz0 = 0.0, bz1 = 0.0;
+  if (g.robin) {
+    if (i == 0) bx0 = g.bxm[j + g.ny * k];
+    if (i == g.nx - 1) bx1 = g.bxp[j + g.ny * k];
+    if (j == 0) by0 = g.bym[i + g.nx * k];
+    if (j == g.ny - 1) by1 = g.byp[i + g.nx * k];
+    if (k == 0) bz0 = g.bzm[i + g.nx * j];
+    if (k == g.nz - 1) bz1 = g.bzp[i + g.nx * j];
+  }
+  const size_t n000 = (size_t)i + (size_t)j * g.n1x + (size_t)k * g.plane;
+  size_t off[8];
+  for (int cc = 0; cc < 2; ++cc)
+    for (int b = 0; b < 2; ++b)
+      for (int a = 0; a < 2; ++a) off[a + 2 * b + 4 * cc] = n000 + a + b * g.n1x + cc * g.plane;
+  double total = 0.0;
+  for (int col = 0; col < ncol; ++col) {
+    const int sc = ucol ? ucol[col] : col;
+    const double* uc = u + (size_t)sc * ld;
+    const double* lc = lam + (size_t)col * ld;
+    double uv[8], lv[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      uv[m] = uc[off[m]];
+      lv[m] = lc[off[m]];
+    }
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // x edges: pairs (0,1),(2,3),(4,5),(6,7)
+      sx += (uv[2 * q + 1] - uv[2 * q]) * (lv[2 * q + 1] - lv[2 * q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // y edges: (0,2),(1,3),(4,6),(5,7)
+      const int m0 = (q & 1) + 4 * (q >> 1);
+      sy += (uv[m0 + 2] - uv[m0]) * (lv[m0 + 2] - lv[m0]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // z edges: (m, m+4)
+      sz += (uv[q + 4] - uv[q]) * (lv[q + 4] - lv[q]);
+    }
+    double gval = fxw * sx + fyw * sy + fzw * sz;
+    if (g.robin) {
+      double ul[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) ul[m] = uv[m] * lv[m];
+      gval += bx0 * (ul[0] + ul[2] + ul[4] + ul[6]) + bx1 * (ul[1] + ul[3] + ul[5] + ul[7]);
+      gval += by0 * (ul[0] + ul[1] + ul[4] + ul[5]) + by1 * (ul[2] + ul[3] + ul[6] + ul[7]);
+      gval += bz0 * (ul[0] + ul[1] + ul[2] + ul[3]) + bz1 * (ul[4] + ul[5] + ul[6] + ul[7]);
+    }
+    if (PERCOL)
+      out[(size_t)col * ldo + c] = gval;
+    else
+      total += gval;
+  }
+  if (!PERCOL) out[c] = total;
+}
+
+// ------------------------------------------------------------------------------------------
+// sparse projection to the receivers and its transpose (fp64)
+
+// d[i] = sum_{e in row i} w[e] * u[idx[e]]   (idx = flat index col*ld + node)
+__global__ __launch_bounds__(SDC_BLOCK) void k_gather(int nrow, const int* __restrict__ rowptr,
+                                                      const int64_t* __restrict__ idx,
+                                                      const double* __restrict__ w,
+                                                      const double* __restrict__ u,
+                                                      double* __restrict__ d) {
+  const int i = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (i >= nrow) return;
+  double acc = 0.0;
+  for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) acc += w[e] * u[idx[e]];
+  d[i] = acc;
+}
+
+// out[dst[i]] = sum_{e in row i} w[e] * v[src[e]]   (rows are distinct flat destinations)
+__global__ __launch_bounds__(SDC_BLOCK) void k_scatter(int nrow, const int64_t* __restrict__ dst,
+                                                       const int* __restrict__ rowptr,
+                                                       const int* __restrict__ src,
+                                                       const double* __restrict__ w,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ out) {
+  const int i = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (i >= nrow) return;
+  double acc = 0.0;
+  for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) acc += w[e] * v[src[e]];
+  out[dst[i]] = acc;
+}

@@ -1,0 +1,944 @@
+// Host side of libsimpeg_dc.so: handle, multigrid hierarchy, PCG / V-cycle / iterative
+// refinement drivers and the C ABI declared in include/simpeg_dc.h.
+#include "sdc_kernels.h"
+#include "simpeg_dc.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCK(x)                                                                     \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      return fail(SDC_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define CK(x)                 \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_ != SDC_OK) return rc_; \
+  } while (0)
+
+inline uint32_t nblk(size_t n) { return (uint32_t)((n + SDC_BLOCK - 1) / SDC_BLOCK); }
+
+template <typename T>
+int dmalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(SDC_ERR_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  return SDC_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// one tensor axis of a level: nodes 0..n (n cells)
+struct Axis {
+  int n = 0;
+  std::vector<double> X;  // node coordinates (cumulative widths)
+};
+
+// coarsen an axis: keep every other node, plus the last node when n is odd
+void coarsen_axis(const Axis& f, Axis& c, std::vector<int>& fidx) {
+  fidx.clear();
+  for (int i = 0; i <= f.n; i += 2) fidx.push_back(i);
+  if (fidx.back() != f.n) fidx.push_back(f.n);
+  c.n = (int)fidx.size() - 1;
+  c.X.resize(fidx.size());
+  for (size_t I = 0; I < fidx.size(); ++I) c.X[I] = f.X[fidx[I]];
+}
+
+template <typename T>
+struct XferDev {
+  int *rix = nullptr, *riy = nullptr, *riz = nullptr;
+  T *rlx = nullptr, *rly = nullptr, *rlz = nullptr, *rhx = nullptr, *rhy = nullptr, *rhz = nullptr;
+  int *pix = nullptr, *piy = nullptr, *piz = nullptr;
+  T *pwx = nullptr, *pwy = nullptr, *pwz = nullptr;
+  void release() {
+    dfree(rix); dfree(riy); dfree(riz);
+    dfree(rlx); dfree(rly); dfree(rlz); dfree(rhx); dfree(rhy); dfree(rhz);
+    dfree(pix); dfree(piy); dfree(piz);
+    dfree(pwx); dfree(pwy); dfree(pwz);
+  }
+};
+
+struct Level {
+  Axis ax[3];
+  uint32_t n1x = 0, n1y = 0, n1z = 0, nN = 0, plane = 0;
+  size_t ld = 0;
+  double *cx64 = nullptr, *cy64 = nullptr, *cz64 = nullptr, *bd64 = nullptr, *dg64 = nullptr,
+         *idg64 = nullptr;
+  float *cx32 = nullptr, *cy32 = nullptr, *cz32 = nullptr, *dg32 = nullptr, *idg32 = nullptr;
+  // transfer from this level to the next coarser one
+  bool has_coarse = false;
+  XferDev<double> x64;
+  XferDev<float> x32;
+  void release() {
+    dfree(cx64); dfree(cy64); dfree(cz64); dfree(bd64); dfree(dg64); dfree(idg64);
+    dfree(cx32); dfree(cy32); dfree(cz32); dfree(dg32); dfree(idg32);
+    x64.release();
+    x32.release();
+  }
+};
+
+template <typename T>
+struct Work {
+  int ncap = 0;
+  // level 0: PCG vectors and smoother scratch
+  T *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  // per level smoother / V-cycle vectors (index 0 uses r as rhs and z as solution)
+  std::vector<T*> b, xs, xa, d, res;
+  double* partial = nullptr;
+  double *alpha = nullptr, *beta = nullptr, *rz = nullptr, *rr = nullptr, *bn2 = nullptr,
+         *aux = nullptr;
+  int* active = nullptr;
+  int* counters = nullptr;  // [0] active count, [1] ticket, [2] last active count, [3] done
+  // refinement (outer) state: |q|^2 in aux, |q - A u|^2 in orr, open columns in omask
+  double* orr = nullptr;
+  int* omask = nullptr;
+  int* ocount = nullptr;
+  void release() {
+    dfree(x); dfree(r); dfree(z); dfree(p); dfree(q);
+    for (auto* v : b) dfree(v);
+    for (auto* v : xs) dfree(v);
+    for (auto* v : xa) dfree(v);
+    for (auto* v : d) dfree(v);
+    for (auto* v : res) dfree(v);
+    b.clear(); xs.clear(); xa.clear(); d.clear(); res.clear();
+    dfree(partial); dfree(alpha); dfree(beta); dfree(rz); dfree(rr); dfree(bn2); dfree(aux);
+    dfree(active); dfree(counters);
+    dfree(orr); dfree(omask); dfree(ocount);
+    ncap = 0;
+  }
+};
+
+}  // namespace
+
+struct sdc_handle_s {
+  int device = 0;
+  int bc = SDC_BC_ROBIN;
+  uint32_t nx = 0, ny = 0, nz = 0;
+  double *hx = nullptr, *hy = nullptr, *hz = nullptr;
+  double *bxm = nullptr, *bxp = nullptr, *bym = nullptr, *byp = nullptr, *bzm = nullptr,
+         *bzp = nullptr;
+  std::vector<Level> L;
+  int ncoarse = 0;
+  double* ainv64 = nullptr;
+  float* ainv32 = nullptr;
+  double *gj_a = nullptr, *gj_f = nullptr;
+  // scratch coefficient arrays for A(dsigma) (Jvec right-hand side)
+  double *tcx = nullptr, *tcy = nullptr, *tcz = nullptr, *tbd = nullptr, *tdg = nullptr,
+         *tidg = nullptr;
+  bool sigma_set = false;
+  sdc_params prm;
+  Work<float> w32;
+  Work<double> w64;
+  int* h_flag = nullptr;  // pinned host copy of the convergence flag
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+template <typename T>
+Sten<T> make_sten(const sdc_handle_s* h, int l, int pin_mode);
+
+template <>
+Sten<double> make_sten<double>(const sdc_handle_s* h, int l, int pin_mode) {
+  const Level& v = h->L[l];
+  Sten<double> s;
+  s.cx = v.cx64; s.cy = v.cy64; s.cz = v.cz64; s.dg = v.dg64; s.idg = v.idg64;
+  s.n1x = v.n1x; s.n1y = v.n1y; s.n1z = v.n1z; s.nN = v.nN; s.plane = v.plane;
+  s.fx = make_fastdiv(v.n1x); s.fy = make_fastdiv(v.n1y);
+  s.pin0 = (h->bc == SDC_BC_NEUMANN) ? pin_mode : 0;
+  return s;
+}
+
+template <>
+Sten<float> make_sten<float>(const sdc_handle_s* h, int l, int pin_mode) {
+  const Level& v = h->L[l];
+  Sten<float> s;
+  s.cx = v.cx32; s.cy = v.cy32; s.cz = v.cz32; s.dg = v.dg32; s.idg = v.idg32;
+  s.n1x = v.n1x; s.n1y = v.n1y; s.n1z = v.n1z; s.nN = v.nN; s.plane = v.plane;
+  s.fx = make_fastdiv(v.n1x); s.fy = make_fastdiv(v.n1y);
+  s.pin0 = (h->bc == SDC_BC_NEUMANN) ? pin_mode : 0;
+  return s;
+}
+
+template <typename T>
+const XferDev<T>& xfer_dev(const Level& v);
+template <>
+const XferDev<double>& xfer_dev<double>(const Level& v) { return v.x64; }
+template <>
+const XferDev<float>& xfer_dev<float>(const Level& v) { return v.x32; }
+
+template <typename T>
+Xfer<T> make_xfer(const sdc_handle_s* h, int l) {
+  const Level& f = h->L[l];
+  const Level& c = h->L[l + 1];
+  const XferDev<T>& d = xfer_dev<T>(f);
+  Xfer<T> t;
+  t.rix = d.rix; t.riy = d.riy; t.riz = d.riz;
+  t.rlx = d.rlx; t.rly = d.rly; t.rlz = d.rlz;
+  t.rhx = d.rhx; t.rhy = d.rhy; t.rhz = d.rhz;
+  t.pix = d.pix; t.piy = d.piy; t.piz = d.piz;
+  t.pwx = d.pwx; t.pwy = d.pwy; t.pwz = d.pwz;
+  t.fn1x = f.n1x; t.fn1y = f.n1y; t.fplane = f.plane; t.fN = f.nN;
+  t.cn1x = c.n1x; t.cn1y = c.n1y; t.cplane = c.plane; t.cN = c.nN;
+  t.ffx = make_fastdiv(f.n1x); t.ffy = make_fastdiv(f.n1y);
+  t.cfx = make_fastdiv(c.n1x); t.cfy = make_fastdiv(c.n1y);
+  return t;
+}
+
+Geo make_geo(const sdc_handle_s* h) {
+  Geo g;
+  g.hx = h->hx; g.hy = h->hy; g.hz = h->hz;
+  g.bxm = h->bxm; g.bxp = h->bxp; g.bym = h->bym; g.byp = h->byp; g.bzm = h->bzm; g.bzp = h->bzp;
+  g.robin = h->bc == SDC_BC_ROBIN;
+  g.nx = h->nx; g.ny = h->ny; g.nz = h->nz;
+  g.n1x = h->nx + 1; g.n1y = h->ny + 1; g.nN = h->L[0].nN; g.plane = g.n1x * g.n1y;
+  g.fx = make_fastdiv(g.n1x); g.fy = make_fastdiv(g.n1y);
+  g.cfx = make_fastdiv(h->nx); g.cfy = make_fastdiv(h->ny);
+  return g;
+}
+
+template <typename T>
+int upload(T** dst, const std::vector<T>& src) {
+  CK(dmalloc(dst, src.size()));
+  HIPCK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return SDC_OK;
+}
+
+template <typename T>
+int upload_xfer(XferDev<T>& d, const std::vector<int> (&ri)[3], const std::vector<double> (&rl)[3],
+                const std::vector<double> (&rh)[3], const std::vector<int> (&pi)[3],
+                const std::vector<double> (&pw)[3]) {
+  auto cvt = [](const std::vector<double>& a) { return std::vector<T>(a.begin(), a.end()); };
+  CK(upload(&d.rix, ri[0])); CK(upload(&d.riy, ri[1])); CK(upload(&d.riz, ri[2]));
+  CK(upload(&d.rlx, cvt(rl[0]))); CK(upload(&d.rly, cvt(rl[1]))); CK(upload(&d.rlz, cvt(rl[2])));
+  CK(upload(&d.rhx, cvt(rh[0]))); CK(upload(&d.rhy, cvt(rh[1]))); CK(upload(&d.rhz, cvt(rh[2])));
+  CK(upload(&d.pix, pi[0])); CK(upload(&d.piy, pi[1])); CK(upload(&d.piz, pi[2]));
+  CK(upload(&d.pwx, cvt(pw[0]))); CK(upload(&d.pwy, cvt(pw[1]))); CK(upload(&d.pwz, cvt(pw[2])));
+  return SDC_OK;
+}
+
+size_t round_ld(size_t n) { return (n + 63) / 64 * 64; }
+
+void fill_level_dims(Level& v) {
+  v.n1x = v.ax[0].n + 1;
+  v.n1y = v.ax[1].n + 1;
+  v.n1z = v.ax[2].n + 1;
+  v.plane = v.n1x * v.n1y;
+  v.nN = v.plane * v.n1z;
+  v.ld = round_ld(v.nN);
+}
+
+int alloc_level_arrays(Level& v) {
+  const size_t n = v.nN;
+  CK(dmalloc(&v.cx64, n)); CK(dmalloc(&v.cy64, n)); CK(dmalloc(&v.cz64, n));
+  CK(dmalloc(&v.bd64, n)); CK(dmalloc(&v.dg64, n)); CK(dmalloc(&v.idg64, n));
+  CK(dmalloc(&v.cx32, n)); CK(dmalloc(&v.cy32, n)); CK(dmalloc(&v.cz32, n));
+  CK(dmalloc(&v.dg32, n)); CK(dmalloc(&v.idg32, n));
+  return SDC_OK;
+}
+
+// transfer tables between fine level f and the coarse level built from it
+int make_transfer(Level& f, Level& c) {
+  std::vector<int> fidx[3], ri[3], pi[3];
+  std::vector<double> rl[3], rh[3], pw[3];
+  for (int a = 0; a < 3; ++a) {
+    coarsen_axis(f.ax[a], c.ax[a], fidx[a]);
+    const int nf = f.ax[a].n, nc = c.ax[a].n;
+    pi[a].assign(nf + 1, 0);
+    pw[a].assign(nf + 1, 0.0);
+    std::vector<char> is_c(nf + 1, 0);
+    for (int I = 0; I <= nc; ++I) {
+      is_c[fidx[a][I]] = 1;
+      pi[a][fidx[a][I]] = std::min(I, std::max(nc - 1, 0));
+      pw[a][fidx[a][I]] = (I == nc && nc > 0) ? 1.0 : 0.0;
+    }
+    for (int I = 0; I < nc; ++I) {
+      for (int i = fidx[a][I] + 1; i < fidx[a][I + 1]; ++i) {
+        pi[a][i] = I;
+        pw[a][i] = (f.ax[a].X[i] - c.ax[a].X[I]) / (c.ax[a].X[I + 1] - c.ax[a].X[I]);
+      }
+    }
+    ri[a].assign(nc + 1, 0);
+    rl[a].assign(nc + 1, 0.0);
+    rh[a].assign(nc + 1, 0.0);
+    for (int I = 0; I <= nc; ++I) {
+      const int fi = fidx[a][I];
+      ri[a][I] = fi;
+      if (I > 0 && fi - 1 > fidx[a][I - 1]) rl[a][I] = pw[a][fi - 1];
+      if (I < nc && fi + 1 < fidx[a][I + 1]) rh[a][I] = 1.0 - pw[a][fi + 1];
+    }
+  }
+  fill_level_dims(c);
+  CK(upload_xfer(f.x64, ri, rl, rh, pi, pw));
+  CK(upload_xfer(f.x32, ri, rl, rh, pi, pw));
+  f.has_coarse = true;
+  return SDC_OK;
+}
+
+template <typename T>
+Work<T>& work_of(sdc_handle_s* h);
+template <>
+Work<float>& work_of<float>(sdc_handle_s* h) { return h->w32; }
+template <>
+Work<double>& work_of<double>(sdc_handle_s* h) { return h->w64; }
+
+template <typename T>
+int ensure_work(sdc_handle_s* h, int ncol) {
+  Work<T>& w = work_of<T>(h);
+  if (w.ncap >= ncol) return SDC_OK;
+  w.release();
+  const int cap = std::max(ncol, 1);
+  const size_t ld0 = h->L[0].ld;
+  const size_t n0 = ld0 * (size_t)cap;
+  CK(dmalloc(&w.x, n0)); CK(dmalloc(&w.r, n0)); CK(dmalloc(&w.z, n0));
+  CK(dmalloc(&w.p, n0)); CK(dmalloc(&w.q, n0));
+  const size_t nl = h->L.size();
+  w.b.assign(nl, nullptr); w.xs.assign(nl, nullptr); w.xa.assign(nl, nullptr);
+  w.d.assign(nl, nullptr); w.res.assign(nl, nullptr);
+  for (size_t l = 0; l < nl; ++l) {
+    const size_t nv = h->L[l].ld * (size_t)cap;
+    if (l > 0) {
+      CK(dmalloc(&w.b[l], nv));
+      CK(dmalloc(&w.xs[l], nv));
+    }
+    CK(dmalloc(&w.xa[l], nv));
+    CK(dmalloc(&w.d[l], nv));
+    CK(dmalloc(&w.res[l], nv));
+  }
+  const size_t nb0 = nblk(h->L[0].nN);
+  CK(dmalloc(&w.partial, nb0 * (size_t)cap));
+  CK(dmalloc(&w.alpha, cap)); CK(dmalloc(&w.beta, cap)); CK(dmalloc(&w.rz, cap));
+  CK(dmalloc(&w.rr, cap)); CK(dmalloc(&w.bn2, cap)); CK(dmalloc(&w.aux, cap));
+  CK(dmalloc(&w.active, cap));
+  CK(dmalloc(&w.counters, 4));
+  HIPCK(hipMemset(w.counters, 0, 4 * sizeof(int)));
+  CK(dmalloc(&w.orr, cap));
+  CK(dmalloc(&w.omask, cap));
+  CK(dmalloc(&w.ocount, 1));
+  w.ncap = cap;
+  return SDC_OK;
+}
+
+int pick_r(const sdc_handle_s* h, int ncol) {
+  int r = h->prm.rhs_per_thread;
+  if (r == 1 || r == 2 || r == 4 || r == 8) return r;
+  if (ncol >= 8) return 8;
+  if (ncol >= 4) return 4;
+  if (ncol >= 2) return 2;
+  return 1;
+}
+
+#define DISPATCH_R(Rv, ...)                     \
+  switch (Rv) {                                 \
+    case 8: { constexpr int R = 8; __VA_ARGS__; } break; \
+    case 4: { constexpr int R = 4; __VA_ARGS__; } break; \
+    case 2: { constexpr int R = 2; __VA_ARGS__; } break; \
+    default: { constexpr int R = 1; __VA_ARGS__; } break; \
+  }
+
+inline dim3 grid2(uint32_t nN, int ncol, int R) {
+  return dim3(nblk(nN), (uint32_t)((ncol + R - 1) / R), 1);
+}
+
+// Chebyshev step coefficients for the interval [lmax/ratio, lmax] of D^-1 A
+struct Cheb {
+  double c1[16], c2[16];
+};
+
+Cheb cheb_coefs(const sdc_params& p) {
+  Cheb c;
+  const double lmax = p.cheb_lmax, lmin = p.cheb_lmax / p.cheb_ratio;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double s1 = theta / delta;
+  double rho_prev = 1.0 / s1;
+  c.c1[0] = 0.0;
+  c.c2[0] = 1.0 / theta;
+  for (int k = 1; k < 16; ++k) {
+    const double rho = 1.0 / (2.0 * s1 - rho_prev);
+    c.c1[k] = rho * rho_prev;
+    c.c2[k] = 2.0 * rho / delta;
+    rho_prev = rho;
+  }
+  return c;
+}
+
+// one V-cycle z = M^-1 b on level l; with dot_out the last kernel writes partials of b.z
+template <typename T>
+int vcycle(sdc_handle_s* h, int l, const T* b, T* x, int ncol, bool dot_out, const int* done,
+           hipStream_t st) {
+  Work<T>& W = work_of<T>(h);
+  const int R = pick_r(h, ncol);
+  const Level& lv = h->L[l];
+  const int nl = (int)h->L.size();
+  if (l == nl - 1) {
+    const T* ainv = (const T*)(std::is_same<T, float>::value ? (const void*)h->ainv32
+                                                              : (const void*)h->ainv64);
+    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(lv.nN), ncol), dim3(SDC_BLOCK), 0, st, ainv,
+                       (int)lv.nN, b, x, lv.ld, ncol, done);
+    if (dot_out) {
+      DISPATCH_R(R, hipLaunchKernelGGL((k_dot<T, R>), grid2(lv.nN, ncol, R), dim3(SDC_BLOCK), 0,
+                                       st, b, (const T*)x, lv.ld, lv.nN, ncol, W.partial, done));
+    }
+    return SDC_OK;
+  }
+  const Sten<T> s = make_sten<T>(h, l, 1);
+  const Cheb cc = cheb_coefs(h->prm);
+  const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
+  const int nswitch = (nu1 - 1) + nu2;
+  T* cur = (nswitch % 2 == 0) ? x : W.xa[l];
+  T* oth = (cur == x) ? W.xa[l] : x;
+  const dim3 g = grid2(lv.nN, ncol, R);
+  const dim3 blk(SDC_BLOCK);
+  DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol,
+                                   (T)cc.c2[0], done));
+  for (int k = 1; k < nu1; ++k) {
+    const T* dsrc = (k == 1) ? cur : W.d[l];
+    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, true, false>), g, blk, 0, st, s, b,
+                                     (const T*)cur, oth, dsrc, W.d[l], lv.ld, ncol, (T)cc.c1[k],
+                                     (T)cc.c2[k], (double*)nullptr, done));
+    std::swap(cur, oth);
+  }
+  DISPATCH_R(R, hipLaunchKernelGGL((k_residual<T, R>), g, blk, 0, st, s, b, (const T*)cur,
+                                   W.res[l], lv.ld, ncol, done));
+  const Xfer<T> t = make_xfer<T>(h, l);
+  const Level& cl = h->L[l + 1];
+  DISPATCH_R(R, hipLaunchKernelGGL((k_restrict<T, R>), grid2(cl.nN, ncol, R), blk, 0, st, t,
+                                   (const T*)W.res[l], W.b[l + 1], lv.ld, cl.ld, ncol, done));
+  CK(vcycle<T>(h, l + 1, W.b[l + 1], W.xs[l + 1], ncol, false, done, st));
+  DISPATCH_R(R, hipLaunchKernelGGL((k_prolong_add<T, R>), g, blk, 0, st, t,
+                                   (const T*)W.xs[l + 1], cur, lv.ld, cl.ld, ncol, done));
+  for (int k = 0; k < nu2; ++k) {
+    const bool last = (k == nu2 - 1) && dot_out;
+    if (k == 0) {
+      if (last) {
+        DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, false, true>), g, blk, 0, st, s, b,
+                                         (const T*)cur, oth, (const T*)W.d[l], W.d[l], lv.ld,
+                                         ncol, (T)0, (T)cc.c2[0], W.partial, done));
+      } else {
+        DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, false, false>), g, blk, 0, st, s, b,
+                                         (const T*)cur, oth, (const T*)W.d[l], W.d[l], lv.ld,
+                                         ncol, (T)0, (T)cc.c2[0], (double*)nullptr, done));
+      }
+    } else {
+      if (last) {
+        DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, true, true>), g, blk, 0, st, s, b,
+                                         (const T*)cur, oth, (const T*)W.d[l], W.d[l], lv.ld,
+                                         ncol, (T)cc.c1[k], (T)cc.c2[k], W.partial, done));
+      } else {
+        DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, true, false>), g, blk, 0, st, s, b,
+                                         (const T*)cur, oth, (const T*)W.d[l], W.d[l], lv.ld,
+                                         ncol, (T)cc.c1[k], (T)cc.c2[k], (double*)nullptr, done));
+      }
+    }
+    std::swap(cur, oth);
+  }
+  if (cur != x) return fail(SDC_ERR_STATE, "internal: V-cycle buffer parity");
+  return SDC_OK;
+}
+
+// Preconditioned CG on ncol columns: rhs in W.r (overwritten by the residual), solution in x.
+// Each column stops when |r| <= tol |b|.  col_mask (device, may be null) excludes columns.
+template <typename T>
+int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_mask, int* iters,
+        hipStream_t st) {
+  Work<T>& W = work_of<T>(h);
+  const Level& l0 = h->L[0];
+  const int R = pick_r(h, ncol);
+  const dim3 g = grid2(l0.nN, ncol, R);
+  const dim3 blk(SDC_BLOCK);
+  const uint32_t nbx = nblk(l0.nN);
+  const size_t len = l0.ld * (size_t)ncol;
+  int* done = W.counters + 3;
+  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(len)), blk, 0, st, x, len);
+  hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)4);
+  DISPATCH_R(R, hipLaunchKernelGGL((k_dot<T, R>), g, blk, 0, st, (const T*)W.r, (const T*)W.r,
+                                   l0.ld, l0.nN, ncol, W.partial, (const int*)nullptr));
+  hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.bn2,
+                     W.active, 1);
+  if (col_mask)
+    hipLaunchKernelGGL(k_and_active, dim3(nblk(ncol)), blk, 0, st, W.active, col_mask, ncol);
+  CK(vcycle<T>(h, 0, W.r, W.z, ncol, true, done, st));
+  hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
+                     W.beta, (const int*)W.active, 1, (const int*)done);
+  DISPATCH_R(R, hipLaunchKernelGGL((k_update_p<T, R, true>), g, blk, 0, st, (const T*)W.z, W.p,
+                                   (const double*)W.beta, l0.ld, l0.nN, ncol, (const int*)done));
+  const Sten<T> s = make_sten<T>(h, 0, 1);
+  const double tol2 = tol * tol;
+  const int check = std::max(1, (int)h->prm.check_every);
+  int it = 0;
+  bool finished = false;
+  for (it = 1; it <= maxit && !finished; ++it) {
+    DISPATCH_R(R, hipLaunchKernelGGL((k_apply<T, R, true>), g, blk, 0, st, s, (const T*)W.p, W.q,
+                                     l0.ld, ncol, W.partial, (const int*)done));
+    hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
+                       (const double*)W.rz, W.alpha, (const int*)W.active, (const int*)done);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_update_xr<T, R>), g, blk, 0, st, x, (const T*)W.p, W.r,
+                                     (const T*)W.q, (const double*)W.alpha, l0.ld, l0.nN, ncol,
+                                     W.partial, (const int*)done));
+    hipLaunchKernelGGL(k_fin_rr, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
+                       (const double*)W.bn2, tol2, W.rr, W.active, W.counters, done,
+                       (int*)nullptr, (int)(it == maxit));
+    if (it % check == 0 || it == maxit) {
+      HIPCK(hipMemcpyAsync(h->h_flag, done, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCK(hipStreamSynchronize(st));
+      if (h->h_flag[0]) {
+        finished = true;
+        break;
+      }
+    }
+    CK(vcycle<T>(h, 0, W.r, W.z, ncol, true, done, st));
+    hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
+                       W.beta, (const int*)W.active, 0, (const int*)done);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_update_p<T, R, false>), g, blk, 0, st, (const T*)W.z,
+                                     W.p, (const double*)W.beta, l0.ld, l0.nN, ncol,
+                                     (const int*)done));
+  }
+  HIPCK(hipGetLastError());
+  if (iters) *iters = std::min(it, maxit);
+  return SDC_OK;
+}
+
+}  // namespace
+
+// ============================================================================================
+// C ABI
+
+extern "C" {
+
+const char* sdc_version(void) { return "simpeg_dc 0.1 (gfx950)"; }
+
+const char* sdc_last_error(void) { return g_err.c_str(); }
+
+int sdc_default_params(sdc_params* p) {
+  if (!p) return fail(SDC_ERR_ARG, "null params");
+  p->nu_pre = 2;
+  p->nu_post = 2;
+  p->cheb_lmax = 2.0;
+  p->cheb_ratio = 4.0;
+  p->rhs_per_thread = 0;
+  p->check_every = 4;
+  return SDC_OK;
+}
+
+int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
+  if (!d || !out) return fail(SDC_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(SDC_ERR_ARG, "mesh needs >= 1 cell per axis");
+  if (!d->hx || !d->hy || !d->hz) return fail(SDC_ERR_ARG, "null width array");
+  if (d->bc_type != SDC_BC_ROBIN && d->bc_type != SDC_BC_NEUMANN)
+    return fail(SDC_ERR_ARG, "bc_type must be SDC_BC_ROBIN or SDC_BC_NEUMANN");
+  const uint64_t nN = (uint64_t)(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
+  if (nN >= (1ull << 31)) return fail(SDC_ERR_ARG, "mesh too large (nodes >= 2^31)");
+  for (int i = 0; i < d->nx; ++i)
+    if (!(d->hx[i] > 0)) return fail(SDC_ERR_ARG, "cell widths must be positive");
+  for (int i = 0; i < d->ny; ++i)
+    if (!(d->hy[i] > 0)) return fail(SDC_ERR_ARG, "cell widths must be positive");
+  for (int i = 0; i < d->nz; ++i)
+    if (!(d->hz[i] > 0)) return fail(SDC_ERR_ARG, "cell widths must be positive");
+  if (d->bc_type == SDC_BC_ROBIN &&
+      (!d->beta_xmin || !d->beta_xmax || !d->beta_ymin || !d->beta_ymax || !d->beta_zmin ||
+       !d->beta_zmax))
+    return fail(SDC_ERR_ARG, "Robin boundary needs the six beta arrays");
+  DeviceGuard dg(d->device);
+  sdc_handle_s* h = new sdc_handle_s();
+  h->device = d->device;
+  h->bc = d->bc_type;
+  h->nx = d->nx; h->ny = d->ny; h->nz = d->nz;
+  sdc_default_params(&h->prm);
+  auto bail = [&](int rc) {
+    sdc_destroy(h);
+    return rc;
+  };
+  std::vector<double> hx(d->hx, d->hx + d->nx), hy(d->hy, d->hy + d->ny), hz(d->hz, d->hz + d->nz);
+  int rc;
+  if ((rc = upload(&h->hx, hx)) || (rc = upload(&h->hy, hy)) || (rc = upload(&h->hz, hz)))
+    return bail(rc);
+  const size_t nyz = (size_t)d->ny * d->nz, nxz = (size_t)d->nx * d->nz, nxy = (size_t)d->nx * d->ny;
+  auto up_or_zero = [&](double** dst, const double* src, size_t n) {
+    std::vector<double> v(n, 0.0);
+    if (src && d->bc_type == SDC_BC_ROBIN) v.assign(src, src + n);
+    return upload(dst, v);
+  };
+  if ((rc = up_or_zero(&h->bxm, d->beta_xmin, nyz)) || (rc = up_or_zero(&h->bxp, d->beta_xmax, nyz)) ||
+      (rc = up_or_zero(&h->bym, d->beta_ymin, nxz)) || (rc = up_or_zero(&h->byp, d->beta_ymax, nxz)) ||
+      (rc = up_or_zero(&h->bzm, d->beta_zmin, nxy)) || (rc = up_or_zero(&h->bzp, d->beta_zmax, nxy)))
+    return bail(rc);
+  // multigrid hierarchy
+  Level l0;
+  const std::vector<double>* w[3] = {&hx, &hy, &hz};
+  for (int a = 0; a < 3; ++a) {
+    l0.ax[a].n = (int)w[a]->size();
+    l0.ax[a].X.assign(w[a]->size() + 1, 0.0);
+    for (size_t i = 0; i < w[a]->size(); ++i) l0.ax[a].X[i + 1] = l0.ax[a].X[i] + (*w[a])[i];
+  }
+  fill_level_dims(l0);
+  h->L.push_back(l0);
+  const uint32_t cmax = d->coarse_max > 0 ? (uint32_t)d->coarse_max : 512u;
+  while (h->L.back().nN > cmax && h->L.size() < 24) {
+    Level& f = h->L.back();
+    if (f.ax[0].n < 2 && f.ax[1].n < 2 && f.ax[2].n < 2) break;
+    Level c;
+    if ((rc = make_transfer(f, c))) return bail(rc);
+    h->L.push_back(c);
+  }
+  if (h->L.back().nN > 4096) return bail(fail(SDC_ERR_ARG, "coarsest multigrid level too large"));
+  for (auto& v : h->L)
+    if ((rc = alloc_level_arrays(v))) return bail(rc);
+  h->ncoarse = (int)h->L.back().nN;
+  const size_t nc = h->ncoarse;
+  if ((rc = dmalloc(&h->ainv64, nc * nc)) || (rc = dmalloc(&h->ainv32, nc * nc)) ||
+      (rc = dmalloc(&h->gj_a, nc * nc)) || (rc = dmalloc(&h->gj_f, nc)))
+    return bail(rc);
+  const size_t n0 = h->L[0].nN;
+  if ((rc = dmalloc(&h->tcx, n0)) || (rc = dmalloc(&h->tcy, n0)) || (rc = dmalloc(&h->tcz, n0)) ||
+      (rc = dmalloc(&h->tbd, n0)) || (rc = dmalloc(&h->tdg, n0)) || (rc = dmalloc(&h->tidg, n0)))
+    return bail(rc);
+  if (hipHostMalloc((void**)&h->h_flag, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    h->h_flag = nullptr;
+    return bail(fail(SDC_ERR_ALLOC, "hipHostMalloc failed"));
+  }
+  *out = h;
+  return SDC_OK;
+}
+
+int sdc_destroy(sdc_handle h) {
+  if (!h) return SDC_OK;
+  DeviceGuard dg(h->device);
+  (void)hipDeviceSynchronize();
+  dfree(h->hx); dfree(h->hy); dfree(h->hz);
+  dfree(h->bxm); dfree(h->bxp); dfree(h->bym); dfree(h->byp); dfree(h->bzm); dfree(h->bzp);
+  for (auto& v : h->L) v.release();
+  dfree(h->ainv64); dfree(h->ainv32); dfree(h->gj_a); dfree(h->gj_f);
+  dfree(h->tcx); dfree(h->tcy); dfree(h->tcz); dfree(h->tbd); dfree(h->tdg); dfree(h->tidg);
+  h->w32.release();
+  h->w64.release();
+  if (h->h_flag) (void)hipHostFree(h->h_flag);
+  delete h;
+  return SDC_OK;
+}
+
+int64_t sdc_ld(sdc_handle h) { return h ? (int64_t)h->L[0].ld : -1; }
+int64_t sdc_num_nodes(sdc_handle h) { return h ? (int64_t)h->L[0].nN : -1; }
+int64_t sdc_num_cells(sdc_handle h) { return h ? (int64_t)h->nx * h->ny * h->nz : -1; }
+int32_t sdc_num_levels(sdc_handle h) { return h ? (int32_t)h->L.size() : -1; }
+
+int sdc_set_params(sdc_handle h, const sdc_params* p) {
+  if (!h || !p) return fail(SDC_ERR_ARG, "null argument");
+  if (p->nu_pre < 1 || p->nu_pre > 15 || p->nu_post < 1 || p->nu_post > 15)
+    return fail(SDC_ERR_ARG, "smoothing steps must be in [1, 15]");
+  if (!(p->cheb_lmax > 0) || !(p->cheb_ratio > 1))
+    return fail(SDC_ERR_ARG, "cheb_lmax must be > 0 and cheb_ratio > 1");
+  if (p->rhs_per_thread != 0 && p->rhs_per_thread != 1 && p->rhs_per_thread != 2 &&
+      p->rhs_per_thread != 4 && p->rhs_per_thread != 8)
+    return fail(SDC_ERR_ARG, "rhs_per_thread must be 0, 1, 2, 4 or 8");
+  h->prm = *p;
+  if (h->prm.check_every < 1) h->prm.check_every = 1;
+  return SDC_OK;
+}
+
+int sdc_set_sigma(sdc_handle h, const double* sigma, void* stream) {
+  if (!h || !sigma) return fail(SDC_ERR_ARG, "null argument");
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  const Geo g = make_geo(h);
+  Level& l0 = h->L[0];
+  hipLaunchKernelGGL(k_coef_from_sigma, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, g, sigma,
+                     l0.cx64, l0.cy64, l0.cz64, l0.bd64, 1.0);
+  for (size_t l = 0; l + 1 < h->L.size(); ++l) {
+    Level& f = h->L[l];
+    Level& c = h->L[l + 1];
+    const Xfer<double> t = make_xfer<double>(h, (int)l);
+    hipLaunchKernelGGL(k_coarsen_coef, dim3(nblk(c.nN)), dim3(SDC_BLOCK), 0, st, t,
+                       (const double*)f.cx64, (const double*)f.cy64, (const double*)f.cz64,
+                       (const double*)f.bd64, c.cx64, c.cy64, c.cz64, c.bd64, c.n1z);
+  }
+  const int pin = h->bc == SDC_BC_NEUMANN ? 1 : 0;
+  for (auto& v : h->L) {
+    hipLaunchKernelGGL(k_coef_diag, dim3(nblk(v.nN)), dim3(SDC_BLOCK), 0, st, v.n1x, v.n1y, v.nN,
+                       make_fastdiv(v.n1x), make_fastdiv(v.n1y), (const double*)v.cx64,
+                       (const double*)v.cy64, (const double*)v.cz64, (const double*)v.bd64,
+                       v.dg64, v.idg64, pin);
+    const dim3 gg(nblk(v.nN));
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cx64, v.cx32, (size_t)v.nN);
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cy64, v.cy32, (size_t)v.nN);
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cz64, v.cz32, (size_t)v.nN);
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.dg64, v.dg32, (size_t)v.nN);
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.idg64, v.idg32, (size_t)v.nN);
+  }
+  // coarsest level: dense inverse
+  const int lc = (int)h->L.size() - 1;
+  const Sten<double> sc = make_sten<double>(h, lc, 1);
+  const size_t nc = h->ncoarse;
+  hipLaunchKernelGGL(k_dense_build, dim3(nblk(nc)), dim3(SDC_BLOCK), 0, st, sc, h->gj_a);
+  hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), 0, st, h->gj_a, h->ainv64, h->gj_f,
+                     (int)nc);
+  hipLaunchKernelGGL(k_symmetrize, dim3(nblk(nc * nc)), dim3(SDC_BLOCK), 0, st, h->ainv64,
+                     h->ainv32, (int)nc);
+  HIPCK(hipGetLastError());
+  h->sigma_set = true;
+  return SDC_OK;
+}
+
+int sdc_apply(sdc_handle h, int dtype, const void* x, void* y, int ncol, void* stream) {
+  if (!h || !x || !y || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
+  if (!h->sigma_set) return fail(SDC_ERR_STATE, "sdc_set_sigma has not been called");
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  const Level& l0 = h->L[0];
+  const int R = pick_r(h, ncol);
+  const dim3 g = grid2(l0.nN, ncol, R);
+  if (dtype == SDC_F64) {
+    const Sten<double> s = make_sten<double>(h, 0, 2);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_apply<double, R, false>), g, dim3(SDC_BLOCK), 0, st, s,
+                                     (const double*)x, (double*)y, l0.ld, ncol, (double*)nullptr,
+                                     (const int*)nullptr));
+  } else if (dtype == SDC_F32) {
+    const Sten<float> s = make_sten<float>(h, 0, 2);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_apply<float, R, false>), g, dim3(SDC_BLOCK), 0, st, s,
+                                     (const float*)x, (float*)y, l0.ld, ncol, (double*)nullptr,
+                                     (const int*)nullptr));
+  } else {
+    return fail(SDC_ERR_ARG, "dtype must be SDC_F32 or SDC_F64");
+  }
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+int sdc_vcycle(sdc_handle h, int dtype, const void* b, void* z, int ncol, void* stream) {
+  if (!h || !b || !z || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
+  if (!h->sigma_set) return fail(SDC_ERR_STATE, "sdc_set_sigma has not been called");
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == SDC_F64) {
+    CK(ensure_work<double>(h, ncol));
+    CK(vcycle<double>(h, 0, (const double*)b, (double*)z, ncol, false, nullptr, st));
+  } else if (dtype == SDC_F32) {
+    CK(ensure_work<float>(h, ncol));
+    CK(vcycle<float>(h, 0, (const float*)b, (float*)z, ncol, false, nullptr, st));
+  } else {
+    return fail(SDC_ERR_ARG, "dtype must be SDC_F32 or SDC_F64");
+  }
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solve_opts* o,
+              sdc_solve_stats* stats, void* stream) {
+  if (!h || !q || !u || !o || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
+  if (!h->sigma_set) return fail(SDC_ERR_STATE, "sdc_set_sigma has not been called");
+  if (o->mode != SDC_MODE_MIXED && o->mode != SDC_MODE_FP64)
+    return fail(SDC_ERR_ARG, "mode must be SDC_MODE_MIXED or SDC_MODE_FP64");
+  if (!(o->tol > 0) || o->max_iter < 1) return fail(SDC_ERR_ARG, "tol must be > 0, max_iter >= 1");
+  if (o->mode == SDC_MODE_MIXED && (!(o->inner_tol > 0) || o->max_outer < 1))
+    return fail(SDC_ERR_ARG, "mixed mode needs inner_tol > 0 and max_outer >= 1");
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  const Level& l0 = h->L[0];
+  const int R = pick_r(h, ncol);
+  const dim3 g = grid2(l0.nN, ncol, R);
+  const dim3 blk(SDC_BLOCK);
+  const uint32_t nbx = nblk(l0.nN);
+  const size_t len = l0.ld * (size_t)ncol;
+  PinFix pf;
+  pf.on = h->bc == SDC_BC_NEUMANN;
+  pf.n1x = l0.n1x;
+  pf.plane = l0.plane;
+  pf.cx = l0.cx64;
+  pf.cy = l0.cy64;
+  pf.cz = l0.cz64;
+  const Sten<double> s2 = make_sten<double>(h, 0, 2);
+  sdc_solve_stats S;
+  S.outer = 0;
+  S.iters = 0;
+  S.max_rel_res = 0.0;
+  S.converged = 0;
+  std::vector<double> h_rr(ncol), h_qn(ncol);
+  auto rel_max = [&](const double* d_rr, const double* d_qn) -> int {
+    HIPCK(hipMemcpyAsync(h_rr.data(), d_rr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(h_qn.data(), d_qn, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    double m = 0.0;
+    for (int c = 0; c < ncol; ++c)
+      if (h_qn[c] > 0) m = std::max(m, std::sqrt(h_rr[c] / h_qn[c]));
+    S.max_rel_res = m;
+    return SDC_OK;
+  };
+  if (o->mode == SDC_MODE_FP64) {
+    CK(ensure_work<double>(h, ncol));
+    Work<double>& W = h->w64;
+    DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, true>), g, blk, 0, st, s2, q,
+                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
+                       W.active, 0);
+    int it = 0;
+    CK(pcg<double>(h, u, ncol, o->tol, o->max_iter, nullptr, &it, st));
+    S.iters = it;
+    S.outer = 1;
+    DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
+                                     (const double*)u, W.z, l0.ld, ncol, pf, W.partial));
+    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
+                       W.active, 0);
+    CK(rel_max(W.orr, W.aux));
+    // converged when every column met the PCG test (recursive residual)
+    HIPCK(hipMemcpyAsync(h->h_flag, W.counters + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    S.converged = (h->h_flag[0] == 0) ? 1 : 0;
+  } else {
+    CK(ensure_work<float>(h, ncol));
+    Work<float>& W = h->w32;
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
+                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
+                       W.active, 0);
+    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
+                       W.active, 0);
+    const double tol2 = o->tol * o->tol;
+    for (int outer = 0;; ++outer) {
+      if (outer > 0) {
+        DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, false>), g, blk, 0, st, s2, q,
+                                         (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+        hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
+                           W.orr, W.active, 0);
+      }
+      hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.ocount, (size_t)1);
+      hipLaunchKernelGGL(k_outer_check, dim3(nblk(ncol)), blk, 0, st, (const double*)W.orr,
+                         (const double*)W.aux, tol2, W.omask, W.ocount, ncol);
+      HIPCK(hipMemcpyAsync(h->h_flag, W.ocount, sizeof(int), hipMemcpyDeviceToHost, st));
+      CK(rel_max(W.orr, W.aux));
+      const int open = h->h_flag[0];
+      S.outer = outer;
+      if (open == 0) {
+        S.converged = 1;
+        break;
+      }
+      if (outer == o->max_outer) break;
+      int it = 0;
+      CK(pcg<float>(h, W.x, ncol, o->inner_tol, o->max_iter, W.omask, &it, st));
+      S.iters += it;
+      DISPATCH_R(R, hipLaunchKernelGGL((k_ir_update<R>), g, blk, 0, st, u, (const float*)W.x,
+                                       l0.ld, l0.nN, ncol));
+    }
+  }
+  HIPCK(hipGetLastError());
+  if (stats) *stats = S;
+  if (!S.converged) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "solver stopped at relative residual %.3e (tol %.3e)",
+                  S.max_rel_res, o->tol);
+    return fail(SDC_ERR_NOCONV, buf);
+  }
+  return SDC_OK;
+}
+
+int sdc_jvec_rhs(sdc_handle h, const double* dsig, const double* u, double* out, int ncol,
+                 void* stream) {
+  if (!h || !dsig || !u || !out || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  const Level& l0 = h->L[0];
+  const Geo geo = make_geo(h);
+  hipLaunchKernelGGL(k_coef_from_sigma, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, geo, dsig,
+                     h->tcx, h->tcy, h->tcz, h->tbd, -1.0);
+  hipLaunchKernelGGL(k_coef_diag, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, l0.n1x, l0.n1y,
+                     l0.nN, make_fastdiv(l0.n1x), make_fastdiv(l0.n1y), (const double*)h->tcx,
+                     (const double*)h->tcy, (const double*)h->tcz, (const double*)h->tbd, h->tdg,
+                     h->tidg, 0);
+  Sten<double> s;
+  s.cx = h->tcx; s.cy = h->tcy; s.cz = h->tcz; s.dg = h->tdg; s.idg = h->tidg;
+  s.n1x = l0.n1x; s.n1y = l0.n1y; s.n1z = l0.n1z; s.nN = l0.nN; s.plane = l0.plane;
+  s.fx = make_fastdiv(l0.n1x); s.fy = make_fastdiv(l0.n1y);
+  s.pin0 = 0;
+  const int R = pick_r(h, ncol);
+  DISPATCH_R(R, hipLaunchKernelGGL((k_apply<double, R, false>), grid2(l0.nN, ncol, R),
+                                   dim3(SDC_BLOCK), 0, st, s, u, out, l0.ld, ncol,
+                                   (double*)nullptr, (const int*)nullptr));
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+int sdc_jtvec_contract(sdc_handle h, const double* u, const double* lam, int ncol,
+                       const int32_t* ucol, int per_column, double* out, int64_t ldo,
+                       void* stream) {
+  if (!h || !u || !lam || !out || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
+  const int64_t nC = (int64_t)h->nx * h->ny * h->nz;
+  if (per_column && ldo < nC) return fail(SDC_ERR_ARG, "ldo must be >= number of cells");
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  const Geo geo = make_geo(h);
+  const Level& l0 = h->L[0];
+  if (per_column)
+    hipLaunchKernelGGL(k_jtvec<true>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
+                       ncol, (const int*)ucol, out, (size_t)ldo);
+  else
+    hipLaunchKernelGGL(k_jtvec<false>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
+                       ncol, (const int*)ucol, out, (size_t)ldo);
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+int sdc_gather(sdc_handle h, int nrow, const int32_t* rowptr, const int64_t* idx, const double* w,
+               const double* u, double* d, void* stream) {
+  if (!h || nrow < 0) return fail(SDC_ERR_ARG, "bad argument");
+  if (nrow == 0) return SDC_OK;
+  if (!rowptr || !idx || !w || !u || !d) return fail(SDC_ERR_ARG, "null pointer");
+  DeviceGuard dg(h->device);
+  hipLaunchKernelGGL(k_gather, dim3(nblk(nrow)), dim3(SDC_BLOCK), 0, (hipStream_t)stream, nrow,
+                     (const int*)rowptr, idx, w, u, d);
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+int sdc_scatter(sdc_handle h, int nrow, const int64_t* dst, const int32_t* rowptr,
+                const int32_t* src, const double* w, const double* v, double* out, int ncol,
+                void* stream) {
+  if (!h || nrow < 0 || ncol < 0 || !out) return fail(SDC_ERR_ARG, "bad argument");
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t len = h->L[0].ld * (size_t)ncol;
+  if (len) hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), dim3(SDC_BLOCK), 0, st, out, len);
+  if (nrow > 0) {
+    if (!dst || !rowptr || !src || !w || !v) return fail(SDC_ERR_ARG, "null pointer");
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(nrow)), dim3(SDC_BLOCK), 0, st, nrow, dst,
+                       (const int*)rowptr, (const int*)src, w, v, out);
+  }
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+"""Model -> physical property maps used by the nodal DC simulation.
+
+Same call surface as ``simpeg.maps``: ``map * m`` (or ``map @ m``) evaluates the map,
+``map.deriv(m)`` returns the (sparse) derivative and ``map.deriv(m, v)`` its product
+with ``v``; ``nP`` is the number of model parameters.  Reference: ``IdentityMap``
+(maps/_base.py:26-256), ``ExpMap`` (maps/_property_maps.py:18-112),
+``ReciprocalMap`` (maps/_property_maps.py), ``ComboMap`` (maps/_base.py:357-548).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+
+class IdentityMap:
+    def __init__(self, mesh=None, nP=None, **kwargs):
+        if nP is None and mesh is not None:
+            nP = mesh.n_cells if hasattr(mesh, "n_cells") else mesh.nC
+        self._nP = nP
+
+    @property
+    def nP(self):
+        return "*" if self._nP is None else int(self._nP)
+
+    @property
+    def shape(self):
+        return (self.nP, self.nP)
+
+    @property
+    def is_linear(self):
+        return True
+
+    def _transform(self, m):
+        return np.asarray(m, dtype=float).ravel()
+
+    def deriv(self, m, v=None):
+        d = sp.identity(np.size(m), format="csr")
+        return d if v is None else d @ v
+
+    def __mul__(self, val):
+        if isinstance(val, (IdentityMap, ComboMap)):
+            return ComboMap([self, val])
+        return self._transform(val)
+
+    __matmul__ = __mul__
+
+    def inverse(self, D):
+        return np.asarray(D, dtype=float).ravel()
+
+
+class ExpMap(IdentityMap):
+    """sigma = exp(m); derivative diag(exp(m))."""
+
+    def _transform(self, m):
+        return np.exp(np.asarray(m, dtype=float).ravel())
+
+    def inverse(self, D):
+        return np.log(np.asarray(D, dtype=float).ravel())
+
+    def deriv(self, m, v=None):
+        d = sp.diags(np.exp(np.asarray(m, dtype=float).ravel()), format="csr")
+        return d if v is None else d @ v
+
+    @property
+    def is_linear(self):
+        return False
+
+
+class ReciprocalMap(IdentityMap):
+    """sigma = 1/m; derivative diag(-1/m^2)."""
+
+    def _transform(self, m):
+        return 1.0 / np.asarray(m, dtype=float).ravel()
+
+    def inverse(self, D):
+        return 1.0 / np.asarray(D, dtype=float).ravel()
+
+    def deriv(self, m, v=None):
+        m = np.asarray(m, dtype=float).ravel()
+        d = sp.diags(-1.0 / m ** 2, format="csr")
+        return d if v is None else d @ v
+
+    @property
+    def is_linear(self):
+        return False
+
+
+class ComboMap(IdentityMap):
+    """Composition maps[0] * maps[1] * ... * m (right-most applied first)."""
+
+    def __init__(self, maps, **kwargs):
+        super().__init__(nP=None)
+        self.maps = []
+        for m in maps:
+            if isinstance(m, ComboMap):
+                self.maps.extend(m.maps)
+            else:
+                self.maps.append(m)
+
+    @property
+    def nP(self):
+        return self.maps[-1].nP
+
+    def _transform(self, m):
+        for mp in reversed(self.maps):
+            m = mp * m
+        return m
+
+    def deriv(self, m, v=None):
+        deriv = None
+        mi = m
+        for mp in reversed(self.maps):
+            d = mp.deriv(mi)
+            deriv = d if deriv is None else d @ deriv
+            mi = mp * mi
+        return deriv if v is None else deriv @ v
+
+    @property
+    def is_linear(self):
+        return all(getattr(m, "is_linear", False) for m in self.maps)

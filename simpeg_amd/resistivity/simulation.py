@@ -1,0 +1,452 @@
+"""GPU ``Simulation3DNodal``: drop-in for
+``simpeg.electromagnetics.static.resistivity.Simulation3DNodal``.
+
+Reference (paths relative to the simpeg repository):
+  BaseDCSimulation          electromagnetics/static/resistivity/simulation.py:20-318
+  Simulation3DNodal         electromagnetics/static/resistivity/simulation.py:483-670
+  BaseSimulation.dpred      simulation.py:161-196; residual 336-363; make_synthetic_data 365-433
+  model / sigma properties  props.py:117-225, 408-477
+
+The reference factorises the assembled A = G^T M_e(sigma) G + diag(AvgBC sigma) and runs one
+triangular solve pair per right-hand side.  Here every solve is a batched (all sources at once)
+multigrid-preconditioned CG on the GPU with a matrix-free stencil (libsimpeg_dc):
+
+* ``fields(m)``: one batched solve over the sources (fp32 PCG inside fp64 refinement by default,
+  or plain fp64 PCG with ``precision="fp64"``).
+* ``Jvec``: right-hand sides -A(sigmaDeriv v) u_s built by the stencil kernel, one batched solve.
+* ``Jtvec``: P^T v summed per source (A is symmetric, so the reference's one adjoint solve per
+  (source, receiver) pair collapses to one per source), one batched solve, then the
+  edge-gradient contraction kernel sums d(u_s^T A lam_s)/d sigma over sources.
+* ``getJ``: one adjoint column per datum, per-column contraction.
+
+Results are host numpy arrays, as in the reference.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from .. import maps as _maps
+from ..data import Data, SyntheticData
+from ..engine import NodalEngine, coo_to_gather, coo_to_scatter
+from ..mesh import edge_mass, mesh_widths, node_interpolation, robin_diagonal
+from .fields import Fields3DNodal
+from .survey import Survey
+
+
+class Simulation3DNodal:
+    """3D nodal DC problem solved on an AMD Instinct GPU.
+
+    Parameters follow the reference (``mesh, survey, bc_type, sigma, sigmaMap, rho, rhoMap,
+    solver, solver_opts, storeJ, miniaturize, surface_faces, sensitivity_path, counter,
+    verbose``).  ``solver``/``solver_opts`` are accepted for signature compatibility; the GPU
+    solver replaces the direct factorisation they configure.  GPU-specific keywords:
+
+    precision : {"mixed", "fp64"}
+        "mixed": fp32 multigrid PCG wrapped in fp64 iterative refinement (default);
+        "fp64": plain fp64 multigrid PCG.
+    rtol : float
+        per-source target of |q - A u| / |q| (fp64 residual).
+    inner_rtol, max_outer, max_iter : refinement / PCG limits.
+    gpu_params : dict of multigrid parameters (nu_pre, nu_post, cheb_ratio, ...).
+    """
+
+    _solutionType = "phiSolution"
+    _formulation = "EB"
+    fieldsPair = Fields3DNodal
+
+    def __init__(self, mesh, survey=None, bc_type="Robin", sigma=None, sigmaMap=None, rho=None,
+                 rhoMap=None, solver=None, solver_opts=None, storeJ=False, miniaturize=False,
+                 surface_faces=None, sensitivity_path=None, counter=None, verbose=False,
+                 model=None, *, precision="mixed", rtol=1e-10, inner_rtol=1e-5, max_outer=10,
+                 max_iter=1000, device=None, gpu_params=None, coarse_max=0, jchunk=512):
+        self.mesh = mesh
+        self._widths, self._origin = mesh_widths(mesh)
+        self._nC = int(np.prod([len(h) for h in self._widths]))
+        self._nN = int(np.prod([len(h) + 1 for h in self._widths]))
+        self._survey = None
+        self.survey = survey
+        if bc_type not in ("Neumann", "Robin", "Mixed"):
+            if bc_type == "Dirichlet":
+                raise ValueError("Dirichlet conditions are not supported in the Nodal formulation")
+            raise ValueError(f"bc_type must be 'Neumann', 'Robin' or 'Mixed', got {bc_type!r}")
+        self._bc_type = bc_type
+        if not isinstance(storeJ, bool):
+            raise TypeError("storeJ must be a bool")
+        self.storeJ = storeJ
+        if not isinstance(miniaturize, bool):
+            raise TypeError("miniaturize must be a bool")
+        self.miniaturize = miniaturize
+        if miniaturize:
+            raise NotImplementedError("miniaturize=True is not supported by the GPU simulation yet")
+        self.surface_faces = surface_faces
+        self.solver = solver
+        self.solver_opts = {} if solver_opts is None else dict(solver_opts)
+        self.sensitivity_path = sensitivity_path
+        self.counter = counter
+        self.verbose = verbose
+        if sigma is not None and rho is not None:
+            raise ValueError("give sigma or rho, not both")
+        if sigmaMap is not None and rhoMap is not None:
+            raise ValueError("give sigmaMap or rhoMap, not both")
+        self.sigmaMap = sigmaMap
+        self.rhoMap = rhoMap
+        self._sigma_fixed = None if sigma is None else np.asarray(sigma, dtype=float).ravel()
+        self._rho_fixed = None if rho is None else np.asarray(rho, dtype=float).ravel()
+        if precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        self.precision = precision
+        self.rtol = float(rtol)
+        self.inner_rtol = float(inner_rtol)
+        self.max_outer = int(max_outer)
+        self.max_iter = int(max_iter)
+        self.jchunk = int(jchunk)
+        self._device = device
+        self._gpu_params = dict(gpu_params or {})
+        self._coarse_max = int(coarse_max)
+        self._engine = None
+        self._model = None
+        self._sigma_on_gpu = None
+        self._Jmatrix = None
+        self._gtgdiag = None
+        self._proj = None
+        self._q = None
+        self._q_block = None
+        self.last_solve = None
+        if model is not None:
+            self.model = model
+
+    # ------------------------------------------------------------------ properties
+    @property
+    def survey(self):
+        if self._survey is None:
+            raise AttributeError("Simulation must have a survey")
+        return self._survey
+
+    @survey.setter
+    def survey(self, value):
+        if value is not None and not isinstance(value, Survey) and not hasattr(value, "source_list"):
+            raise TypeError("survey must be a DC Survey")
+        self._survey = value
+        self._proj = None
+        self._q = None
+        self._q_block = None
+        self._Jmatrix = None
+        self._gtgdiag = None
+
+    @property
+    def bc_type(self):
+        return self._bc_type
+
+    @property
+    def _active_map(self):
+        return self.sigmaMap if self.sigmaMap is not None else self.rhoMap
+
+    @property
+    def model(self):
+        return self._model
+
+    @model.setter
+    def model(self, value):
+        if value is None:
+            self._model = None
+            return
+        value = np.asarray(value, dtype=float).ravel()
+        mp = self._active_map
+        if mp is not None:
+            nP = mp.nP
+            if nP != "*" and value.size != nP:
+                raise ValueError(
+                    f"Model is the wrong size: {value.size}, expected {nP} parameters")
+        if self._model is None or self._model.shape != value.shape or not np.allclose(
+                self._model, value):
+            self._sigma_on_gpu = None
+            self._Jmatrix = None
+            self._gtgdiag = None
+        self._model = value.copy()
+
+    @property
+    def sigma(self):
+        if self.sigmaMap is not None:
+            if self._model is None:
+                raise AttributeError("a model is needed to evaluate sigmaMap")
+            return np.asarray(self.sigmaMap * self._model, dtype=float).ravel()
+        if self.rhoMap is not None:
+            if self._model is None:
+                raise AttributeError("a model is needed to evaluate rhoMap")
+            return 1.0 / np.asarray(self.rhoMap * self._model, dtype=float).ravel()
+        if self._sigma_fixed is not None:
+            return self._sigma_fixed
+        if self._rho_fixed is not None:
+            return 1.0 / self._rho_fixed
+        raise AttributeError("sigma is not set: give sigma, rho, sigmaMap or rhoMap")
+
+    @property
+    def rho(self):
+        return 1.0 / self.sigma
+
+    @property
+    def sigmaDeriv(self):
+        if self.sigmaMap is not None:
+            return sp.csr_matrix(self.sigmaMap.deriv(self._model))
+        if self.rhoMap is not None:
+            s = self.sigma
+            return sp.csr_matrix(sp.diags(-s * s) @ self.rhoMap.deriv(self._model))
+        raise AttributeError("sigmaDeriv needs sigmaMap or rhoMap")
+
+    # ------------------------------------------------------------------ GPU state
+    @property
+    def engine(self):
+        if self._engine is None:
+            self._engine = NodalEngine(
+                self._widths, self._origin,
+                bc_type="Neumann" if self._bc_type == "Neumann" else "Robin",
+                surface_faces=self.surface_faces, device=self._device,
+                coarse_max=self._coarse_max, params=self._gpu_params)
+        return self._engine
+
+    def _ensure_sigma(self):
+        if self._sigma_on_gpu is None or self.sigmaMap is None and self.rhoMap is None:
+            sig = self.sigma
+            if sig.size != self._nC:
+                raise NotImplementedError(
+                    "Anisotropic conductivity is not supported for Robin boundary "
+                    "conditions, please use 'Neumann'." if sig.size != self._nC else "")
+            self.engine.set_sigma(sig)
+            self._sigma_on_gpu = True
+
+    def _solve(self, rhs, ncol):
+        u, res = self.engine.solve(rhs, ncol, mode=self.precision, tol=self.rtol,
+                                   inner_tol=self.inner_rtol, max_outer=self.max_outer,
+                                   max_iter=self.max_iter)
+        self.last_solve = res
+        return u
+
+    # ------------------------------------------------------------------ survey -> sparse maps
+    def _projection(self):
+        """Receiver projections (all data) and source right-hand sides as sparse COO."""
+        if self._proj is not None:
+            return self._proj
+        eng = self.engine
+        ld = eng.ld
+        rows, flat, w, nodes, dsrc = [], [], [], [], []
+        i0 = 0
+        for s, src in enumerate(self.survey.source_list):
+            for rx in src.receiver_list:
+                nd = rx.nD
+                factor = np.ones(nd)
+                if rx.data_type == "apparent_resistivity":
+                    try:
+                        factor = 1.0 / np.asarray(rx.geometric_factor[src], dtype=float)
+                    except KeyError:
+                        raise KeyError(
+                            "Receiver geometric factor has not been set, please execute "
+                            "survey.set_geometric_factor()")
+                for locs, sign in rx._electrode_sets():
+                    idx, wt = node_interpolation(self._widths, self._origin, locs)
+                    rows.append(np.repeat(i0 + np.arange(nd), 8))
+                    nodes.append(idx.ravel())
+                    flat.append(s * ld + idx.ravel())
+                    w.append((sign * factor[:, None] * wt).ravel())
+                dsrc.append(np.full(nd, s, dtype=np.int64))
+                i0 += nd
+        nD = i0
+        rows = np.concatenate(rows) if rows else np.zeros(0, np.int64)
+        flat = np.concatenate(flat) if flat else np.zeros(0, np.int64)
+        nodes = np.concatenate(nodes) if nodes else np.zeros(0, np.int64)
+        w = np.concatenate(w) if w else np.zeros(0)
+        keep = w != 0.0
+        rows, flat, nodes, w = rows[keep], flat[keep], nodes[keep], w[keep]
+        data_src = np.concatenate(dsrc) if dsrc else np.zeros(0, np.int64)
+        rp, idx, ww = coo_to_gather(rows, flat, w, nD)
+        gather = eng.gather_csr(rp, idx, ww)
+        dst, srp, ssrc, sw = coo_to_scatter(flat, rows, w)
+        scatter = eng.scatter_csr(dst, srp, ssrc, sw)
+        self._proj = dict(nD=nD, rows=rows, nodes=nodes, w=w, data_src=data_src,
+                          gather=gather, scatter=scatter)
+        return self._proj
+
+    def getSourceTerm(self):
+        """q (n_nodes, nSrc), Fortran ordered (resistivity/simulation.py:265-289)."""
+        if self._q is None:
+            srcs = self.survey.source_list
+            q = np.zeros((self._nN, len(srcs)), order="F")
+            for i, src in enumerate(srcs):
+                q[:, i] = src.eval(self)
+            self._q = q
+        return self._q
+
+    getRHS = getSourceTerm
+
+    def _source_block(self):
+        if self._q_block is None:
+            eng = self.engine
+            dest, col, val = [], [], []
+            for i, src in enumerate(self.survey.source_list):
+                locs, cur = src.electrodes()
+                idx, wt = node_interpolation(self._widths, self._origin, locs)
+                dest.append(i * eng.ld + idx.ravel())
+                val.append((cur[:, None] * wt).ravel())
+            dest = np.concatenate(dest)
+            val = np.concatenate(val)
+            keep = val != 0.0
+            dest, val = dest[keep], val[keep]
+            d, rp, sidx, sw = coo_to_scatter(dest, np.zeros(dest.size, np.int64), val)
+            sc = eng.scatter_csr(d, rp, sidx, sw)
+            self._q_block = eng.scatter(sc, np.ones(1), self.survey.nSrc)
+        return self._q_block
+
+    # ------------------------------------------------------------------ public API
+    def fields(self, m=None, calcJ=True):
+        """Nodal potentials of every source (``BaseDCSimulation.fields``, simulation.py:100-113)."""
+        if m is not None:
+            self.model = m
+        self._ensure_sigma()
+        q = self._source_block()
+        u = self._solve(q, self.survey.nSrc)
+        return Fields3DNodal(self, u)
+
+    def dpred(self, m=None, f=None):
+        """Predicted data (``BaseSimulation.dpred``, simulation.py:161-196)."""
+        if self._survey is None:
+            raise AttributeError(
+                "The survey has not yet been set and is required to compute "
+                "data. Please set the survey for the simulation: "
+                "simulation.survey = survey")
+        if f is None:
+            if m is None:
+                m = self.model
+            f = self.fields(m)
+        pr = self._projection()
+        return self.engine.gather(pr["gather"], f.u_block).cpu().numpy()
+
+    def residual(self, m, dobs, f=None):
+        return self.dpred(m, f=f) - np.asarray(dobs, dtype=float).ravel()
+
+    def Jvec(self, m, v, f=None):
+        """J v (``BaseDCSimulation.Jvec``, simulation.py:156-187)."""
+        if f is None:
+            f = self.fields(m)
+        self.model = m
+        if self.storeJ:
+            return self.getJ(m, f=f) @ np.asarray(v, dtype=float)
+        self._ensure_sigma()
+        dsig = self.sigmaDeriv @ np.asarray(v, dtype=float).ravel()
+        eng = self.engine
+        nS = self.survey.nSrc
+        rhs = eng.jvec_rhs(dsig, f.u_block, nS)
+        du = self._solve(rhs, nS)
+        return eng.gather(self._projection()["gather"], du).cpu().numpy()
+
+    def Jtvec(self, m, v, f=None):
+        """J^T v (``BaseDCSimulation.Jtvec``/``_Jtvec``, simulation.py:189-263)."""
+        if f is None:
+            f = self.fields(m)
+        self.model = m
+        if isinstance(v, Data):
+            v = v.dobs
+        v = np.asarray(v, dtype=float).ravel()
+        if self.storeJ:
+            return np.asarray(self.getJ(m, f=f).T @ v)
+        self._ensure_sigma()
+        eng = self.engine
+        nS = self.survey.nSrc
+        pr = self._projection()
+        if v.size != pr["nD"]:
+            raise ValueError(f"v must have {pr['nD']} entries, got {v.size}")
+        rhs = eng.scatter(pr["scatter"], v, nS)
+        lam = self._solve(rhs, nS)
+        g = eng.contract(f.u_block, lam, nS).cpu().numpy()
+        return -(self.sigmaDeriv.T @ g)
+
+    def getJ(self, m, f=None):
+        """Dense sensitivity (nD, nP) (``getJ``, simulation.py:115-121), one adjoint column per datum."""
+        self.model = m
+        if self._Jmatrix is None:
+            if f is None:
+                f = self.fields(m)
+            self._ensure_sigma()
+            eng = self.engine
+            pr = self._projection()
+            nD = pr["nD"]
+            sd = self.sigmaDeriv
+            J = np.empty((nD, sd.shape[1]))
+            rows, nodes, w = pr["rows"], pr["nodes"], pr["w"]
+            for c0 in range(0, nD, self.jchunk):
+                c1 = min(nD, c0 + self.jchunk)
+                k = c1 - c0
+                sel = (rows >= c0) & (rows < c1)
+                dst = (rows[sel] - c0) * eng.ld + nodes[sel]
+                d, rp, sidx, sw = coo_to_scatter(dst, rows[sel] - c0, w[sel])
+                rhs = eng.scatter(eng.scatter_csr(d, rp, sidx, sw), np.ones(k), k)
+                lam = self._solve(rhs, k)
+                G = eng.contract(f.u_block, lam, k, ucol=pr["data_src"][c0:c1], per_column=True)
+                J[c0:c1] = -(sd.T @ G.cpu().numpy().T).T
+            self._Jmatrix = J
+        return self._Jmatrix
+
+    def getJtJdiag(self, m, W=None, f=None):
+        """diag(J^T W^T W J) (``getJtJdiag``, simulation.py:137-154)."""
+        if self._gtgdiag is None:
+            J = self.getJ(m, f=f)
+            w = np.ones(J.shape[0]) if W is None else np.asarray(W.diagonal()) ** 2
+            self._gtgdiag = (w[:, None] * J * J).sum(axis=0)
+        return self._gtgdiag
+
+    def make_synthetic_data(self, m, relative_error=0.05, noise_floor=0.0, f=None,
+                            add_noise=False, random_seed=None, **kwargs):
+        if kwargs.pop("std", None) is not None:
+            raise TypeError("The std parameter has been removed. Please use relative_error.")
+        if f is None:
+            f = self.fields(m)
+        dclean = self.dpred(m, f=f)
+        if add_noise:
+            rng = np.random.default_rng(seed=random_seed)
+            std = np.sqrt((relative_error * np.abs(dclean)) ** 2 + noise_floor ** 2)
+            dobs = dclean + rng.normal(loc=0, scale=std, size=dclean.shape)
+        else:
+            dobs = dclean
+        return SyntheticData(survey=self.survey, dobs=dobs, dclean=dclean,
+                             relative_error=relative_error, noise_floor=noise_floor)
+
+    # ------------------------------------------------------------------ operators on the host
+    def _edge_mass(self, sigma):
+        return edge_mass(self._widths, sigma)
+
+    @property
+    def MeSigma(self):
+        return sp.diags(self._edge_mass(self.sigma), format="csr")
+
+    def getA(self, resistivity=None):
+        """Assembled A (CSR), for inspection; the GPU path never assembles it."""
+        sig = self.sigma if resistivity is None else 1.0 / np.asarray(resistivity, dtype=float)
+        G = self.mesh.nodal_gradient if hasattr(self.mesh, "nodal_gradient") else None
+        if G is None:
+            raise AttributeError("the mesh provides no nodal_gradient")
+        A = (G.T @ sp.diags(self._edge_mass(sig)) @ G).tocsr()
+        if self._bc_type == "Neumann":
+            A = A.tolil()
+            A[0, :] = 0.0
+            A[0, 0] = 1.0
+            return A.tocsr()
+        from ..mesh import robin_face_weights
+        beta = robin_face_weights(self._widths, self._origin, self.surface_faces)
+        return (A + sp.diags(robin_diagonal(self._widths, beta, sig))).tocsr()
+
+    def getADeriv(self, u, v, adjoint=False):
+        """dA/dm products (resistivity/simulation.py:557-571), evaluated on the GPU."""
+        self._ensure_sigma()
+        eng = self.engine
+        ub = eng.to_block(np.asarray(u, dtype=float))
+        if not adjoint:
+            dsig = self.sigmaDeriv @ np.asarray(v, dtype=float).ravel()
+            out = eng.jvec_rhs(dsig, ub, ub.shape[0])
+            return -eng.from_block(out)[:, 0] if ub.shape[0] == 1 else -eng.from_block(out)
+        lb = eng.to_block(np.asarray(v, dtype=float))
+        g = eng.contract(ub, lb, lb.shape[0]).cpu().numpy()
+        return self.sigmaDeriv.T @ g
+
+    def getRHSDeriv(self, source, v, adjoint=False):
+        return 0.0

@@ -1,0 +1,160 @@
+"""CPU tests of the product's host-side logic against the oracle: mesh widths and ordering,
+Robin boundary weights, source term, receiver projection, survey ordering, host operator."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import OracleMesh
+from oracle.dc_nodal import robin_boundary_matrix
+from simpeg_amd import maps, workloads
+from simpeg_amd.mesh import (TensorMesh, edge_mass, node_interpolation, robin_diagonal,
+                             robin_face_weights, unpack_widths)
+from simpeg_amd.resistivity import receivers as Rx
+from simpeg_amd.resistivity import sources as Src
+from simpeg_amd.resistivity.survey import Survey
+from tests._helpers import rel, tiny_problem
+
+
+def test_unpack_widths_and_origin():
+    h = unpack_widths([(10.0, 3, -1.5), (10.0, 2), 5.0, (10.0, 2, 2.0)])
+    assert np.allclose(h, [33.75, 22.5, 15.0, 10, 10, 5, 20, 40])
+    m = TensorMesh([[(1.0, 4)], [(2.0, 2)], [(1.0, 3)]], origin="CCN")
+    assert np.allclose(m.origin, [-2.0, -2.0, -3.0])
+    assert m.shape_cells == (4, 2, 3) and m.n_nodes == 5 * 3 * 4
+    om = OracleMesh(m.h, origin=m.origin)
+    assert np.allclose(m.nodes, om.nodes) and np.allclose(m.cell_centers, om.cell_centers)
+    assert np.allclose(m.cell_volumes, om.cell_volumes)
+    assert abs(m.cell_volumes.sum() - 4 * 4 * 3) < 1e-12
+
+
+def test_nodal_gradient_matches_oracle():
+    mesh, _, _ = tiny_problem()
+    om = OracleMesh(mesh.h, origin=mesh.origin)
+    assert abs(mesh.nodal_gradient - om.nodal_gradient).max() < 1e-15
+
+
+@pytest.mark.parametrize("surface", [False, True])
+def test_robin_weights_match_oracle_avgbc(surface):
+    mesh, _, m = tiny_problem()
+    om = OracleMesh(mesh.h, origin=mesh.origin)
+    sf = None
+    if surface:
+        rng = np.random.default_rng(0)
+        n_bf = om.boundary_faces.shape[0]
+        sf = rng.random(n_bf) < 0.3
+    AvgBC, alpha = robin_boundary_matrix(om, sf)
+    beta = robin_face_weights(mesh.h, mesh.origin, sf)
+    assert np.allclose(np.nan_to_num(beta["alpha"]), np.nan_to_num(alpha), rtol=1e-13, atol=0)
+    sigma = np.exp(m)
+    b_ref = AvgBC @ sigma
+    b = robin_diagonal(mesh.h, beta, sigma)
+    assert rel(b, b_ref) < 1e-13
+
+
+def test_edge_mass_matches_oracle():
+    mesh, _, m = tiny_problem()
+    om = OracleMesh(mesh.h, origin=mesh.origin)
+    s = np.exp(m)
+    assert rel(edge_mass(mesh.h, s), om.get_edge_inner_product(s).diagonal()) < 1e-14
+    # e^T M_e(1) e / 3 = total volume (tests/base/test_base_pde_sim.py:126-133)
+    assert abs(edge_mass(mesh.h).sum() / 3 - mesh.cell_volumes.sum()) < 1e-9 * mesh.cell_volumes.sum()
+
+
+def test_interpolation_matches_oracle():
+    mesh, _, _ = tiny_problem()
+    om = OracleMesh(mesh.h, origin=mesh.origin)
+    rng = np.random.default_rng(1)
+    lo = np.array([mesh.nodes_x[0], mesh.nodes_y[0], mesh.nodes_z[0]])
+    hi = np.array([mesh.nodes_x[-1], mesh.nodes_y[-1], mesh.nodes_z[-1]])
+    pts = lo + rng.random((50, 3)) * (hi - lo)
+    pts[:5] = mesh.nodes[[0, 7, 100, 500, mesh.n_nodes - 1]]
+    idx, w = node_interpolation(mesh.h, mesh.origin, pts)
+    P = sp.csr_matrix((w.ravel(), (np.repeat(np.arange(len(pts)), 8), idx.ravel())),
+                      shape=(len(pts), mesh.n_nodes))
+    assert abs(P - om.get_interpolation_matrix(pts)).max() < 1e-14
+    # a node is reproduced exactly
+    assert np.allclose(P[:5].toarray().max(axis=1), 1.0)
+    with pytest.raises(ValueError):
+        node_interpolation(mesh.h, mesh.origin, [[1e6, 0, 0]])
+
+
+def test_host_getA_matches_oracle():
+    from simpeg_amd.resistivity import Simulation3DNodal
+    from tests._helpers import oracle_for
+
+    mesh, survey, m = tiny_problem()
+    for bc in ("Robin", "Neumann"):
+        sim = Simulation3DNodal(mesh, survey=survey, bc_type=bc, sigmaMap=maps.ExpMap(mesh))
+        sim.model = m
+        orc = oracle_for(mesh, survey, bc)
+        orc.set_model(m)
+        A = sim.getA()
+        assert abs(A - orc.getA()).max() / abs(orc.getA()).max() < 1e-13
+
+
+def test_source_term_matches_oracle():
+    from simpeg_amd.resistivity import Simulation3DNodal
+    from tests._helpers import oracle_for
+
+    mesh, survey, m = tiny_problem()
+    sim = Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.ExpMap(mesh))
+    orc = oracle_for(mesh, survey)
+    assert np.allclose(sim.getSourceTerm(), orc.source_term(), atol=1e-15)
+
+
+def test_survey_ordering_and_generator():
+    src = workloads.line_sources("dipole-dipole", [-120.0, 120.0, 40.0, 40.0], 20.0, 6)
+    # 13 stations -> 10 sources, receivers n = 1..6 then 5, 4, 3, 2, 1
+    assert len(src) == 10
+    assert [s.nD for s in src] == [6, 6, 6, 6, 6, 5, 4, 3, 2, 1]
+    sv = Survey(src)
+    assert sv.nD == 45 and sv.nSrc == 10
+    sl = sv.get_all_slices()
+    assert sl[(src[1], src[1].receiver_list[0])] == slice(6, 12)
+    assert sv.get_slice(src[9], src[9].receiver_list[0]) == slice(44, 45)
+    # pole-dipole and pole-pole layouts
+    pd = workloads.line_sources("pole-dipole", [0.0, 100.0, 0.0, 0.0], 10.0, 3)
+    assert isinstance(pd[0], Src.Pole) and isinstance(pd[0].receiver_list[0], Rx.Dipole)
+    pp = workloads.line_sources("pole-pole", [0.0, 100.0, 0.0, 0.0], 10.0, 3)
+    assert isinstance(pp[0].receiver_list[0], Rx.Pole)
+
+
+def test_geometric_factor_closed_form():
+    a, b = np.r_[0.0, 0, 0], np.r_[10.0, 0, 0]
+    M, N = np.array([[20.0, 0, 0]]), np.array([[30.0, 0, 0]])
+    rx = Rx.Dipole(M, N, data_type="apparent_resistivity")
+    src = Src.Dipole([rx], a, b)
+    sv = Survey([src])
+    G = sv.set_geometric_factor()
+    ref = (1 / 20 - 1 / 10 - 1 / 30 + 1 / 20) / (2 * np.pi)
+    assert np.allclose(G, ref)
+    assert np.allclose(rx.geometric_factor[src], ref)
+
+
+def test_api_errors():
+    mesh, survey, _ = tiny_problem()
+    from simpeg_amd.resistivity import Simulation3DNodal
+
+    with pytest.raises(ValueError):
+        Simulation3DNodal(mesh, survey=survey, bc_type="Dirichlet")
+    with pytest.raises(TypeError):
+        Src.Dipole([], location_a=[0, 0, 0])
+    with pytest.raises(ValueError):
+        Rx.Dipole(locations_m=np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        Src.Pole([], np.zeros((2, 3)))
+    sim = Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.ExpMap(mesh))
+    with pytest.raises(ValueError):
+        sim.model = np.zeros(mesh.n_cells + 1)
+    with pytest.raises(AttributeError):
+        Simulation3DNodal(mesh).survey
+
+
+def test_grf_statistics():
+    mesh = workloads.padded_mesh((24, 24, 12), npad=5)
+    f = workloads.gaussian_random_field(mesh, std=1.0, corr=60.0, seed=3)
+    assert f.shape == (mesh.n_cells,)
+    assert 0.5 < f.std() < 1.5
+    f2 = workloads.gaussian_random_field(mesh, std=1.0, corr=60.0, seed=3)
+    assert np.array_equal(f, f2)
