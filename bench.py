@@ -1,0 +1,307 @@
+#!/usr/bin/env python
+"""Benchmark of the nodal DC forward + adjoint path on one MI355X.
+
+Workload (default "large"): 96x96x64 core of 10 m cells + 8 padding cells x1.3 on the lateral
+sides and below (112x112x72 cells, 932,137 nodes), log sigma = log(0.01) + Gaussian random
+field (std 1, 60 m correlation, seed 0), 128 dipole-dipole sources on 8 surface lines with 16
+receivers each (2,048 data), Robin boundary.
+
+One step = fields(m) for all 128 sources (sigma -> edge conductances of every multigrid level,
+batched solve, receiver projection) + Jtvec(m, v) (P^T v per source, batched adjoint solve,
+edge-gradient contraction, map derivative).  sigma, q, v and the projection tables are resident
+in HBM before the timed region starts.
+
+Reported:
+  value            forward source-solves/s of the batched mixed-precision path
+                   (fp32 multigrid PCG inside fp64 iterative refinement to --rtol)
+  jtvec_ms         mean Jtvec time of the same path
+  fp64_per_source  the plain fp64 multigrid PCG solving one source at a time (same kernels,
+                   one column), source-solves/s, Jtvec ms and the speed-up of the batched
+                   mixed-precision path on forward + Jtvec
+  accuracy         relative l2 errors against an fp64 reference (fp64 batched PCG to 1e-13 on
+                   the GPU; the CPU direct solver cannot factor this mesh)
+  roofline         the dominant kernel's algorithmic bytes per launch / its mean duration,
+                   measured with HIP events on its stream during the timed steps
+  cpu_baseline     the FP64 CPU oracle (scipy splu, the reference's solver without MKL/MUMPS)
+                   on a bounded sample (see "sample")
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(name, seed=0):
+    from simpeg_amd import workloads as W
+
+    if name == "large":
+        return W.large_grf_model(seed)
+    if name == "small":
+        return W.small_block_model(seed)
+    raise ValueError(name)
+
+
+class Pipeline:
+    """The hot path of Simulation3DNodal.fields/dpred/Jtvec at the engine level."""
+
+    def __init__(self, wl, device):
+        import torch
+
+        from simpeg_amd import maps
+        from simpeg_amd.resistivity import Simulation3DNodal
+
+        self.torch = torch
+        self.wl = wl
+        self.sim = Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
+                                     device=device)
+        self.sim.model = wl.model
+        self.eng = self.sim.engine
+        self.nS = wl.survey.nSrc
+        self.proj = self.sim._projection()
+        self.q = self.sim._source_block()
+        self.sig = torch.as_tensor(np.exp(wl.model)).to(self.eng.device)
+        self.sig_deriv = self.sig.clone()  # ExpMap: d sigma / dm = diag(exp(m))
+        rng = np.random.default_rng(1234)
+        self.v = torch.as_tensor(rng.standard_normal(self.proj["nD"])).to(self.eng.device)
+        torch.cuda.synchronize()
+
+    def forward(self, mode, tol, per_source=False, u=None):
+        eng = self.eng
+        eng.set_sigma(self.sig)
+        if per_source:
+            u = eng.block(self.nS, zero=False) if u is None else u
+            it = 0
+            for s in range(self.nS):
+                _, r = eng.solve(self.q[s:s + 1], 1, mode=mode, tol=tol, u=u[s:s + 1])
+                it += r.iters
+            res = r
+        else:
+            u, res = eng.solve(self.q, self.nS, mode=mode, tol=tol)
+            it = res.iters
+        d = eng.gather(self.proj["gather"], u)
+        return u, d, it, res
+
+    def adjoint(self, u, mode, tol, per_source=False):
+        eng = self.eng
+        rhs = eng.scatter(self.proj["scatter"], self.v, self.nS)
+        if per_source:
+            lam = eng.block(self.nS, zero=False)
+            it = 0
+            for s in range(self.nS):
+                _, r = eng.solve(rhs[s:s + 1], 1, mode=mode, tol=tol, u=lam[s:s + 1])
+                it += r.iters
+        else:
+            lam, r = eng.solve(rhs, self.nS, mode=mode, tol=tol)
+            it = r.iters
+        g = eng.contract(u, lam, self.nS)
+        return -(self.sig_deriv * g), it
+
+    def step(self, mode, tol, per_source=False):
+        torch = self.torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u, d, itf, rf = self.forward(mode, tol, per_source)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        jtv, ita = self.adjoint(u, mode, tol, per_source)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return dict(t_fwd=t1 - t0, t_adj=t2 - t1, u=u, d=d, jtv=jtv, it_fwd=itf, it_adj=ita,
+                    res=rf)
+
+
+def rel(a, b):
+    a = a.double()
+    b = b.double()
+    return float((a - b).norm() / b.norm())
+
+
+def rel_cols(u, ref, nN):
+    e = (u[:, :nN] - ref[:, :nN]).norm(dim=1) / ref[:, :nN].norm(dim=1)
+    return float(e.max())
+
+
+def cpu_baseline(budget_s=12.0):
+    """scipy splu oracle on the 24x24x12-core block model (22,050 nodes, 20 sources)."""
+    from simpeg_amd import workloads as W
+    from tests._helpers import oracle_for
+
+    wl = W.small_block_model(0)
+    n = 0
+    t_fwd = t_adj = t_fac = 0.0
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal(wl.survey.nD)
+    start = time.perf_counter()
+    while n == 0 or time.perf_counter() - start < budget_s:
+        orc = oracle_for(wl.mesh, wl.survey)
+        t0 = time.perf_counter()
+        orc.set_model(wl.model)
+        orc.factor()
+        t1 = time.perf_counter()
+        u = orc.solve(orc.source_term())
+        orc.dpred(wl.model, u=u)
+        t2 = time.perf_counter()
+        orc.Jtvec(wl.model, v, u=u)
+        t3 = time.perf_counter()
+        t_fac += t1 - t0
+        t_fwd += t2 - t0
+        t_adj += t3 - t2
+        n += 1
+    nS = wl.survey.nSrc
+    return {
+        "value": round(nS * n / t_fwd, 3),
+        "unit": "source-solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"scipy splu FP64 oracle on the 34x34x17-cell block model ({wl.mesh.n_nodes} "
+                   f"nodes, {nS} sources), {n} repetitions: fields (factor + solves) and Jtvec; "
+                   "splu cannot factor the 932,137-node mesh within the host's time/memory budget"),
+        "factor_s": round(t_fac / n, 3),
+        "jtvec_ms": round(1e3 * t_adj / n, 1),
+        "node_solves_per_s": round(wl.mesh.n_nodes * nS * n / t_fwd, 1),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="large", choices=["large", "small"])
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--profile-kernel", default="smooth",
+                    choices=["smooth", "apply", "residual", "refine", "jtvec", "restrict", "prolong"])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain fp64 per-source baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the fp64 reference solve")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU: the nodal DC benchmark has no multi-GPU path "
+              "(sources would shard across GPUs with one all-reduce of the Jtvec gradient)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(2)
+    dev = torch.device("cuda", 0)
+    wl = build_problem(args.workload)
+    log(f"[bench] workload {wl.name}: cells {wl.mesh.shape_cells}, nodes {wl.mesh.n_nodes}, "
+        f"sources {wl.survey.nSrc}, data {wl.survey.nD}")
+    pl = Pipeline(wl, dev)
+    eng = pl.eng
+    log(f"[bench] multigrid levels {eng.levels}, ld {eng.ld}")
+
+    for w in range(args.warmup):
+        r = pl.step("mixed", args.rtol)
+        log(f"[bench] warmup {w}: fwd {r['t_fwd']*1e3:.1f} ms ({r['it_fwd']} it), "
+            f"jtvec {r['t_adj']*1e3:.1f} ms ({r['it_adj']} it)")
+
+    eng.profile(args.profile_kernel)
+    torch.cuda.synchronize()
+    t_fwd = t_adj = 0.0
+    its = []
+    last = None
+    for s in range(args.steps):
+        r = pl.step("mixed", args.rtol)
+        t_fwd += r["t_fwd"]
+        t_adj += r["t_adj"]
+        its.append((r["it_fwd"], r["it_adj"], r["res"].outer))
+        last = r
+        log(f"[bench] step {s}: fwd {r['t_fwd']*1e3:.1f} ms ({r['it_fwd']} it, "
+            f"{r['res'].outer} refinements, rel res {r['res'].max_rel_res:.2e}), "
+            f"jtvec {r['t_adj']*1e3:.1f} ms ({r['it_adj']} it)")
+    torch.cuda.synchronize()
+    nlaunch, kms, kbytes = eng.profile_read()
+    eng.profile(None)
+    nS = wl.survey.nSrc
+    K = args.steps
+    value = nS * K / t_fwd
+    ms_step = 1e3 * (t_fwd + t_adj) / K
+    jt_ms = 1e3 * t_adj / K
+    out = {
+        "metric": "source-solves/s + Jtvec ms at 96x96x64/128 src, mixed vs FP64 GPU @1e-6",
+        "value": round(value, 2),
+        "unit": "source-solves/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded Gaussian-random-field log-conductivity, generated on the host)",
+        "config": {"workload": wl.name, "cells": list(wl.mesh.shape_cells),
+                   "nodes": wl.mesh.n_nodes, "sources": nS, "data": wl.survey.nD,
+                   "rtol": args.rtol, "levels": eng.levels},
+        "jtvec_ms": round(jt_ms, 3),
+        "forward_ms": round(1e3 * t_fwd / K, 3),
+        "iterations": {"fwd_pcg": its[-1][0], "adj_pcg": its[-1][1], "refinements": its[-1][2]},
+    }
+    avg_ms = kms / max(nlaunch, 1)
+    avg_bytes = kbytes / max(nlaunch, 1)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if nlaunch else 0.0
+    out["roofline"] = {
+        "bound": "hbm", "kernel": args.profile_kernel, "launches": nlaunch,
+        "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": avg_bytes,
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    }
+
+    if not args.no_accuracy:
+        ref_u, ref_res = eng.solve(pl.q, nS, mode="fp64", tol=1e-13, max_iter=2000)
+        ref_d = eng.gather(pl.proj["gather"], ref_u)
+        ref_jtv, _ = pl.adjoint(ref_u, "fp64", 1e-13)
+        out["accuracy"] = {
+            "reference": "fp64 batched multigrid PCG on the GPU to relative residual 1e-13",
+            "potential_max_rel": rel_cols(last["u"], ref_u, eng.nN),
+            "data_rel": rel(last["d"], ref_d),
+            "jtvec_rel": rel(last["jtv"], ref_jtv),
+        }
+        log(f"[bench] accuracy {out['accuracy']}")
+
+    if not args.no_fp64:
+        r64 = pl.step("fp64", args.rtol, per_source=True)
+        fwd64 = r64["t_fwd"]
+        adj64 = r64["t_adj"]
+        out["fp64_per_source"] = {
+            "source_solves_per_s": round(nS / fwd64, 2),
+            "forward_ms": round(1e3 * fwd64, 1),
+            "jtvec_ms": round(1e3 * adj64, 1),
+            "pcg_iterations": {"fwd": r64["it_fwd"], "adj": r64["it_adj"]},
+            "speedup_fwd_plus_jtvec": round((fwd64 + adj64) / ((t_fwd + t_adj) / K), 2),
+        }
+        if not args.no_accuracy:
+            out["fp64_per_source"]["potential_max_rel"] = rel_cols(r64["u"], ref_u, eng.nN)
+            out["fp64_per_source"]["jtvec_rel"] = rel(r64["jtv"], ref_jtv)
+        log(f"[bench] fp64 per-source: {out['fp64_per_source']}")
+
+    if not args.no_cpu:
+        log("[bench] CPU oracle baseline ...")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -41,8 +41,10 @@ extern "C" {
 #define SDC_F32 0
 #define SDC_F64 1
 
-#define SDC_MODE_MIXED 0   /* fp32 GMG-PCG wrapped in fp64 iterative refinement */
-#define SDC_MODE_FP64 1    /* plain fp64 GMG-PCG */
+#define SDC_MODE_MIXED 0     /* fp32 GMG-PCG with fp64 reliable updates (true residual and
+                                solution accumulation in fp64, Krylov space kept) */
+#define SDC_MODE_FP64 1      /* plain fp64 GMG-PCG */
+#define SDC_MODE_MIXED_IR 2  /* fp32 GMG-PCG restarted inside classic fp64 iterative refinement */
 
 typedef struct sdc_handle_s* sdc_handle;
 
@@ -75,18 +77,20 @@ typedef struct {
   double cheb_ratio;           /* lower bound = cheb_lmax / cheb_ratio */
   int32_t rhs_per_thread;      /* 0 = automatic, else 1, 2, 4 or 8 columns per work-item */
   int32_t check_every;         /* PCG iterations between host convergence polls */
+  double reliable_delta;       /* mixed mode: replace the fp32 residual by the fp64 true residual
+                                  once it dropped by this factor (0 < delta < 1) */
 } sdc_params;
 
 typedef struct {
   int32_t mode;                /* SDC_MODE_MIXED or SDC_MODE_FP64 */
   double tol;                  /* per column target |q - A u|_2 / |q|_2 (fp64 residual) */
-  double inner_tol;            /* mixed: relative residual reduction of each fp32 PCG solve */
-  int32_t max_outer;           /* mixed: maximum refinement steps */
+  double inner_tol;            /* mixed IR: relative residual reduction of each fp32 PCG solve */
+  int32_t max_outer;           /* mixed IR: maximum refinement steps */
   int32_t max_iter;            /* maximum PCG iterations per (inner) solve */
 } sdc_solve_opts;
 
 typedef struct {
-  int32_t outer;               /* refinement steps taken (1 for fp64 mode) */
+  int32_t outer;               /* fp64 true-residual recomputations (refinement steps) */
   int32_t iters;               /* PCG iterations summed over refinement steps */
   double max_rel_res;          /* max over columns of |q - A u|/|q| after the solve */
   int32_t converged;           /* 1 if every column reached tol */
@@ -153,6 +157,22 @@ int sdc_gather(sdc_handle h, int nrow, const int32_t* rowptr_dev, const int64_t*
 int sdc_scatter(sdc_handle h, int nrow, const int64_t* dst_dev, const int32_t* rowptr_dev,
                 const int32_t* src_dev, const double* w_dev, const double* v_dev,
                 double* out_dev, int ncol, void* stream);
+
+/* Event profiler (no reference counterpart; instrumentation for the roofline report).
+ * sdc_profile(h, tag) selects one kernel class; each later launch of it is bracketed by HIP
+ * events on its stream.  sdc_profile_read returns the number of launches, their summed
+ * duration (ms) and summed algorithmic bytes (the minimum HBM traffic of each launch: every
+ * array element it must read or write counted once), then clears the record.  tag 0 = off. */
+#define SDC_PROF_OFF 0
+#define SDC_PROF_SMOOTH 1     /* level-0 Chebyshev smoothing step (stencil + update) */
+#define SDC_PROF_APPLY 2      /* level-0 operator application inside PCG (q = A p, p.q) */
+#define SDC_PROF_RESIDUAL 3   /* level-0 V-cycle residual */
+#define SDC_PROF_REFINE 4     /* fp64 refinement residual q - A u */
+#define SDC_PROF_JTVEC 5      /* adjoint edge-gradient contraction */
+#define SDC_PROF_RESTRICT 6   /* restriction level 0 -> 1 */
+#define SDC_PROF_PROLONG 7    /* interpolation level 1 -> 0 */
+int sdc_profile(sdc_handle h, int tag);
+int sdc_profile_read(sdc_handle h, int64_t* count, double* total_ms, double* total_bytes);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,7 @@ SDC_F32 = 0
 SDC_F64 = 1
 SDC_MODE_MIXED = 0
 SDC_MODE_FP64 = 1
+SDC_MODE_MIXED_IR = 2
 
 _dp = C.POINTER(C.c_double)
 
@@ -52,6 +53,7 @@ class Params(C.Structure):
         ("nu_pre", C.c_int32), ("nu_post", C.c_int32),
         ("cheb_lmax", C.c_double), ("cheb_ratio", C.c_double),
         ("rhs_per_thread", C.c_int32), ("check_every", C.c_int32),
+        ("reliable_delta", C.c_double),
     ]
 
 
@@ -90,7 +92,13 @@ _SIGS = {
     "sdc_jtvec_contract": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int64, _vp]),
     "sdc_gather": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sdc_scatter": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "sdc_profile": (C.c_int, [_vp, C.c_int]),
+    "sdc_profile_read": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
 }
+
+PROF_TAGS = {"smooth": 1, "apply": 2, "residual": 3, "refine": 4, "jtvec": 5, "restrict": 6,
+             "prolong": 7}
 
 EXPORTED = tuple(_SIGS)
 

@@ -112,6 +112,31 @@ __device__ __forceinline__ T stencil(const NodeC<T>& c, const T* __restrict__ x,
   return v;
 }
 
+// (A x)_n for R columns starting at column c0 (columns >= ncol are skipped).  All loads are
+// issued before the arithmetic so that a wavefront keeps 7R requests in flight.
+template <typename T, int R>
+__device__ __forceinline__ void stencil_cols(const NodeC<T>& c, const T* __restrict__ x, uint32_t n,
+                                             size_t ld, int c0, int ncol, uint32_t n1x,
+                                             uint32_t plane, T (&xc)[R], T (&v)[R]) {
+  T xw[R], xe[R], xs[R], xn[R], xb[R], xt[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool ok = c0 + r < ncol;
+    const T* col = x + (size_t)(c0 + r) * ld;
+    xc[r] = ok ? col[n] : T(0);
+    xw[r] = (ok && c.hxm) ? col[n - 1] : T(0);
+    xe[r] = (ok && c.hxp) ? col[n + 1] : T(0);
+    xs[r] = (ok && c.hym) ? col[n - n1x] : T(0);
+    xn[r] = (ok && c.hyp) ? col[n + n1x] : T(0);
+    xb[r] = (ok && c.hzm) ? col[n - plane] : T(0);
+    xt[r] = (ok && c.hzp) ? col[n + plane] : T(0);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    v[r] = c.d * xc[r] - c.wxm * xw[r] - c.wxp * xe[r] - c.wym * xs[r] - c.wyp * xn[r] -
+           c.wzm * xb[r] - c.wzp * xt[r];
+}
+
 // Sum R per-thread doubles over the work-group and write one partial per
 // column: partial[(c0 + r) * nbx + bx].
 template <int R>

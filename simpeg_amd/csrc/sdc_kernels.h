@@ -13,11 +13,15 @@
 // ------------------------------------------------------------------------------------------
 // generic launch geometry: blockIdx.x -> 256 nodes (XCD-remapped), blockIdx.y -> R columns
 
-#define SDC_NODE_PROLOGUE(NN)                                  \
-  const uint32_t nbx = gridDim.x;                              \
-  const uint32_t bx = xcd_remap(blockIdx.x, nbx);              \
-  const uint32_t n = bx * SDC_BLOCK + threadIdx.x;             \
-  const int c0 = blockIdx.y * R;                               \
+// The linear work-group id (x fastest, as the dispatcher numbers them) is remapped so that
+// every XCD works through one contiguous range of (column group, node block) pairs.
+#define SDC_NODE_PROLOGUE(NN)                                                   \
+  const uint32_t nbx = gridDim.x;                                               \
+  const uint32_t lin_ = xcd_remap(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y); \
+  const uint32_t by_ = lin_ / nbx;                                              \
+  const uint32_t bx = lin_ - by_ * nbx;                                         \
+  const uint32_t n = bx * SDC_BLOCK + threadIdx.x;                              \
+  const int c0 = (int)by_ * R;                                                  \
   const bool live = n < (NN);
 
 // ------------------------------------------------------------------------------------------
@@ -35,17 +39,17 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> s, const T* __restr
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
   if (live) {
     const NodeC<T> c = node_coef(s, n);
+    T xc[R], v[R];
+    stencil_cols<T, R>(c, x, n, ld, c0, ncol, s.n1x, s.plane, xc, v);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (c0 + r < ncol) {
-        const T* xc = x + (size_t)(c0 + r) * ld;
-        const T v = stencil(c, xc, n, s.n1x, s.plane);
-        y[(size_t)(c0 + r) * ld + n] = v;
-        if (DOT) acc[r] += (double)xc[n] * (double)v;
+        y[(size_t)(c0 + r) * ld + n] = v[r];
+        if (DOT) acc[r] += (double)xc[r] * (double)v[r];
       }
     }
   }
-  if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+  if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
 
 // r = b - A x
@@ -58,13 +62,13 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const T* __re
   SDC_NODE_PROLOGUE(s.nN)
   if (!live) return;
   const NodeC<T> c = node_coef(s, n);
+  T xc[R], v[R], bv[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (c0 + r < ncol) {
-      const size_t o = (size_t)(c0 + r) * ld;
-      res[o + n] = b[o + n] - stencil(c, x + o, n, s.n1x, s.plane);
-    }
-  }
+  for (int r = 0; r < R; ++r) bv[r] = (c0 + r < ncol) ? b[(size_t)(c0 + r) * ld + n] : T(0);
+  stencil_cols<T, R>(c, x, n, ld, c0, ncol, s.n1x, s.plane, xc, v);
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (c0 + r < ncol) res[(size_t)(c0 + r) * ld + n] = bv[r] - v[r];
 }
 
 // First Chebyshev step from a zero guess: x = c0 * D^-1 b
@@ -103,22 +107,29 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth(Sten<T> s, const T* __rest
   if (live) {
     const NodeC<T> c = node_coef(s, n);
     const T w = c2 * s.idg[n];
+    T xc[R], v[R], bv[R], dv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const size_t o = (size_t)(c0 + r) * ld + n;
+      const bool ok = c0 + r < ncol;
+      bv[r] = ok ? b[o] : T(0);
+      dv[r] = (USE_D && ok) ? dsrc[o] : T(0);
+    }
+    stencil_cols<T, R>(c, xin, n, ld, c0, ncol, s.n1x, s.plane, xc, v);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (c0 + r < ncol) {
-        const size_t o = (size_t)(c0 + r) * ld;
-        const T bv = b[o + n];
-        const T res = bv - stencil(c, xin + o, n, s.n1x, s.plane);
-        T dn = w * res;
-        if (USE_D) dn += c1 * dsrc[o + n];
-        const T xn = xin[o + n] + dn;
-        xout[o + n] = xn;
-        ddst[o + n] = dn;
-        if (DOT) acc[r] += (double)bv * (double)xn;
+        const size_t o = (size_t)(c0 + r) * ld + n;
+        T dn = w * (bv[r] - v[r]);
+        if (USE_D) dn += c1 * dv[r];
+        const T xn = xc[r] + dn;
+        xout[o] = xn;
+        ddst[o] = dn;
+        if (DOT) acc[r] += (double)bv[r] * (double)xn;
       }
     }
   }
-  if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+  if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -267,7 +278,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_dot(const T* __restrict__ a, cons
         acc[r] = (double)a[o] * (double)b[o];
       }
   }
-  block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+  block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
 
 // x += alpha p ; r -= alpha q ; partial |r|^2
@@ -296,7 +307,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr(T* __restrict__ x, cons
         acc[r] = (double)rn * (double)rn;
       }
   }
-  block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+  block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
 
 // p = z + beta p  (FIRST: p = z)
@@ -485,7 +496,39 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
         rout[o + n] = (TO)rv;
       }
   }
-  block_partials<R>(acc, partial, c0, ncol, nbx, blockIdx.x);
+  block_partials<R>(acc, partial, c0, ncol, nbx, bx);
+}
+
+// u += x, x = 0 (fold the fp32 partial solution into the fp64 one)
+template <int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold(double* __restrict__ u, float* __restrict__ x,
+                                                       size_t ld, uint32_t nN, int ncol) {
+  SDC_NODE_PROLOGUE(nN)
+  if (!live) return;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (c0 + r < ncol) {
+      const size_t o = (size_t)(c0 + r) * ld + n;
+      u[o] += (double)x[o];
+      x[o] = 0.0f;
+    }
+}
+
+// true-residual check after a reliable update: rr = |q - A u|^2, rmax = rr,
+// active = rr > tol^2 |q|^2
+__global__ __launch_bounds__(SDC_BLOCK) void k_fin_true(const double* __restrict__ partial,
+                                                        uint32_t nbx,
+                                                        const double* __restrict__ qn2,
+                                                        double tol2, double* __restrict__ rr,
+                                                        double* __restrict__ rmax,
+                                                        int* __restrict__ active) {
+  const int c = blockIdx.x;
+  const double v = block_sum(partial + (size_t)c * nbx, nbx);
+  if (threadIdx.x == 0) {
+    rr[c] = v;
+    rmax[c] = v;
+    active[c] = (qn2[c] > 0.0 && v > tol2 * qn2[c]) ? 1 : 0;
+  }
 }
 
 // u += e
@@ -772,31 +815,31 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_symmetrize(double* __restrict__ a
 
 // In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one work-group of 1024.
 // a: nn x nn (destroyed), inv: nn x nn output, f: nn scratch for the elimination factors.
+// Each pivot step updates all nn*nn entries of both matrices in parallel.
 __global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ a, double* __restrict__ inv,
                                                        double* __restrict__ f, int nn) {
   const int tid = threadIdx.x;
-  const size_t N = nn;
-  for (size_t e = tid; e < N * N; e += 1024) inv[e] = ((e / N) == (e % N)) ? 1.0 : 0.0;
+  const int N = nn;
+  const int NN = N * N;
+  for (int e = tid; e < NN; e += 1024) inv[e] = ((e / N) == (e % N)) ? 1.0 : 0.0;
   __syncthreads();
-  for (int p = 0; p < nn; ++p) {
-    const double piv = a[(size_t)p * N + p];
+  for (int p = 0; p < N; ++p) {
+    const double ip = 1.0 / a[p * N + p];
     __syncthreads();
-    const double ip = 1.0 / piv;
-    for (int m = tid; m < nn; m += 1024) {
-      a[(size_t)p * N + m] *= ip;
-      inv[(size_t)p * N + m] *= ip;
+    for (int m = tid; m < N; m += 1024) {
+      a[p * N + m] *= ip;
+      inv[p * N + m] *= ip;
     }
     __syncthreads();
-    for (int r = tid; r < nn; r += 1024) f[r] = (r == p) ? 0.0 : a[(size_t)r * N + p];
+    for (int r = tid; r < N; r += 1024) f[r] = (r == p) ? 0.0 : a[r * N + p];
     __syncthreads();
-    for (int m = tid; m < nn; m += 1024) {
-      const double ap = a[(size_t)p * N + m], vp = inv[(size_t)p * N + m];
-      for (int r = 0; r < nn; ++r) {
-        const double fr = f[r];
-        if (fr != 0.0) {
-          a[(size_t)r * N + m] -= fr * ap;
-          inv[(size_t)r * N + m] -= fr * vp;
-        }
+    for (int e = tid; e < NN; e += 1024) {
+      const int r = e / N;
+      const int m = e - r * N;
+      const double fr = f[r];
+      if (fr != 0.0) {
+        a[e] -= fr * a[p * N + m];
+        inv[e] -= fr * inv[p * N + m];
       }
     }
     __syncthreads();

@@ -154,6 +154,11 @@ struct sdc_handle_s {
   Work<float> w32;
   Work<double> w64;
   int* h_flag = nullptr;  // pinned host copy of the convergence flag
+  // event profiler for one kernel class (sdc_profile)
+  int prof_tag = 0;
+  std::vector<hipEvent_t> prof_ev;
+  size_t prof_used = 0;
+  std::vector<double> prof_bytes;
 };
 
 namespace {
@@ -169,6 +174,24 @@ struct DeviceGuard {
     if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
   }
 };
+
+// Brackets one launch of the kernel class selected by sdc_profile with events on its stream.
+void prof_begin(sdc_handle_s* h, int tag, hipStream_t st) {
+  if (h->prof_tag != tag) return;
+  while (h->prof_ev.size() < h->prof_used + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    h->prof_ev.push_back(e);
+  }
+  (void)hipEventRecord(h->prof_ev[h->prof_used], st);
+}
+
+void prof_end(sdc_handle_s* h, int tag, hipStream_t st, double bytes) {
+  if (h->prof_tag != tag || h->prof_ev.size() < h->prof_used + 2) return;
+  (void)hipEventRecord(h->prof_ev[h->prof_used + 1], st);
+  h->prof_used += 2;
+  h->prof_bytes.push_back(bytes);
+}
 
 template <typename T>
 Sten<T> make_sten(const sdc_handle_s* h, int l, int pin_mode);
@@ -426,24 +449,35 @@ int vcycle(sdc_handle_s* h, int l, const T* b, T* x, int ncol, bool dot_out, con
   const dim3 blk(SDC_BLOCK);
   DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol,
                                    (T)cc.c2[0], done));
+  const double nodes = (double)lv.nN, cols = (double)ncol, sT = (double)sizeof(T);
+  const int ptag = (l == 0) ? SDC_PROF_SMOOTH : -1;
   for (int k = 1; k < nu1; ++k) {
     const T* dsrc = (k == 1) ? cur : W.d[l];
+    prof_begin(h, ptag, st);
     DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, true, false>), g, blk, 0, st, s, b,
                                      (const T*)cur, oth, dsrc, W.d[l], lv.ld, ncol, (T)cc.c1[k],
                                      (T)cc.c2[k], (double*)nullptr, done));
+    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * (dsrc == cur ? 4 : 5));
     std::swap(cur, oth);
   }
+  prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
   DISPATCH_R(R, hipLaunchKernelGGL((k_residual<T, R>), g, blk, 0, st, s, b, (const T*)cur,
                                    W.res[l], lv.ld, ncol, done));
+  prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * sT * 3);
   const Xfer<T> t = make_xfer<T>(h, l);
   const Level& cl = h->L[l + 1];
+  prof_begin(h, l == 0 ? SDC_PROF_RESTRICT : -1, st);
   DISPATCH_R(R, hipLaunchKernelGGL((k_restrict<T, R>), grid2(cl.nN, ncol, R), blk, 0, st, t,
                                    (const T*)W.res[l], W.b[l + 1], lv.ld, cl.ld, ncol, done));
+  prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, (nodes + (double)cl.nN) * cols * sT);
   CK(vcycle<T>(h, l + 1, W.b[l + 1], W.xs[l + 1], ncol, false, done, st));
+  prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
   DISPATCH_R(R, hipLaunchKernelGGL((k_prolong_add<T, R>), g, blk, 0, st, t,
                                    (const T*)W.xs[l + 1], cur, lv.ld, cl.ld, ncol, done));
+  prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sT);
   for (int k = 0; k < nu2; ++k) {
     const bool last = (k == nu2 - 1) && dot_out;
+    prof_begin(h, ptag, st);
     if (k == 0) {
       if (last) {
         DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, false, true>), g, blk, 0, st, s, b,
@@ -465,6 +499,7 @@ int vcycle(sdc_handle_s* h, int l, const T* b, T* x, int ncol, bool dot_out, con
                                          ncol, (T)cc.c1[k], (T)cc.c2[k], (double*)nullptr, done));
       }
     }
+    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * (k == 0 ? 4 : 5));
     std::swap(cur, oth);
   }
   if (cur != x) return fail(SDC_ERR_STATE, "internal: V-cycle buffer parity");
@@ -503,8 +538,11 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   int it = 0;
   bool finished = false;
   for (it = 1; it <= maxit && !finished; ++it) {
+    prof_begin(h, SDC_PROF_APPLY, st);
     DISPATCH_R(R, hipLaunchKernelGGL((k_apply<T, R, true>), g, blk, 0, st, s, (const T*)W.p, W.q,
                                      l0.ld, ncol, W.partial, (const int*)done));
+    prof_end(h, SDC_PROF_APPLY, st,
+             (double)l0.nN * 4 * sizeof(T) + (double)l0.nN * ncol * 2 * sizeof(T));
     hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                        (const double*)W.rz, W.alpha, (const int*)W.active, (const int*)done);
     DISPATCH_R(R, hipLaunchKernelGGL((k_update_xr<T, R>), g, blk, 0, st, x, (const T*)W.p, W.r,
@@ -533,6 +571,124 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   return SDC_OK;
 }
 
+// fp32 PCG with reliable updates: the fp32 partial solution is folded into the fp64 solution u
+// and the fp32 residual is replaced by the fp64 true residual q - A u whenever every active
+// column's fp32 residual passed the tolerance, or some column's residual dropped by
+// reliable_delta since its last replacement.  Columns stop on the true residual only.
+int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sdc_solve_opts* o,
+                 sdc_solve_stats& S, hipStream_t st) {
+  CK(ensure_work<float>(h, ncol));
+  Work<float>& W = h->w32;
+  const Level& l0 = h->L[0];
+  const int R = pick_r(h, ncol);
+  const dim3 g = grid2(l0.nN, ncol, R);
+  const dim3 blk(SDC_BLOCK);
+  const uint32_t nbx = nblk(l0.nN);
+  const size_t len = l0.ld * (size_t)ncol;
+  PinFix pf;
+  pf.on = h->bc == SDC_BC_NEUMANN;
+  pf.n1x = l0.n1x;
+  pf.plane = l0.plane;
+  pf.cx = l0.cx64;
+  pf.cy = l0.cy64;
+  pf.cz = l0.cz64;
+  const Sten<double> s2 = make_sten<double>(h, 0, 2);
+  const Sten<float> s1 = make_sten<float>(h, 0, 1);
+  const double tol2 = o->tol * o->tol;
+  const double delta2 = h->prm.reliable_delta * h->prm.reliable_delta;
+  hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
+  hipLaunchKernelGGL(k_zero<float>, dim3(nblk(len)), blk, 0, st, W.x, len);
+  DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
+                                   (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+  // aux = |q|^2 and active = |q| > 0; orr = |q|^2
+  hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
+                     W.active, 1);
+  hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
+                     W.active, 0);
+  std::vector<double> h_qn(ncol), h_rr(ncol), h_rmax(ncol);
+  HIPCK(hipMemcpyAsync(h_qn.data(), W.aux, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCK(hipStreamSynchronize(st));
+  h_rmax = h_qn;
+  std::vector<char> act(ncol);
+  int nact = 0;
+  for (int c = 0; c < ncol; ++c) {
+    act[c] = h_qn[c] > 0.0;
+    nact += act[c];
+  }
+  S.outer = 0;
+  S.iters = 0;
+  S.max_rel_res = 0.0;
+  S.converged = nact == 0;
+  if (nact == 0) return SDC_OK;
+  // the finalisers test the convergence flag counters[3]; a previous solve may have left it set
+  hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)4);
+  CK(vcycle<float>(h, 0, W.r, W.z, ncol, true, nullptr, st));
+  hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
+                     W.beta, (const int*)W.active, 1, (const int*)(W.counters + 3));
+  DISPATCH_R(R, hipLaunchKernelGGL((k_update_p<float, R, true>), g, blk, 0, st, (const float*)W.z,
+                                   W.p, (const double*)W.beta, l0.ld, l0.nN, ncol,
+                                   (const int*)nullptr));
+  const int check = std::max(1, (int)h->prm.check_every);
+  const int maxit = o->max_iter;
+  int it;
+  for (it = 1; it <= maxit; ++it) {
+    prof_begin(h, SDC_PROF_APPLY, st);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_apply<float, R, true>), g, blk, 0, st, s1, (const float*)W.p,
+                                     W.q, l0.ld, ncol, W.partial, (const int*)nullptr));
+    prof_end(h, SDC_PROF_APPLY, st, (double)l0.nN * 16 + (double)l0.nN * ncol * 8);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
+                       (const double*)W.rz, W.alpha, (const int*)W.active,
+                       (const int*)(W.counters + 3));
+    DISPATCH_R(R, hipLaunchKernelGGL((k_update_xr<float, R>), g, blk, 0, st, W.x, (const float*)W.p,
+                                     W.r, (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN,
+                                     ncol, W.partial, (const int*)nullptr));
+    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rr,
+                       W.active, 0);
+    if (it % check == 0 || it == maxit) {
+      HIPCK(hipMemcpyAsync(h_rr.data(), W.rr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPCK(hipStreamSynchronize(st));
+      bool cand = true, drop = false;
+      for (int c = 0; c < ncol; ++c) {
+        if (!act[c]) continue;
+        if (h_rr[c] > tol2 * h_qn[c]) cand = false;
+        if (h_rr[c] <= delta2 * h_rmax[c]) drop = true;
+      }
+      if (cand || drop || it == maxit) {
+        prof_begin(h, SDC_PROF_REFINE, st);
+        DISPATCH_R(R, hipLaunchKernelGGL((k_ir_fold<R>), g, blk, 0, st, u, W.x, l0.ld, l0.nN, ncol));
+        DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, false>), g, blk, 0, st, s2, q,
+                                         (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+        prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 44);
+        hipLaunchKernelGGL(k_fin_true, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
+                           (const double*)W.aux, tol2, W.orr, W.rr, W.active);
+        HIPCK(hipMemcpyAsync(h_rr.data(), W.orr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        S.outer += 1;
+        nact = 0;
+        double m = 0.0;
+        for (int c = 0; c < ncol; ++c) {
+          h_rmax[c] = h_rr[c];
+          act[c] = h_qn[c] > 0.0 && h_rr[c] > tol2 * h_qn[c];
+          nact += act[c];
+          if (h_qn[c] > 0.0) m = std::max(m, std::sqrt(h_rr[c] / h_qn[c]));
+        }
+        S.max_rel_res = m;
+        if (nact == 0 || it == maxit) break;
+      }
+    }
+    CK(vcycle<float>(h, 0, W.r, W.z, ncol, true, nullptr, st));
+    hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
+                       W.beta, (const int*)W.active, 0, (const int*)(W.counters + 3));
+    DISPATCH_R(R, hipLaunchKernelGGL((k_update_p<float, R, false>), g, blk, 0, st,
+                                     (const float*)W.z, W.p, (const double*)W.beta, l0.ld, l0.nN,
+                                     ncol, (const int*)nullptr));
+  }
+  S.iters = std::min(it, maxit);
+  S.converged = nact == 0;
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
 }  // namespace
 
 // ============================================================================================
@@ -551,7 +707,8 @@ int sdc_default_params(sdc_params* p) {
   p->cheb_lmax = 2.0;
   p->cheb_ratio = 4.0;
   p->rhs_per_thread = 0;
-  p->check_every = 4;
+  p->check_every = 2;
+  p->reliable_delta = 1e-2;
   return SDC_OK;
 }
 
@@ -608,7 +765,7 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
   }
   fill_level_dims(l0);
   h->L.push_back(l0);
-  const uint32_t cmax = d->coarse_max > 0 ? (uint32_t)d->coarse_max : 512u;
+  const uint32_t cmax = d->coarse_max > 0 ? (uint32_t)d->coarse_max : 128u;
   while (h->L.back().nN > cmax && h->L.size() < 24) {
     Level& f = h->L.back();
     if (f.ax[0].n < 2 && f.ax[1].n < 2 && f.ax[2].n < 2) break;
@@ -648,6 +805,7 @@ int sdc_destroy(sdc_handle h) {
   h->w32.release();
   h->w64.release();
   if (h->h_flag) (void)hipHostFree(h->h_flag);
+  for (auto e : h->prof_ev) (void)hipEventDestroy(e);
   delete h;
   return SDC_OK;
 }
@@ -666,6 +824,8 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
   if (p->rhs_per_thread != 0 && p->rhs_per_thread != 1 && p->rhs_per_thread != 2 &&
       p->rhs_per_thread != 4 && p->rhs_per_thread != 8)
     return fail(SDC_ERR_ARG, "rhs_per_thread must be 0, 1, 2, 4 or 8");
+  if (!(p->reliable_delta > 0) || !(p->reliable_delta < 1))
+    return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
   h->prm = *p;
   if (h->prm.check_every < 1) h->prm.check_every = 1;
   return SDC_OK;
@@ -761,10 +921,10 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
               sdc_solve_stats* stats, void* stream) {
   if (!h || !q || !u || !o || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
   if (!h->sigma_set) return fail(SDC_ERR_STATE, "sdc_set_sigma has not been called");
-  if (o->mode != SDC_MODE_MIXED && o->mode != SDC_MODE_FP64)
-    return fail(SDC_ERR_ARG, "mode must be SDC_MODE_MIXED or SDC_MODE_FP64");
+  if (o->mode != SDC_MODE_MIXED && o->mode != SDC_MODE_FP64 && o->mode != SDC_MODE_MIXED_IR)
+    return fail(SDC_ERR_ARG, "mode must be SDC_MODE_MIXED, SDC_MODE_MIXED_IR or SDC_MODE_FP64");
   if (!(o->tol > 0) || o->max_iter < 1) return fail(SDC_ERR_ARG, "tol must be > 0, max_iter >= 1");
-  if (o->mode == SDC_MODE_MIXED && (!(o->inner_tol > 0) || o->max_outer < 1))
+  if (o->mode == SDC_MODE_MIXED_IR && (!(o->inner_tol > 0) || o->max_outer < 1))
     return fail(SDC_ERR_ARG, "mixed mode needs inner_tol > 0 and max_outer >= 1");
   DeviceGuard dg(h->device);
   hipStream_t st = (hipStream_t)stream;
@@ -798,7 +958,9 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     S.max_rel_res = m;
     return SDC_OK;
   };
-  if (o->mode == SDC_MODE_FP64) {
+  if (o->mode == SDC_MODE_MIXED) {
+    CK(pcg_reliable(h, q, u, ncol, o, S, st));
+  } else if (o->mode == SDC_MODE_FP64) {
     CK(ensure_work<double>(h, ncol));
     Work<double>& W = h->w64;
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, true>), g, blk, 0, st, s2, q,
@@ -831,8 +993,10 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     const double tol2 = o->tol * o->tol;
     for (int outer = 0;; ++outer) {
       if (outer > 0) {
+        prof_begin(h, SDC_PROF_REFINE, st);
         DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, false>), g, blk, 0, st, s2, q,
                                          (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+        prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 20);
         hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                            W.orr, W.active, 0);
       }
@@ -902,13 +1066,45 @@ int sdc_jtvec_contract(sdc_handle h, const double* u, const double* lam, int nco
   hipStream_t st = (hipStream_t)stream;
   const Geo geo = make_geo(h);
   const Level& l0 = h->L[0];
+  prof_begin(h, SDC_PROF_JTVEC, st);
   if (per_column)
     hipLaunchKernelGGL(k_jtvec<true>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
                        ncol, (const int*)ucol, out, (size_t)ldo);
   else
     hipLaunchKernelGGL(k_jtvec<false>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
                        ncol, (const int*)ucol, out, (size_t)ldo);
+  prof_end(h, SDC_PROF_JTVEC, st,
+           (double)l0.nN * ncol * 16 + (double)nC * 8 * (per_column ? ncol : 1));
   HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+int sdc_profile(sdc_handle h, int tag) {
+  if (!h) return fail(SDC_ERR_ARG, "null handle");
+  if (tag < 0 || tag > SDC_PROF_PROLONG) return fail(SDC_ERR_ARG, "unknown profile tag");
+  h->prof_tag = tag;
+  h->prof_used = 0;
+  h->prof_bytes.clear();
+  return SDC_OK;
+}
+
+int sdc_profile_read(sdc_handle h, int64_t* count, double* total_ms, double* total_bytes) {
+  if (!h || !count || !total_ms || !total_bytes) return fail(SDC_ERR_ARG, "null argument");
+  DeviceGuard dg(h->device);
+  double ms = 0.0, by = 0.0;
+  const size_t n = h->prof_used / 2;
+  for (size_t i = 0; i < n; ++i) {
+    HIPCK(hipEventSynchronize(h->prof_ev[2 * i + 1]));
+    float t = 0.f;
+    HIPCK(hipEventElapsedTime(&t, h->prof_ev[2 * i], h->prof_ev[2 * i + 1]));
+    ms += t;
+    by += h->prof_bytes[i];
+  }
+  *count = (int64_t)n;
+  *total_ms = ms;
+  *total_bytes = by;
+  h->prof_used = 0;
+  h->prof_bytes.clear();
   return SDC_OK;
 }
 

@@ -136,7 +136,10 @@ class NodalEngine:
 
     # ------------------------------------------------------------------ operator
     def set_sigma(self, sigma):
-        s = torch.as_tensor(np.asarray(sigma, dtype=np.float64)).to(self.device)
+        if isinstance(sigma, torch.Tensor):
+            s = sigma.to(self.device, torch.float64)
+        else:
+            s = torch.as_tensor(np.asarray(sigma, dtype=np.float64)).to(self.device)
         if s.numel() != self.nC:
             raise ValueError(f"sigma must have {self.nC} entries, got {s.numel()}")
         s = s.contiguous()
@@ -165,7 +168,8 @@ class NodalEngine:
         if u is None:
             u = self.block(q.shape[0], zero=False)
         o = _lib.SolveOpts()
-        o.mode = _lib.SDC_MODE_FP64 if mode == "fp64" else _lib.SDC_MODE_MIXED
+        o.mode = {"fp64": _lib.SDC_MODE_FP64, "mixed": _lib.SDC_MODE_MIXED,
+                  "mixed_ir": _lib.SDC_MODE_MIXED_IR}[mode]
         o.tol, o.inner_tol, o.max_outer, o.max_iter = tol, inner_tol, max_outer, max_iter
         st = _lib.SolveStats()
         rc = self.lib.sdc_solve(self.h, _lib.ptr(q), _lib.ptr(u), ncol, C.byref(o), C.byref(st),
@@ -175,9 +179,24 @@ class NodalEngine:
             _lib.check(rc, "sdc_solve")
         return u, res
 
+    def profile(self, tag):
+        """Bracket launches of one kernel class with events ("smooth", "apply", ...; None = off)."""
+        t = 0 if tag is None else _lib.PROF_TAGS[tag]
+        _lib.check(self.lib.sdc_profile(self.h, t), "sdc_profile")
+
+    def profile_read(self):
+        """(launches, total ms, total algorithmic bytes) since profile() or the last read."""
+        n, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        _lib.check(self.lib.sdc_profile_read(self.h, C.byref(n), C.byref(ms), C.byref(by)),
+                   "sdc_profile_read")
+        return int(n.value), float(ms.value), float(by.value)
+
     def jvec_rhs(self, dsig, u, ncol=None):
         ncol = u.shape[0] if ncol is None else ncol
-        ds = torch.as_tensor(np.asarray(dsig, dtype=np.float64)).to(self.device).contiguous()
+        if isinstance(dsig, torch.Tensor):
+            ds = dsig.to(self.device, torch.float64).contiguous()
+        else:
+            ds = torch.as_tensor(np.asarray(dsig, dtype=np.float64)).to(self.device).contiguous()
         out = torch.empty_like(u)
         _lib.check(self.lib.sdc_jvec_rhs(self.h, _lib.ptr(ds), _lib.ptr(u), _lib.ptr(out), ncol,
                                          self._stream()), "sdc_jvec_rhs")

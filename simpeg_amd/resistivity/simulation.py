@@ -44,9 +44,10 @@ class Simulation3DNodal:
     verbose``).  ``solver``/``solver_opts`` are accepted for signature compatibility; the GPU
     solver replaces the direct factorisation they configure.  GPU-specific keywords:
 
-    precision : {"mixed", "fp64"}
-        "mixed": fp32 multigrid PCG wrapped in fp64 iterative refinement (default);
-        "fp64": plain fp64 multigrid PCG.
+    precision : {"mixed", "mixed_ir", "fp64"}
+        "mixed": fp32 multigrid PCG with fp64 reliable updates (true residual and solution
+        accumulation in fp64; default); "mixed_ir": fp32 PCG restarted inside classic fp64
+        iterative refinement; "fp64": plain fp64 multigrid PCG.
     rtol : float
         per-source target of |q - A u| / |q| (fp64 residual).
     inner_rtol, max_outer, max_iter : refinement / PCG limits.
@@ -95,8 +96,8 @@ class Simulation3DNodal:
         self.rhoMap = rhoMap
         self._sigma_fixed = None if sigma is None else np.asarray(sigma, dtype=float).ravel()
         self._rho_fixed = None if rho is None else np.asarray(rho, dtype=float).ravel()
-        if precision not in ("mixed", "fp64"):
-            raise ValueError("precision must be 'mixed' or 'fp64'")
+        if precision not in ("mixed", "fp64", "mixed_ir"):
+            raise ValueError("precision must be 'mixed', 'mixed_ir' or 'fp64'")
         self.precision = precision
         self.rtol = float(rtol)
         self.inner_rtol = float(inner_rtol)

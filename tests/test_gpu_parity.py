@@ -127,3 +127,50 @@ def test_small_block_model_against_oracle(gpu):
     w = rng.standard_normal(wl.survey.nD)
     assert rel(sim.Jvec(wl.model, v, f=f), orc.Jvec(wl.model, v, u=u_ref)) < 1e-6
     assert rel(sim.Jtvec(wl.model, w, f=f), orc.Jtvec(wl.model, w, u=u_ref)) < 1e-6
+
+
+def test_small_block_model_against_golden_fixture(gpu):
+    import os
+
+    from simpeg_amd import workloads as W
+
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "small_block_model.npz"))
+    wl = W.small_block_model(0)
+    sim = _sim(wl.mesh, wl.survey)
+    f = sim.fields(wl.model)
+    u = f[:, "phiSolution"]
+    for k, s in enumerate(gold["phi_sources"]):
+        assert rel(u[:, s], gold["phi"][:, k]) < 1e-7
+    assert rel(sim.dpred(wl.model, f=f), gold["dpred"]) < 1e-6
+    v = np.random.default_rng(11).standard_normal(wl.mesh.n_cells)
+    w = np.random.default_rng(12).standard_normal(wl.survey.nD)
+    assert rel(sim.Jvec(wl.model, v, f=f), gold["jvec"]) < 1e-6
+    assert rel(sim.Jtvec(wl.model, w, f=f), gold["jtvec"]) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed_ir", "fp64"])
+def test_precision_modes_agree(gpu, tiny, precision):
+    mesh, survey, m = tiny
+    sim = _sim(mesh, survey, precision=precision)
+    orc = oracle_for(mesh, survey)
+    u_ref = orc.fields(m)
+    f = sim.fields(m)
+    assert sim.last_solve.converged
+    assert sim.last_solve.max_rel_res <= (1.5e-10 if precision == "fp64" else 1e-10)
+    for s in range(survey.nSrc):
+        assert rel(f[:, "phiSolution"][:, s], u_ref[:, s]) < 1e-7
+
+
+def test_mode_switching_on_one_handle(gpu, tiny):
+    """Solves in different precision modes on one handle must not leak state (convergence
+    flags, reduction scalars) into each other."""
+    mesh, survey, m = tiny
+    sim = _sim(mesh, survey)
+    orc = oracle_for(mesh, survey)
+    u_ref = orc.fields(m)
+    for mode in ("mixed_ir", "mixed", "fp64", "mixed", "mixed_ir", "mixed"):
+        sim.precision = mode
+        f = sim.fields(m)
+        assert sim.last_solve.converged, mode
+        for s in range(survey.nSrc):
+            assert rel(f[:, "phiSolution"][:, s], u_ref[:, s]) < 1e-7, mode
