@@ -147,6 +147,14 @@ int sdc_jtvec_contract(sdc_handle h, const double* u_dev, const double* lam_dev,
                        const int32_t* ucol_dev, int per_column, double* out_dev, int64_t ldo,
                        void* stream);
 
+/* out_dev[c] += sum_j wsq[j] * g_{j,c}^2 with g_j the contraction above for column j (paired
+ * with u column ucol[j]): the diagonal of J^T W^T W J before the map derivative, accumulated
+ * over chunks of data columns without materialising J (getJtJdiag,
+ * resistivity/simulation.py:137-154).  wsq_dev: ncol squared data weights. */
+int sdc_jtj_diag_accumulate(sdc_handle h, const double* u_dev, const double* lam_dev, int ncol,
+                            const int32_t* ucol_dev, const double* wsq_dev, double* out_dev,
+                            void* stream);
+
 /* d[i] = sum_{e in [rowptr[i], rowptr[i+1])} w[e] * u[idx[e]]: the receiver projection
  * P u of BaseRx.eval (receivers.py:156-192), idx = flat column*ld + node. */
 int sdc_gather(sdc_handle h, int nrow, const int32_t* rowptr_dev, const int64_t* idx_dev,

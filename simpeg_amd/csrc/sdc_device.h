@@ -98,6 +98,18 @@ __device__ __forceinline__ NodeC<T> node_coef(const Sten<T>& s, uint32_t n) {
   return c;
 }
 
+// Loads/stores of operands that a kernel touches exactly once (right-hand sides, search
+// directions, outputs) carry the non-temporal hint so that L2 keeps the stencil operand's
+// neighbouring planes.
+template <typename T>
+__device__ __forceinline__ T ld_once(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ void st_once(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // (A x)_n for one column (x points at the column base).
 template <typename T>
 __device__ __forceinline__ T stencil(const NodeC<T>& c, const T* __restrict__ x, uint32_t n,

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> s, const T* __restr
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (c0 + r < ncol) {
-        y[(size_t)(c0 + r) * ld + n] = v[r];
+        st_once(y + (size_t)(c0 + r) * ld + n, v[r]);
         if (DOT) acc[r] += (double)xc[r] * (double)v[r];
       }
     }
@@ -64,11 +64,11 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const T* __re
   const NodeC<T> c = node_coef(s, n);
   T xc[R], v[R], bv[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) bv[r] = (c0 + r < ncol) ? b[(size_t)(c0 + r) * ld + n] : T(0);
+  for (int r = 0; r < R; ++r) bv[r] = (c0 + r < ncol) ? ld_once(b + (size_t)(c0 + r) * ld + n) : T(0);
   stencil_cols<T, R>(c, x, n, ld, c0, ncol, s.n1x, s.plane, xc, v);
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if (c0 + r < ncol) res[(size_t)(c0 + r) * ld + n] = bv[r] - v[r];
+    if (c0 + r < ncol) st_once(res + (size_t)(c0 + r) * ld + n, bv[r] - v[r]);
 }
 
 // First Chebyshev step from a zero guess: x = c0 * D^-1 b
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const T* __res
   for (int r = 0; r < R; ++r) {
     if (c0 + r < ncol) {
       const size_t o = (size_t)(c0 + r) * ld;
-      x[o + n] = w * b[o + n];
+      st_once(x + o + n, w * ld_once(b + o + n));
     }
   }
 }
@@ -112,8 +112,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth(Sten<T> s, const T* __rest
     for (int r = 0; r < R; ++r) {
       const size_t o = (size_t)(c0 + r) * ld + n;
       const bool ok = c0 + r < ncol;
-      bv[r] = ok ? b[o] : T(0);
-      dv[r] = (USE_D && ok) ? dsrc[o] : T(0);
+      bv[r] = ok ? ld_once(b + o) : T(0);
+      dv[r] = (USE_D && ok) ? ld_once(dsrc + o) : T(0);
     }
     stencil_cols<T, R>(c, xin, n, ld, c0, ncol, s.n1x, s.plane, xc, v);
 #pragma unroll
@@ -123,13 +123,50 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth(Sten<T> s, const T* __rest
         T dn = w * (bv[r] - v[r]);
         if (USE_D) dn += c1 * dv[r];
         const T xn = xc[r] + dn;
-        xout[o] = xn;
-        ddst[o] = dn;
+        st_once(xout + o, xn);
+        st_once(ddst + o, dn);
         if (DOT) acc[r] += (double)bv[r] * (double)xn;
       }
     }
   }
   if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, bx);
+}
+
+// The first two Chebyshev steps from a zero guess in one pass:
+//   x1 = c0 D^-1 b (recomputed at the 7 stencil points, never stored), r1 = b - A x1,
+//   d1 = c1 x1 + c2 D^-1 r1, x2 = x1 + d1.  Writes x2 and d1.
+template <typename T, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_smooth_pair(Sten<T> s, const T* __restrict__ b,
+                                                           T* __restrict__ xout,
+                                                           T* __restrict__ dout, size_t ld,
+                                                           int ncol, T cz, T c1, T c2,
+                                                           const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE(s.nN)
+  if (!live) return;
+  NodeC<T> c = node_coef(s, n);
+  const T g = s.idg[n];
+  // A x1 = c0 * A' b with the neighbour weights scaled by their 1/diagonal
+  c.d *= g;
+  c.wxm *= c.hxm ? s.idg[n - 1] : T(0);
+  c.wxp *= c.hxp ? s.idg[n + 1] : T(0);
+  c.wym *= c.hym ? s.idg[n - s.n1x] : T(0);
+  c.wyp *= c.hyp ? s.idg[n + s.n1x] : T(0);
+  c.wzm *= c.hzm ? s.idg[n - s.plane] : T(0);
+  c.wzp *= c.hzp ? s.idg[n + s.plane] : T(0);
+  T bc[R], ab[R];
+  stencil_cols<T, R>(c, b, n, ld, c0, ncol, s.n1x, s.plane, bc, ab);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (c0 + r < ncol) {
+      const size_t o = (size_t)(c0 + r) * ld + n;
+      const T x1 = cz * g * bc[r];
+      const T r1 = bc[r] - cz * ab[r];
+      const T d1 = c1 * x1 + c2 * g * r1;
+      st_once(xout + o, x1 + d1);
+      st_once(dout + o, d1);
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -161,6 +198,9 @@ struct Xfer {
   FastDiv ffx, ffy, cfx, cfy;
 };
 
+// rc = P^T rf: every coarse node gathers the 3x3x3 fine nodes around its fine position with
+// separable weights; a missing neighbour (domain edge or coincident coarse node) has weight 0
+// and its index is clamped into the grid.
 template <typename T, int R>
 __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __restrict__ rf,
                                                         T* __restrict__ rc, size_t ldf, size_t ldc,
@@ -176,23 +216,31 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __re
   const T wx[3] = {t.rlx[I], T(1), t.rhx[I]};
   const T wy[3] = {t.rly[J], T(1), t.rhy[J]};
   const T wz[3] = {t.rlz[K], T(1), t.rhz[K]};
+  const int fn1z = (int)(t.fN / t.fplane);
+  const int xi[3] = {fi > 0 ? fi - 1 : 0, fi, fi + 1 < (int)t.fn1x ? fi + 1 : fi};
+  const int yj[3] = {fj > 0 ? fj - 1 : 0, fj, fj + 1 < (int)t.fn1y ? fj + 1 : fj};
+  const int zk[3] = {fk > 0 ? fk - 1 : 0, fk, fk + 1 < fn1z ? fk + 1 : fk};
   T acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = T(0);
+#pragma unroll 1
   for (int c = 0; c < 3; ++c) {
-    if (wz[c] == T(0)) continue;
-    for (int b = 0; b < 3; ++b) {
-      if (wy[b] == T(0)) continue;
-      const T wyz = wy[b] * wz[c];
-      const size_t rowb = (size_t)(fj + b - 1) * t.fn1x + (size_t)(fk + c - 1) * t.fplane;
-      for (int a = 0; a < 3; ++a) {
-        if (wx[a] == T(0)) continue;
-        const T w = wx[a] * wyz;
-        const size_t f = rowb + (size_t)(fi + a - 1);
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-          if (c0 + r < ncol) acc[r] += w * rf[(size_t)(c0 + r) * ldf + f];
+    for (int b = 0; b < 3; ++b) {
+      const T wyz = wy[b] * wz[c];
+      const size_t row = (size_t)yj[b] * t.fn1x + (size_t)zk[c] * t.fplane;
+      T v[R][3];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const bool ok = c0 + r < ncol;
+        const T* f = rf + (size_t)(c0 + r) * ldf + row;
+        v[r][0] = ok ? f[xi[0]] : T(0);
+        v[r][1] = ok ? f[xi[1]] : T(0);
+        v[r][2] = ok ? f[xi[2]] : T(0);
       }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        acc[r] += wyz * (wx[0] * v[r][0] + v[r][1] + wx[2] * v[r][2]);
     }
   }
 #pragma unroll
@@ -200,7 +248,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __re
     if (c0 + r < ncol) rc[(size_t)(c0 + r) * ldc + n] = acc[r];
 }
 
-// xf += P xc
+// xf += P xc.  Every fine node reads the 2x2x2 coarse nodes around it; a coincident axis has
+// weight 0 on its second node, whose index still lies inside the coarse grid.
 template <typename T, int R>
 __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const T* __restrict__ xc,
                                                            T* __restrict__ xf, size_t ldf,
@@ -213,31 +262,36 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const T* _
   const uint32_t i = n - rr * t.fn1x;
   const uint32_t k = fdiv(rr, t.ffy);
   const uint32_t j = rr - k * t.fn1y;
-  const int I = t.pix[i], J = t.piy[j], K = t.piz[k];
   const T ux = t.pwx[i], uy = t.pwy[j], uz = t.pwz[k];
-  const T wx[2] = {T(1) - ux, ux};
-  const T wy[2] = {T(1) - uy, uy};
-  const T wz[2] = {T(1) - uz, uz};
-  T acc[R];
+  const size_t c000 = (size_t)t.pix[i] + (size_t)t.piy[j] * t.cn1x + (size_t)t.piz[k] * t.cplane;
+  const size_t sy = t.cn1x, sz = t.cplane;
+  T cv[R][8], fv[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = T(0);
-  for (int c = 0; c < 2; ++c) {
-    if (wz[c] == T(0)) continue;
-    for (int b = 0; b < 2; ++b) {
-      if (wy[b] == T(0)) continue;
-      for (int a = 0; a < 2; ++a) {
-        if (wx[a] == T(0)) continue;
-        const T w = wx[a] * wy[b] * wz[c];
-        const size_t cc = (size_t)(I + a) + (size_t)(J + b) * t.cn1x + (size_t)(K + c) * t.cplane;
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-          if (c0 + r < ncol) acc[r] += w * xc[(size_t)(c0 + r) * ldc + cc];
-      }
-    }
+  for (int r = 0; r < R; ++r) {
+    const bool ok = c0 + r < ncol;
+    const T* c = xc + (size_t)(c0 + r) * ldc + c000;
+    cv[r][0] = ok ? c[0] : T(0);
+    cv[r][1] = ok ? c[1] : T(0);
+    cv[r][2] = ok ? c[sy] : T(0);
+    cv[r][3] = ok ? c[sy + 1] : T(0);
+    cv[r][4] = ok ? c[sz] : T(0);
+    cv[r][5] = ok ? c[sz + 1] : T(0);
+    cv[r][6] = ok ? c[sz + sy] : T(0);
+    cv[r][7] = ok ? c[sz + sy + 1] : T(0);
+    fv[r] = ok ? xf[(size_t)(c0 + r) * ldf + n] : T(0);
   }
 #pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (c0 + r < ncol) xf[(size_t)(c0 + r) * ldf + n] += acc[r];
+  for (int r = 0; r < R; ++r) {
+    if (c0 + r < ncol) {
+      const T a0 = cv[r][0] + ux * (cv[r][1] - cv[r][0]);
+      const T a1 = cv[r][2] + ux * (cv[r][3] - cv[r][2]);
+      const T a2 = cv[r][4] + ux * (cv[r][5] - cv[r][4]);
+      const T a3 = cv[r][6] + ux * (cv[r][7] - cv[r][6]);
+      const T b0 = a0 + uy * (a1 - a0);
+      const T b1 = a2 + uy * (a3 - a2);
+      xf[(size_t)(c0 + r) * ldf + n] = fv[r] + (b0 + uz * (b1 - b0));
+    }
+  }
 }
 
 // coarsest level: x = Ainv b (dense inverse, n x n; symmetric, so column m is read as row m
@@ -852,13 +906,16 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ a, d
 //              + (hx hy / 4hz) sum_{4 z-edges} du dl
 //              + sum_{boundary faces f of c} beta_f sum_{4 nodes of f} u l
 // (the derivative of u^T A(sigma) lambda with respect to the cell value sigma_c).
-// PERCOL writes one column per lambda column (getJ), otherwise the sum over columns.
+// (per-column output for getJ, weighted squares for getJtJdiag, or the sum over columns)
 
-template <bool PERCOL>
+// MODE 0: out[c] = sum_j g_j;  MODE 1: out[j*ldo + c] = g_j;
+// MODE 2: out[c] += sum_j wsq[j] g_j^2 (diagonal of J^T W^2 J before the map derivative).
+template <int MODE>
 __global__ __launch_bounds__(SDC_BLOCK) void k_jtvec(Geo g, const double* __restrict__ u,
                                                      const double* __restrict__ lam, size_t ld,
                                                      int ncol, const int* __restrict__ ucol,
-                                                     double* __restrict__ out, size_t ldo) {
+                                                     double* __restrict__ out, size_t ldo,
+                                                     const double* __restrict__ wsq) {
   const uint32_t nC = g.nx * g.ny * g.nz;
   const uint32_t nbx = gridDim.x;
   const uint32_t c = xcd_remap(blockIdx.x, nbx) * SDC_BLOCK + threadIdx.x;
@@ -920,12 +977,15 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_jtvec(Geo g, const double* __rest
       gval += by0 * (ul[0] + ul[1] + ul[4] + ul[5]) + by1 * (ul[2] + ul[3] + ul[6] + ul[7]);
       gval += bz0 * (ul[0] + ul[1] + ul[2] + ul[3]) + bz1 * (ul[4] + ul[5] + ul[6] + ul[7]);
     }
-    if (PERCOL)
+    if (MODE == 1)
       out[(size_t)col * ldo + c] = gval;
+    else if (MODE == 2)
+      total += wsq[col] * gval * gval;
     else
       total += gval;
   }
-  if (!PERCOL) out[c] = total;
+  if (MODE == 0) out[c] = total;
+  if (MODE == 2) out[c] += total;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -447,11 +447,22 @@ int vcycle(sdc_handle_s* h, int l, const T* b, T* x, int ncol, bool dot_out, con
   T* oth = (cur == x) ? W.xa[l] : x;
   const dim3 g = grid2(lv.nN, ncol, R);
   const dim3 blk(SDC_BLOCK);
-  DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol,
-                                   (T)cc.c2[0], done));
   const double nodes = (double)lv.nN, cols = (double)ncol, sT = (double)sizeof(T);
   const int ptag = (l == 0) ? SDC_PROF_SMOOTH : -1;
-  for (int k = 1; k < nu1; ++k) {
+  int kstart = 1;
+  if (nu1 >= 2) {
+    // steps 0 and 1 fused; the step-1 output goes where the unfused step 1 would write it
+    prof_begin(h, ptag, st);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth_pair<T, R>), g, blk, 0, st, s, b, oth, W.d[l],
+                                     lv.ld, ncol, (T)cc.c2[0], (T)cc.c1[1], (T)cc.c2[1], done));
+    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * 3);
+    std::swap(cur, oth);
+    kstart = 2;
+  } else {
+    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol,
+                                     (T)cc.c2[0], done));
+  }
+  for (int k = kstart; k < nu1; ++k) {
     const T* dsrc = (k == 1) ? cur : W.d[l];
     prof_begin(h, ptag, st);
     DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, true, false>), g, blk, 0, st, s, b,
@@ -656,8 +667,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       if (cand || drop || it == maxit) {
         prof_begin(h, SDC_PROF_REFINE, st);
         DISPATCH_R(R, hipLaunchKernelGGL((k_ir_fold<R>), g, blk, 0, st, u, W.x, l0.ld, l0.nN, ncol));
-        DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, false>), g, blk, 0, st, s2, q,
-                                         (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+        // fp64 stencil: 4 columns per work-item keep the neighbouring planes inside L2
+        const int R64 = std::min(R, 4);
+        DISPATCH_R(R64, hipLaunchKernelGGL((k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64),
+                                           blk, 0, st, s2, q, (const double*)u, W.r, l0.ld, ncol,
+                                           pf, W.partial));
         prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 44);
         hipLaunchKernelGGL(k_fin_true, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                            (const double*)W.aux, tol2, W.orr, W.rr, W.active);
@@ -705,7 +719,7 @@ int sdc_default_params(sdc_params* p) {
   p->nu_pre = 2;
   p->nu_post = 2;
   p->cheb_lmax = 2.0;
-  p->cheb_ratio = 4.0;
+  p->cheb_ratio = 16.0;
   p->rhs_per_thread = 0;
   p->check_every = 2;
   p->reliable_delta = 1e-2;
@@ -1060,6 +1074,7 @@ int sdc_jtvec_contract(sdc_handle h, const double* u, const double* lam, int nco
                        const int32_t* ucol, int per_column, double* out, int64_t ldo,
                        void* stream) {
   if (!h || !u || !lam || !out || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
+  if (per_column < 0 || per_column > 1) return fail(SDC_ERR_ARG, "per_column must be 0 or 1");
   const int64_t nC = (int64_t)h->nx * h->ny * h->nz;
   if (per_column && ldo < nC) return fail(SDC_ERR_ARG, "ldo must be >= number of cells");
   DeviceGuard dg(h->device);
@@ -1068,13 +1083,26 @@ int sdc_jtvec_contract(sdc_handle h, const double* u, const double* lam, int nco
   const Level& l0 = h->L[0];
   prof_begin(h, SDC_PROF_JTVEC, st);
   if (per_column)
-    hipLaunchKernelGGL(k_jtvec<true>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
-                       ncol, (const int*)ucol, out, (size_t)ldo);
+    hipLaunchKernelGGL(k_jtvec<1>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
+                       ncol, (const int*)ucol, out, (size_t)ldo, (const double*)nullptr);
   else
-    hipLaunchKernelGGL(k_jtvec<false>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
-                       ncol, (const int*)ucol, out, (size_t)ldo);
+    hipLaunchKernelGGL(k_jtvec<0>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
+                       ncol, (const int*)ucol, out, (size_t)ldo, (const double*)nullptr);
   prof_end(h, SDC_PROF_JTVEC, st,
            (double)l0.nN * ncol * 16 + (double)nC * 8 * (per_column ? ncol : 1));
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+int sdc_jtj_diag_accumulate(sdc_handle h, const double* u, const double* lam, int ncol,
+                            const int32_t* ucol, const double* wsq, double* out, void* stream) {
+  if (!h || !u || !lam || !out || !wsq || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
+  const int64_t nC = (int64_t)h->nx * h->ny * h->nz;
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  const Geo geo = make_geo(h);
+  hipLaunchKernelGGL(k_jtvec<2>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam,
+                     h->L[0].ld, ncol, (const int*)ucol, out, (size_t)nC, wsq);
   HIPCK(hipGetLastError());
   return SDC_OK;
 }

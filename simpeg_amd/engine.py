@@ -215,6 +215,15 @@ class NodalEngine:
                                                self._stream()), "sdc_jtvec_contract")
         return out
 
+    def jtj_diag_accumulate(self, u, lam, ncol, ucol, wsq, out):
+        """out[c] += sum_j wsq[j] g_j[c]^2 over ncol adjoint columns (device arrays)."""
+        uc = torch.as_tensor(ucol, dtype=torch.int32).to(self.device).contiguous()
+        w = torch.as_tensor(np.asarray(wsq, dtype=np.float64)).to(self.device).contiguous()
+        _lib.check(self.lib.sdc_jtj_diag_accumulate(self.h, _lib.ptr(u), _lib.ptr(lam), ncol,
+                                                    _lib.ptr(uc), _lib.ptr(w), _lib.ptr(out),
+                                                    self._stream()), "sdc_jtj_diag_accumulate")
+        return out
+
     # ------------------------------------------------------------------ projections
     def gather_csr(self, rowptr, idx, w) -> GatherCSR:
         dev = self.device

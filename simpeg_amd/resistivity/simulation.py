@@ -389,11 +389,41 @@ class Simulation3DNodal:
         return self._Jmatrix
 
     def getJtJdiag(self, m, W=None, f=None):
-        """diag(J^T W^T W J) (``getJtJdiag``, simulation.py:137-154)."""
+        """diag(J^T W^T W J) (``getJtJdiag``, simulation.py:137-154).
+
+        With a stored J (storeJ or an earlier getJ) it is formed from J like the reference.
+        Otherwise, for a diagonal map derivative, it is accumulated on the GPU over chunks of
+        adjoint columns (one per datum) without materialising J."""
         if self._gtgdiag is None:
-            J = self.getJ(m, f=f)
-            w = np.ones(J.shape[0]) if W is None else np.asarray(W.diagonal()) ** 2
-            self._gtgdiag = (w[:, None] * J * J).sum(axis=0)
+            self.model = m
+            sd = self.sigmaDeriv
+            diag_map = sd.shape[0] == sd.shape[1] and sd.nnz <= sd.shape[0] and np.all(
+                sd.tocoo().row == sd.tocoo().col)
+            if self._Jmatrix is not None or self.storeJ or not diag_map:
+                J = self.getJ(m, f=f)
+                w = np.ones(J.shape[0]) if W is None else np.asarray(W.diagonal()) ** 2
+                self._gtgdiag = (w[:, None] * J * J).sum(axis=0)
+                return self._gtgdiag
+            if f is None:
+                f = self.fields(m)
+            self._ensure_sigma()
+            eng = self.engine
+            pr = self._projection()
+            nD = pr["nD"]
+            wsq = np.ones(nD) if W is None else np.asarray(W.diagonal(), dtype=float) ** 2
+            acc = torch.zeros(self._nC, dtype=torch.float64, device=eng.device)
+            rows, nodes, w = pr["rows"], pr["nodes"], pr["w"]
+            for c0 in range(0, nD, self.jchunk):
+                c1 = min(nD, c0 + self.jchunk)
+                k = c1 - c0
+                sel = (rows >= c0) & (rows < c1)
+                dst = (rows[sel] - c0) * eng.ld + nodes[sel]
+                d, rp, sidx, sw = coo_to_scatter(dst, rows[sel] - c0, w[sel])
+                rhs = eng.scatter(eng.scatter_csr(d, rp, sidx, sw), np.ones(k), k)
+                lam = self._solve(rhs, k)
+                eng.jtj_diag_accumulate(f.u_block, lam, k, pr["data_src"][c0:c1], wsq[c0:c1], acc)
+            dsig = sd.diagonal()
+            self._gtgdiag = dsig * dsig * acc.cpu().numpy()
         return self._gtgdiag
 
     def make_synthetic_data(self, m, relative_error=0.05, noise_floor=0.0, f=None,

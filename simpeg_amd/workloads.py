@@ -141,3 +141,19 @@ def large_grf_model(seed=0, nlines=8, nsrc_per_line=16, nrx=16):
     survey = grid_survey(nlines, nsrc_per_line, nrx)
     return Workload("grf_96x96x64_128src", mesh, survey, m,
                     {"core": (96, 96, 64), "npad": 8, "seed": seed})
+
+
+def misfit_problem(seed=0):
+    """Block model + survey of ``small_block_model`` with a data-misfit setup: observed data
+    from a second log-conductivity (log 0.01 + GRF(std 0.5, corr 40 m, seed+1)) plus 3 % Gaussian
+    noise (seed+2); weights W = 1 / (0.03 |dobs| + 1e-5).  Returns (workload, true model).
+    The observed data are computed by the caller (they need a forward solve)."""
+    wl = small_block_model(seed)
+    m_true = np.log(0.01) + gaussian_random_field(wl.mesh, std=0.5, corr=40.0, seed=seed + 1)
+    return wl, m_true
+
+
+def noisy_weights(dclean, seed=2, rel=0.03, floor=1e-5):
+    rng = np.random.default_rng(seed)
+    dobs = dclean + rel * np.abs(dclean) * rng.standard_normal(dclean.shape)
+    return dobs, 1.0 / (rel * np.abs(dobs) + floor)

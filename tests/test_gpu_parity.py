@@ -106,7 +106,12 @@ def test_getJ_and_JtJdiag_match_oracle(gpu, tiny):
 
     W = sp.diags(np.linspace(0.5, 2.0, survey.nD))
     sim._gtgdiag = None
-    assert rel(sim.getJtJdiag(m, W=W), orc.getJtJdiag(m, W=W, u=u_ref)) < 1e-6
+    ref = orc.getJtJdiag(m, W=W, u=u_ref)
+    assert rel(sim.getJtJdiag(m, W=W), ref) < 1e-6
+    # without a stored J the diagonal is accumulated on the GPU
+    sim2 = _sim(mesh, survey)
+    assert rel(sim2.getJtJdiag(m, W=W), ref) < 1e-6
+    assert sim2._Jmatrix is None
 
 
 def test_small_block_model_against_oracle(gpu):
