@@ -178,12 +178,96 @@ def cpu_baseline(budget_s=12.0):
     }
 
 
+def bench_batch(args):
+    """48 independent problems (uneven sizes / source counts) behind MetaSimulation: fields +
+    Jtvec for all of them, one stream after the other vs concurrently on 4 streams."""
+    import torch
+
+    from simpeg_amd import maps, workloads
+    from simpeg_amd.meta import MetaSimulation
+    from simpeg_amd.resistivity import Simulation3DNodal
+
+    batch = workloads.uneven_batch(0)
+    offs = np.cumsum([0] + [wl.mesh.n_cells for wl in batch])
+    m = np.concatenate([wl.model for wl in batch])
+    nsrc = sum(wl.survey.nSrc for wl in batch)
+    res = {}
+    for ns in (1, 4):
+        sims = [Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
+                                  rtol=args.rtol) for wl in batch]
+        maps_ = [maps.Projection(int(offs[-1]), slice(int(offs[k]), int(offs[k + 1])))
+                 for k in range(len(batch))]
+        meta = MetaSimulation(sims, maps_, n_streams=ns)
+        v = np.random.default_rng(5).standard_normal(meta.nD)
+        f = meta.fields(m)  # warm-up (workspace allocation)
+        meta.Jtvec(m, v, f=f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = meta.fields(m + 1e-3)  # a new model: every problem rebuilds its operator
+        t1 = time.perf_counter()
+        meta.Jtvec(m + 1e-3, v, f=f)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res[ns] = (t1 - t0, t2 - t1)
+        log(f"[bench] batch streams={ns}: fields {1e3*(t1-t0):.1f} ms, Jtvec {1e3*(t2-t1):.1f} ms")
+    out = {"metric": "uneven batch: source-solves/s (fields of 48 problems)", "unit": "source-solves/s",
+           "value": round(nsrc / res[4][0], 2), "n_gpus": 1, "higher_is_better": True,
+           "config": {"workload": "uneven_batch_48", "problems": len(batch), "sources": nsrc,
+                      "node_columns": int(sum(wl.mesh.n_nodes * wl.survey.nSrc for wl in batch))},
+           "fields_ms": {str(k): round(1e3 * v[0], 1) for k, v in res.items()},
+           "jtvec_ms": {str(k): round(1e3 * v[1], 1) for k, v in res.items()},
+           "concurrency_speedup": round(sum(res[1]) / sum(res[4]), 2)}
+    print(json.dumps(out), flush=True)
+
+
+def bench_gn(args):
+    """3 Gauss-Newton iterations x 10 CG steps on the 160x160x96-core / 256-source survey,
+    mixed precision vs the fp64 run."""
+    import torch
+
+    from simpeg_amd import maps, workloads
+    from simpeg_amd.inversion import gauss_newton
+    from simpeg_amd.resistivity import Simulation3DNodal
+
+    wl = workloads.layered_ellipsoid_model(0)
+    log(f"[bench] GN workload {wl.name}: {wl.mesh.shape_cells} cells, {wl.mesh.n_nodes} nodes, "
+        f"{wl.survey.nSrc} sources, {wl.survey.nD} data")
+    ref = Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
+                            precision="fp64", rtol=1e-12)
+    dclean = ref.dpred(wl.model)
+    dobs, wd = workloads.noisy_weights(dclean)
+    m0 = np.full(wl.mesh.n_cells, np.log(0.01))
+    runs = {}
+    for prec in ("mixed", "fp64"):
+        sim = Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
+                                precision=prec, rtol=args.rtol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m, h = gauss_newton(sim, dobs, wd, m0, beta=args.gn_beta, n_iter=3, cg_iter=10,
+                            recompute=args.gn_recompute, verbose=True)
+        torch.cuda.synchronize()
+        runs[prec] = (m, h, time.perf_counter() - t0)
+        log(f"[bench] GN {prec}: {runs[prec][2]:.1f} s, phi {h.phi}")
+    mm, hm, tm = runs["mixed"]
+    m64, h64, t64 = runs["fp64"]
+    out = {"metric": "Gauss-Newton (3 x 10 CG) wall time, 176x176x104 cells / 256 sources",
+           "unit": "s", "value": round(tm, 2), "n_gpus": 1, "higher_is_better": False,
+           "config": {"workload": wl.name, "nodes": wl.mesh.n_nodes, "sources": wl.survey.nSrc,
+                      "data": wl.survey.nD, "recompute_fields": args.gn_recompute},
+           "fp64_seconds": round(t64, 2),
+           "phi_mixed": hm.phi, "phi_fp64": h64.phi,
+           "model_update_rel_diff": float(np.linalg.norm(mm - m64) / np.linalg.norm(m64 - m0))}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="large", choices=["large", "small"])
+    ap.add_argument("--workload", default="large", choices=["large", "small", "batch", "gn"])
+    ap.add_argument("--gn-beta", type=float, default=1e-2)
+    ap.add_argument("--gn-recompute", action="store_true")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--profile-kernel", default="smooth",
                     choices=["smooth", "apply", "residual", "refine", "jtvec", "restrict", "prolong"])
@@ -205,6 +289,10 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(2)
     dev = torch.device("cuda", 0)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "gn":
+        return bench_gn(args)
     wl = build_problem(args.workload)
     log(f"[bench] workload {wl.name}: cells {wl.mesh.shape_cells}, nodes {wl.mesh.n_nodes}, "
         f"sources {wl.survey.nSrc}, data {wl.survey.nD}")

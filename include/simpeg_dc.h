@@ -13,6 +13,8 @@
  *  - A batch of node vectors ("columns", one per source or per adjoint right-hand side) is stored
  *    column-major with column stride sdc_ld(h) doubles/floats: column c, node n lives at
  *    base[c * ld + n], node n = i + (nx+1) * (j + (ny+1) * k) (x fastest, the discretize ordering).
+ *  - Column bases (base pointers of a batch) must be 32-byte aligned (sdc_solve checks it);
+ *    sdc_ld(h) is a multiple of 64, so every column of an aligned batch is aligned too.
  *  - Every call is enqueued on `stream` (a hipStream_t; NULL = default stream).  Calls that
  *    return host-visible results (sdc_solve statistics) synchronise that stream.
  *  - Every function returns 0 on success or a nonzero SDC_ERR_* code; sdc_last_error() gives the

@@ -381,6 +381,110 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_p(const T* __restrict__ z,
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// streaming PCG updates, 4 consecutive nodes per work-item (one 16-byte access per array for
+// fp32), one column per blockIdx.y; the last, ragged group of nodes takes a scalar path.
+#define SDC_NODES4 (SDC_BLOCK * 4)
+
+template <typename T>
+using vec4_t = T __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double block_reduce1(double v) {
+  __shared__ double red1[SDC_BLOCK / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red1[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < SDC_BLOCK / 64; ++w) t += red1[w];
+  return t;
+}
+
+// x += alpha p ; r -= alpha q ; partial |r|^2
+template <typename T>
+__global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* __restrict__ x, const T* __restrict__ p,
+                                                          T* __restrict__ r, const T* __restrict__ q,
+                                                          const double* __restrict__ alpha,
+                                                          size_t ld, uint32_t nN,
+                                                          double* __restrict__ partial,
+                                                          const int* __restrict__ done) {
+  if (done && *done) return;
+  const int c = blockIdx.y;
+  const size_t o = (size_t)c * ld;
+  const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
+  const T al = (T)alpha[c];
+  double acc = 0.0;
+  if (n0 + 3 < nN) {
+    vec4_t<T> xv = *(const vec4_t<T>*)(x + o + n0);
+    const vec4_t<T> pv = *(const vec4_t<T>*)(p + o + n0);
+    vec4_t<T> rv = *(const vec4_t<T>*)(r + o + n0);
+    const vec4_t<T> qv = *(const vec4_t<T>*)(q + o + n0);
+    xv += al * pv;
+    rv -= al * qv;
+    *(vec4_t<T>*)(x + o + n0) = xv;
+    *(vec4_t<T>*)(r + o + n0) = rv;
+    acc = (double)rv.x * rv.x + (double)rv.y * rv.y + (double)rv.z * rv.z + (double)rv.w * rv.w;
+  } else {
+    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
+      x[o + n] += al * p[o + n];
+      const T rn = r[o + n] - al * q[o + n];
+      r[o + n] = rn;
+      acc += (double)rn * rn;
+    }
+  }
+  const double t = block_reduce1(acc);
+  if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
+}
+
+// p = z + beta p (FIRST: p = z)
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(SDC_BLOCK) void k_update_p4(const T* __restrict__ z, T* __restrict__ p,
+                                                         const double* __restrict__ beta,
+                                                         size_t ld, uint32_t nN,
+                                                         const int* __restrict__ done) {
+  if (done && *done) return;
+  const int c = blockIdx.y;
+  const size_t o = (size_t)c * ld;
+  const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
+  const T be = FIRST ? T(0) : (T)beta[c];
+  if (n0 + 3 < nN) {
+    const vec4_t<T> zv = *(const vec4_t<T>*)(z + o + n0);
+    if (FIRST) {
+      *(vec4_t<T>*)(p + o + n0) = zv;
+    } else {
+      const vec4_t<T> pv = *(const vec4_t<T>*)(p + o + n0);
+      *(vec4_t<T>*)(p + o + n0) = zv + be * pv;
+    }
+  } else {
+    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n)
+      p[o + n] = FIRST ? z[o + n] : z[o + n] + be * p[o + n];
+  }
+}
+
+// u += x, x = 0
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold4(double* __restrict__ u, float* __restrict__ x,
+                                                        size_t ld, uint32_t nN) {
+  const int c = blockIdx.y;
+  const size_t o = (size_t)c * ld;
+  const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
+  if (n0 + 3 < nN) {
+    const vec4_t<float> xv = *(const vec4_t<float>*)(x + o + n0);
+    vec4_t<double> uv = *(const vec4_t<double>*)(u + o + n0);
+    uv.x += (double)xv.x;
+    uv.y += (double)xv.y;
+    uv.z += (double)xv.z;
+    uv.w += (double)xv.w;
+    *(vec4_t<double>*)(u + o + n0) = uv;
+    *(vec4_t<float>*)(x + o + n0) = vec4_t<float>(0.0f);
+  } else {
+    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
+      u[o + n] += (double)x[o + n];
+      x[o + n] = 0.0f;
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(SDC_BLOCK) void k_zero(T* __restrict__ x, size_t len) {
   const size_t i = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;

@@ -58,11 +58,23 @@ struct Axis {
   std::vector<double> X;  // node coordinates (cumulative widths)
 };
 
-// coarsen an axis: keep every other node, plus the last node when n is odd
-void coarsen_axis(const Axis& f, Axis& c, std::vector<int>& fidx) {
-  fidx.clear();
-  for (int i = 0; i <= f.n; i += 2) fidx.push_back(i);
-  if (fidx.back() != f.n) fidx.push_back(f.n);
+// Coarsen an axis toward isotropic cells: walking from the first node, two neighbouring
+// cells merge into one coarse cell when neither is wider than `target`; otherwise the cell is
+// kept.  On a stretched (padded) tensor mesh the small core cells coarsen first and the wide
+// padding cells only once the target width has grown to theirs, so coarse cells stay close to
+// isotropic and the point smoother keeps working.  Every coarse cell spans 1 or 2 fine cells.
+void coarsen_axis(const Axis& f, Axis& c, std::vector<int>& fidx, double target) {
+  fidx.assign(1, 0);
+  int i = 0;
+  while (i < f.n) {
+    const double w0 = f.X[i + 1] - f.X[i];
+    const double w1 = (i + 1 < f.n) ? f.X[i + 2] - f.X[i + 1] : 0.0;
+    if (i + 1 < f.n && w0 <= target && w1 <= target)
+      i += 2;
+    else
+      i += 1;
+    fidx.push_back(i);
+  }
   c.n = (int)fidx.size() - 1;
   c.X.resize(fidx.size());
   for (size_t I = 0; I < fidx.size(); ++I) c.X[I] = f.X[fidx[I]];
@@ -295,12 +307,12 @@ int alloc_level_arrays(Level& v) {
   return SDC_OK;
 }
 
-// transfer tables between fine level f and the coarse level built from it
-int make_transfer(Level& f, Level& c) {
-  std::vector<int> fidx[3], ri[3], pi[3];
+// transfer tables between fine level f and the coarse level c built from it (c.ax must
+// already hold the coarse axes, fidx the fine index of every coarse node)
+int make_transfer(Level& f, Level& c, std::vector<int> (&fidx)[3]) {
+  std::vector<int> ri[3], pi[3];
   std::vector<double> rl[3], rh[3], pw[3];
   for (int a = 0; a < 3; ++a) {
-    coarsen_axis(f.ax[a], c.ax[a], fidx[a]);
     const int nf = f.ax[a].n, nc = c.ax[a].n;
     pi[a].assign(nf + 1, 0);
     pw[a].assign(nf + 1, 0.0);
@@ -397,6 +409,10 @@ int pick_r(const sdc_handle_s* h, int ncol) {
 inline dim3 grid2(uint32_t nN, int ncol, int R) {
   return dim3(nblk(nN), (uint32_t)((ncol + R - 1) / R), 1);
 }
+
+// 4-nodes-per-work-item streaming kernels: (node blocks, columns)
+inline uint32_t nblk4(size_t n) { return (uint32_t)((n + SDC_NODES4 - 1) / SDC_NODES4); }
+inline dim3 grid4(uint32_t nN, int ncol) { return dim3(nblk4(nN), (uint32_t)ncol, 1); }
 
 // Chebyshev step coefficients for the interval [lmax/ratio, lmax] of D^-1 A
 struct Cheb {
@@ -541,8 +557,8 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   CK(vcycle<T>(h, 0, W.r, W.z, ncol, true, done, st));
   hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                      W.beta, (const int*)W.active, 1, (const int*)done);
-  DISPATCH_R(R, hipLaunchKernelGGL((k_update_p<T, R, true>), g, blk, 0, st, (const T*)W.z, W.p,
-                                   (const double*)W.beta, l0.ld, l0.nN, ncol, (const int*)done));
+  hipLaunchKernelGGL((k_update_p4<T, true>), grid4(l0.nN, ncol), blk, 0, st, (const T*)W.z, W.p,
+                     (const double*)W.beta, l0.ld, l0.nN, (const int*)done);
   const Sten<T> s = make_sten<T>(h, 0, 1);
   const double tol2 = tol * tol;
   const int check = std::max(1, (int)h->prm.check_every);
@@ -556,10 +572,10 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
              (double)l0.nN * 4 * sizeof(T) + (double)l0.nN * ncol * 2 * sizeof(T));
     hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                        (const double*)W.rz, W.alpha, (const int*)W.active, (const int*)done);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_update_xr<T, R>), g, blk, 0, st, x, (const T*)W.p, W.r,
-                                     (const T*)W.q, (const double*)W.alpha, l0.ld, l0.nN, ncol,
-                                     W.partial, (const int*)done));
-    hipLaunchKernelGGL(k_fin_rr, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
+    hipLaunchKernelGGL(k_update_xr4<T>, grid4(l0.nN, ncol), blk, 0, st, x, (const T*)W.p, W.r,
+                       (const T*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
+                       (const int*)done);
+    hipLaunchKernelGGL(k_fin_rr, dim3(ncol), blk, 0, st, (const double*)W.partial, nblk4(l0.nN),
                        (const double*)W.bn2, tol2, W.rr, W.active, W.counters, done,
                        (int*)nullptr, (int)(it == maxit));
     if (it % check == 0 || it == maxit) {
@@ -573,9 +589,8 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
     CK(vcycle<T>(h, 0, W.r, W.z, ncol, true, done, st));
     hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                        W.beta, (const int*)W.active, 0, (const int*)done);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_update_p<T, R, false>), g, blk, 0, st, (const T*)W.z,
-                                     W.p, (const double*)W.beta, l0.ld, l0.nN, ncol,
-                                     (const int*)done));
+    hipLaunchKernelGGL((k_update_p4<T, false>), grid4(l0.nN, ncol), blk, 0, st, (const T*)W.z, W.p,
+                       (const double*)W.beta, l0.ld, l0.nN, (const int*)done);
   }
   HIPCK(hipGetLastError());
   if (iters) *iters = std::min(it, maxit);
@@ -636,9 +651,8 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   CK(vcycle<float>(h, 0, W.r, W.z, ncol, true, nullptr, st));
   hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                      W.beta, (const int*)W.active, 1, (const int*)(W.counters + 3));
-  DISPATCH_R(R, hipLaunchKernelGGL((k_update_p<float, R, true>), g, blk, 0, st, (const float*)W.z,
-                                   W.p, (const double*)W.beta, l0.ld, l0.nN, ncol,
-                                   (const int*)nullptr));
+  hipLaunchKernelGGL((k_update_p4<float, true>), grid4(l0.nN, ncol), blk, 0, st, (const float*)W.z,
+                     W.p, (const double*)W.beta, l0.ld, l0.nN, (const int*)nullptr);
   const int check = std::max(1, (int)h->prm.check_every);
   const int maxit = o->max_iter;
   int it;
@@ -650,11 +664,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                        (const double*)W.rz, W.alpha, (const int*)W.active,
                        (const int*)(W.counters + 3));
-    DISPATCH_R(R, hipLaunchKernelGGL((k_update_xr<float, R>), g, blk, 0, st, W.x, (const float*)W.p,
-                                     W.r, (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN,
-                                     ncol, W.partial, (const int*)nullptr));
-    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rr,
-                       W.active, 0);
+    hipLaunchKernelGGL(k_update_xr4<float>, grid4(l0.nN, ncol), blk, 0, st, W.x, (const float*)W.p,
+                       W.r, (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nblk4(l0.nN),
+                       W.rr, W.active, 0);
     if (it % check == 0 || it == maxit) {
       HIPCK(hipMemcpyAsync(h_rr.data(), W.rr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
       HIPCK(hipStreamSynchronize(st));
@@ -666,7 +680,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       }
       if (cand || drop || it == maxit) {
         prof_begin(h, SDC_PROF_REFINE, st);
-        DISPATCH_R(R, hipLaunchKernelGGL((k_ir_fold<R>), g, blk, 0, st, u, W.x, l0.ld, l0.nN, ncol));
+        hipLaunchKernelGGL(k_ir_fold4, grid4(l0.nN, ncol), blk, 0, st, u, W.x, l0.ld, l0.nN);
         // fp64 stencil: 4 columns per work-item keep the neighbouring planes inside L2
         const int R64 = std::min(R, 4);
         DISPATCH_R(R64, hipLaunchKernelGGL((k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64),
@@ -693,9 +707,9 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     CK(vcycle<float>(h, 0, W.r, W.z, ncol, true, nullptr, st));
     hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                        W.beta, (const int*)W.active, 0, (const int*)(W.counters + 3));
-    DISPATCH_R(R, hipLaunchKernelGGL((k_update_p<float, R, false>), g, blk, 0, st,
-                                     (const float*)W.z, W.p, (const double*)W.beta, l0.ld, l0.nN,
-                                     ncol, (const int*)nullptr));
+    hipLaunchKernelGGL((k_update_p4<float, false>), grid4(l0.nN, ncol), blk, 0, st,
+                       (const float*)W.z, W.p, (const double*)W.beta, l0.ld, l0.nN,
+                       (const int*)nullptr);
   }
   S.iters = std::min(it, maxit);
   S.converged = nact == 0;
@@ -719,7 +733,7 @@ int sdc_default_params(sdc_params* p) {
   p->nu_pre = 2;
   p->nu_post = 2;
   p->cheb_lmax = 2.0;
-  p->cheb_ratio = 16.0;
+  p->cheb_ratio = 8.0;
   p->rhs_per_thread = 0;
   p->check_every = 2;
   p->reliable_delta = 1e-2;
@@ -780,11 +794,23 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
   fill_level_dims(l0);
   h->L.push_back(l0);
   const uint32_t cmax = d->coarse_max > 0 ? (uint32_t)d->coarse_max : 128u;
-  while (h->L.back().nN > cmax && h->L.size() < 24) {
+  double hmin = 1e300;
+  for (int a = 0; a < 3; ++a)
+    for (double v : *w[a]) hmin = std::min(hmin, v);
+  // target width of coarsening step s: hmin * 2^s (with a small tolerance for round-off)
+  for (int step = 0; h->L.back().nN > cmax && h->L.size() < 32 && step < 64; ++step) {
     Level& f = h->L.back();
     if (f.ax[0].n < 2 && f.ax[1].n < 2 && f.ax[2].n < 2) break;
+    const double target = hmin * std::ldexp(1.0, step) * (1.0 + 1e-9);
     Level c;
-    if ((rc = make_transfer(f, c))) return bail(rc);
+    std::vector<int> fidx[3];
+    bool changed = false;
+    for (int a = 0; a < 3; ++a) {
+      coarsen_axis(f.ax[a], c.ax[a], fidx[a], target);
+      changed |= c.ax[a].n != f.ax[a].n;
+    }
+    if (!changed) continue;
+    if ((rc = make_transfer(f, c, fidx))) return bail(rc);
     h->L.push_back(c);
   }
   if (h->L.back().nN > 4096) return bail(fail(SDC_ERR_ARG, "coarsest multigrid level too large"));
@@ -938,6 +964,8 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
   if (o->mode != SDC_MODE_MIXED && o->mode != SDC_MODE_FP64 && o->mode != SDC_MODE_MIXED_IR)
     return fail(SDC_ERR_ARG, "mode must be SDC_MODE_MIXED, SDC_MODE_MIXED_IR or SDC_MODE_FP64");
   if (!(o->tol > 0) || o->max_iter < 1) return fail(SDC_ERR_ARG, "tol must be > 0, max_iter >= 1");
+  if (((uintptr_t)u & 31u) || ((uintptr_t)q & 31u))
+    return fail(SDC_ERR_ARG, "q and u must be 32-byte aligned");
   if (o->mode == SDC_MODE_MIXED_IR && (!(o->inner_tol > 0) || o->max_outer < 1))
     return fail(SDC_ERR_ARG, "mixed mode needs inner_tol > 0 and max_outer >= 1");
   DeviceGuard dg(h->device);

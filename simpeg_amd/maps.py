@@ -119,3 +119,27 @@ class ComboMap(IdentityMap):
     @property
     def is_linear(self):
         return all(getattr(m, "is_linear", False) for m in self.maps)
+
+
+class Projection(IdentityMap):
+    """Select entries ``index`` of an nP-long model (maps/_base.py ``Projection``)."""
+
+    def __init__(self, nP, index, **kwargs):
+        super().__init__(nP=nP)
+        self.index = np.atleast_1d(np.arange(nP)[index] if isinstance(index, slice) else index)
+        self._P = sp.csr_matrix((np.ones(self.index.size), (np.arange(self.index.size), self.index)),
+                                shape=(self.index.size, nP))
+
+    @property
+    def shape(self):
+        return (self.index.size, self.nP)
+
+    @property
+    def is_linear(self):
+        return True
+
+    def _transform(self, m):
+        return np.asarray(m, dtype=float).ravel()[self.index]
+
+    def deriv(self, m, v=None):
+        return self._P if v is None else self._P @ v

@@ -28,7 +28,8 @@ class Fields3DNodal:
     def __init__(self, simulation, u_block=None):
         self.simulation = simulation
         self.mesh = simulation.mesh
-        self.survey = simulation.survey
+        # FieldsDC.survey: the mini survey when the simulation is miniaturised (fields.py:13-17)
+        self.survey = simulation._sim_survey
         self.u_block = u_block
         self._host = None
 

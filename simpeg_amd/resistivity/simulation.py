@@ -52,6 +52,9 @@ class Simulation3DNodal:
         per-source target of |q - A u| / |q| (fp64 residual).
     inner_rtol, max_outer, max_iter : refinement / PCG limits.
     gpu_params : dict of multigrid parameters (nu_pre, nu_post, cheb_ratio, ...).
+    autotune : bool
+        time the candidate multigrid/launch settings on the first ``fields`` call and keep the
+        fastest (cached per mesh shape in ~/.cache/simpeg_amd/autotune.json).
     """
 
     _solutionType = "phiSolution"
@@ -62,7 +65,8 @@ class Simulation3DNodal:
                  rhoMap=None, solver=None, solver_opts=None, storeJ=False, miniaturize=False,
                  surface_faces=None, sensitivity_path=None, counter=None, verbose=False,
                  model=None, *, precision="mixed", rtol=1e-10, inner_rtol=1e-5, max_outer=10,
-                 max_iter=1000, device=None, gpu_params=None, coarse_max=0, jchunk=512):
+                 max_iter=1000, device=None, gpu_params=None, coarse_max=0, jchunk=512,
+                 autotune=False):
         self.mesh = mesh
         self._widths, self._origin = mesh_widths(mesh)
         self._nC = int(np.prod([len(h) for h in self._widths]))
@@ -80,8 +84,7 @@ class Simulation3DNodal:
         if not isinstance(miniaturize, bool):
             raise TypeError("miniaturize must be a bool")
         self.miniaturize = miniaturize
-        if miniaturize:
-            raise NotImplementedError("miniaturize=True is not supported by the GPU simulation yet")
+        self._mini = None
         self.surface_faces = surface_faces
         self.solver = solver
         self.solver_opts = {} if solver_opts is None else dict(solver_opts)
@@ -108,6 +111,8 @@ class Simulation3DNodal:
         self._gpu_params = dict(gpu_params or {})
         self._coarse_max = int(coarse_max)
         self._engine = None
+        self.autotune = bool(autotune)
+        self._tuned = False
         self._model = None
         self._sigma_on_gpu = None
         self._Jmatrix = None
@@ -116,6 +121,10 @@ class Simulation3DNodal:
         self._q = None
         self._q_block = None
         self.last_solve = None
+        if miniaturize:
+            from .utils import _mini_pole_pole
+
+            self._dipoles, self._invs, self._mini = _mini_pole_pole(self.survey)
         if model is not None:
             self.model = model
 
@@ -140,6 +149,25 @@ class Simulation3DNodal:
     @property
     def bc_type(self):
         return self._bc_type
+
+    @property
+    def _sim_survey(self):
+        """The survey whose sources are solved (the pole-pole mini survey when miniaturised)."""
+        return self._mini if self._mini is not None else self.survey
+
+    def _mini_data(self, d):
+        if self._mini is None:
+            return d
+        from .utils import mini_survey_data
+
+        return mini_survey_data(self._dipoles, self._invs, d)
+
+    def _mini_dataT(self, v):
+        if self._mini is None:
+            return v
+        from .utils import mini_survey_dataT
+
+        return mini_survey_dataT(self._dipoles, self._invs, self._mini.nD, v)
 
     @property
     def _active_map(self):
@@ -234,7 +262,7 @@ class Simulation3DNodal:
         ld = eng.ld
         rows, flat, w, nodes, dsrc = [], [], [], [], []
         i0 = 0
-        for s, src in enumerate(self.survey.source_list):
+        for s, src in enumerate(self._sim_survey.source_list):
             for rx in src.receiver_list:
                 nd = rx.nD
                 factor = np.ones(nd)
@@ -272,7 +300,7 @@ class Simulation3DNodal:
     def getSourceTerm(self):
         """q (n_nodes, nSrc), Fortran ordered (resistivity/simulation.py:265-289)."""
         if self._q is None:
-            srcs = self.survey.source_list
+            srcs = self._sim_survey.source_list
             q = np.zeros((self._nN, len(srcs)), order="F")
             for i, src in enumerate(srcs):
                 q[:, i] = src.eval(self)
@@ -285,7 +313,7 @@ class Simulation3DNodal:
         if self._q_block is None:
             eng = self.engine
             dest, col, val = [], [], []
-            for i, src in enumerate(self.survey.source_list):
+            for i, src in enumerate(self._sim_survey.source_list):
                 locs, cur = src.electrodes()
                 idx, wt = node_interpolation(self._widths, self._origin, locs)
                 dest.append(i * eng.ld + idx.ravel())
@@ -296,7 +324,7 @@ class Simulation3DNodal:
             dest, val = dest[keep], val[keep]
             d, rp, sidx, sw = coo_to_scatter(dest, np.zeros(dest.size, np.int64), val)
             sc = eng.scatter_csr(d, rp, sidx, sw)
-            self._q_block = eng.scatter(sc, np.ones(1), self.survey.nSrc)
+            self._q_block = eng.scatter(sc, np.ones(1), self._sim_survey.nSrc)
         return self._q_block
 
     # ------------------------------------------------------------------ public API
@@ -306,7 +334,13 @@ class Simulation3DNodal:
             self.model = m
         self._ensure_sigma()
         q = self._source_block()
-        u = self._solve(q, self.survey.nSrc)
+        if self.autotune and not self._tuned:
+            from ..autotune import tune
+
+            tune(self.engine, q, self._sim_survey.nSrc,
+                 mode="fp64" if self.precision == "fp64" else "mixed", tol=self.rtol)
+            self._tuned = True
+        u = self._solve(q, self._sim_survey.nSrc)
         return Fields3DNodal(self, u)
 
     def dpred(self, m=None, f=None):
@@ -321,7 +355,7 @@ class Simulation3DNodal:
                 m = self.model
             f = self.fields(m)
         pr = self._projection()
-        return self.engine.gather(pr["gather"], f.u_block).cpu().numpy()
+        return self._mini_data(self.engine.gather(pr["gather"], f.u_block).cpu().numpy())
 
     def residual(self, m, dobs, f=None):
         return self.dpred(m, f=f) - np.asarray(dobs, dtype=float).ravel()
@@ -336,10 +370,10 @@ class Simulation3DNodal:
         self._ensure_sigma()
         dsig = self.sigmaDeriv @ np.asarray(v, dtype=float).ravel()
         eng = self.engine
-        nS = self.survey.nSrc
+        nS = self._sim_survey.nSrc
         rhs = eng.jvec_rhs(dsig, f.u_block, nS)
         du = self._solve(rhs, nS)
-        return eng.gather(self._projection()["gather"], du).cpu().numpy()
+        return self._mini_data(eng.gather(self._projection()["gather"], du).cpu().numpy())
 
     def Jtvec(self, m, v, f=None):
         """J^T v (``BaseDCSimulation.Jtvec``/``_Jtvec``, simulation.py:189-263)."""
@@ -353,10 +387,11 @@ class Simulation3DNodal:
             return np.asarray(self.getJ(m, f=f).T @ v)
         self._ensure_sigma()
         eng = self.engine
-        nS = self.survey.nSrc
+        nS = self._sim_survey.nSrc
         pr = self._projection()
-        if v.size != pr["nD"]:
-            raise ValueError(f"v must have {pr['nD']} entries, got {v.size}")
+        if v.size != self.survey.nD:
+            raise ValueError(f"v must have {self.survey.nD} entries, got {v.size}")
+        v = self._mini_dataT(v)
         rhs = eng.scatter(pr["scatter"], v, nS)
         lam = self._solve(rhs, nS)
         g = eng.contract(f.u_block, lam, nS).cpu().numpy()
@@ -385,7 +420,7 @@ class Simulation3DNodal:
                 lam = self._solve(rhs, k)
                 G = eng.contract(f.u_block, lam, k, ucol=pr["data_src"][c0:c1], per_column=True)
                 J[c0:c1] = -(sd.T @ G.cpu().numpy().T).T
-            self._Jmatrix = J
+            self._Jmatrix = self._mini_data(J)
         return self._Jmatrix
 
     def getJtJdiag(self, m, W=None, f=None):
@@ -399,7 +434,7 @@ class Simulation3DNodal:
             sd = self.sigmaDeriv
             diag_map = sd.shape[0] == sd.shape[1] and sd.nnz <= sd.shape[0] and np.all(
                 sd.tocoo().row == sd.tocoo().col)
-            if self._Jmatrix is not None or self.storeJ or not diag_map:
+            if self._Jmatrix is not None or self.storeJ or not diag_map or self._mini is not None:
                 J = self.getJ(m, f=f)
                 w = np.ones(J.shape[0]) if W is None else np.asarray(W.diagonal()) ** 2
                 self._gtgdiag = (w[:, None] * J * J).sum(axis=0)

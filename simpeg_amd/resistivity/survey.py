@@ -83,6 +83,49 @@ class Survey:
         return f"{self.__class__.__name__}(#sources: {self.nSrc}; #data: {self.nD})"
 
     # electrode geometry -------------------------------------------------------------------
+    def _abmn_locations(self):
+        """Per-datum A, B, M, N electrode locations (survey.py:196-237): a pole source repeats A
+        as B, a pole receiver repeats M as N."""
+        a, b, m, n = [], [], [], []
+        for src in self.source_list:
+            for rx in src.receiver_list:
+                k = rx.nD
+                if isinstance(src, Src.Pole):
+                    a.append(np.repeat(src.location.reshape(1, -1), k, axis=0))
+                    b.append(np.repeat(src.location.reshape(1, -1), k, axis=0))
+                elif isinstance(src, Src.Dipole):
+                    a.append(np.repeat(src.location[0].reshape(1, -1), k, axis=0))
+                    b.append(np.repeat(src.location[1].reshape(1, -1), k, axis=0))
+                else:
+                    raise NotImplementedError("A/B locations are undefined for multipole sources")
+                if isinstance(rx, Rx.Pole):
+                    m.append(rx.locations)
+                    n.append(rx.locations)
+                else:
+                    m.append(rx.locations[0])
+                    n.append(rx.locations[1])
+        return np.vstack(a), np.vstack(b), np.vstack(m), np.vstack(n)
+
+    @property
+    def locations_a(self):
+        return self._abmn_locations()[0]
+
+    @property
+    def locations_b(self):
+        return self._abmn_locations()[1]
+
+    @property
+    def locations_m(self):
+        return self._abmn_locations()[2]
+
+    @property
+    def locations_n(self):
+        return self._abmn_locations()[3]
+
+    @property
+    def unique_electrode_locations(self):
+        return np.unique(np.vstack(self._abmn_locations()), axis=0)
+
     def _abmn(self):
         a, b, m, n = [], [], [], []
         for src in self.source_list:

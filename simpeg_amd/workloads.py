@@ -157,3 +157,55 @@ def noisy_weights(dclean, seed=2, rel=0.03, floor=1e-5):
     rng = np.random.default_rng(seed)
     dobs = dclean + rel * np.abs(dclean) * rng.standard_normal(dclean.shape)
     return dobs, 1.0 / (rel * np.abs(dobs) + floor)
+
+
+def layered_ellipsoid_model(seed=0, core=(160, 160, 96), npad=8, nlines=16, nsrc_per_line=16,
+                            nrx=24):
+    """160x160x96 core of 10 m cells + 8 pads x1.3 (176x176x104 cells, 3,289,545 nodes);
+    log sigma: layers 0.005 / 0.02 / 0.002 S/m at depths 0-100 / 100-400 / 400+ m plus three
+    seeded ellipsoids of 0.2-0.5 S/m in the core; 16 lines x 16 dipole-dipole sources x 24 Rx
+    (256 sources, 6,144 data)."""
+    mesh = padded_mesh(core, npad=npad)
+    cc = mesh.cell_centers
+    depth = -cc[:, 2]
+    sig = np.where(depth < 100.0, 0.005, np.where(depth < 400.0, 0.02, 0.002))
+    rng = np.random.default_rng(seed)
+    half = np.array(core) * 10.0 / 2.0
+    for _ in range(3):
+        c = np.array([rng.uniform(-0.6, 0.6) * half[0], rng.uniform(-0.6, 0.6) * half[1],
+                      -rng.uniform(80.0, 0.6 * core[2] * 10.0)])
+        ax = rng.uniform(40.0, 150.0, size=3)
+        inside = (((cc - c) / ax) ** 2).sum(axis=1) <= 1.0
+        sig[inside] = rng.uniform(0.2, 0.5)
+    spacing = 20.0
+    length = (nsrc_per_line + 2 + nrx - 1) * spacing
+    survey = grid_survey(nlines, nsrc_per_line, nrx, spacing=spacing,
+                         y_span=min(1400.0, 2 * half[1] - 100.0))
+    assert length < 2 * half[0]
+    return Workload("layered_160x160x96_256src", mesh, survey, np.log(sig),
+                    {"core": core, "npad": npad, "seed": seed})
+
+
+def uneven_batch(seed=0, n_problems=48, sides=(24, 32, 48, 64, 80), npad=4, min_src=8,
+                 max_src=96):
+    """48 independent problems: cube cores with side drawn from `sides` (z extent half of x),
+    4 pads x1.3, 8-96 pole-dipole sources on random surface lines, log sigma = log 0.01 +
+    GRF(std 0.8, corr 40 m).  Returns a list of Workloads."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for p in range(n_problems):
+        side = int(rng.choice(sides))
+        mesh = padded_mesh((side, side, side // 2), npad=npad)
+        m = np.log(0.01) + gaussian_random_field(mesh, std=0.8, corr=40.0, seed=seed * 1000 + p)
+        nsrc = int(rng.integers(min_src, max_src + 1))
+        half = side * 10.0 / 2.0
+        src = []
+        nl = 0
+        while len(src) < nsrc and nl < 64:
+            y = float(np.round(rng.uniform(-0.8, 0.8) * half / 10.0) * 10.0)
+            line = line_sources("pole-dipole", [-0.9 * half, 0.9 * half, y, y], 10.0, 8)
+            src += line[: nsrc - len(src)]
+            nl += 1
+        out.append(Workload(f"batch{p}_{side}", mesh, Survey(src), m,
+                            {"side": side, "nsrc": len(src)}))
+    return out

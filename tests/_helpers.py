@@ -51,3 +51,19 @@ def tiny_problem(seed=0, core=(10, 8, 6), npad=3, nsrc=4):
     src = W.line_sources("dipole-dipole", [-half, half, 0.0, 0.0], 10.0, 3)[:nsrc]
     from simpeg_amd.resistivity.survey import Survey
     return mesh, Survey(src), m
+
+
+def mixed_survey():
+    """A dipole-dipole line plus pole-dipole, pole-pole and dipole-pole sources sharing
+    electrodes (the mixture of tests/em/static/test_DC_miniaturize.py:230-270)."""
+    from simpeg_amd import workloads as W
+    from simpeg_amd.resistivity import receivers as Rx
+    from simpeg_amd.resistivity import sources as Src
+    from simpeg_amd.resistivity.survey import Survey
+
+    srcs = W.line_sources("dipole-dipole", [-40.0, 40.0, 0.0, 0.0], 10.0, 3)
+    e = Survey(srcs).unique_electrode_locations
+    rx_p = Rx.Pole(e[[2]])
+    rx_d = Rx.Dipole(e[[2]], e[[3]])
+    srcs = srcs + [Src.Pole([rx_d], e[0]), Src.Pole([rx_p], e[0]), Src.Dipole([rx_p], e[0], e[1])]
+    return Survey(srcs)

@@ -179,3 +179,66 @@ def test_mode_switching_on_one_handle(gpu, tiny):
         assert sim.last_solve.converged, mode
         for s in range(survey.nSrc):
             assert rel(f[:, "phiSolution"][:, s], u_ref[:, s]) < 1e-7, mode
+
+
+def test_miniaturize_equivalence(gpu, tiny):
+    """tests/em/static/test_DC_miniaturize.py:285-312: miniaturize=True gives the same dpred,
+    Jvec, Jtvec and J as the plain survey."""
+    from tests._helpers import mixed_survey
+
+    mesh, _, m = tiny
+    survey = mixed_survey()
+    s1 = _sim(mesh, survey)
+    s2 = _sim(mesh, survey, miniaturize=True)
+    assert s2._sim_survey.nSrc != survey.nSrc
+    f1, f2 = s1.fields(m), s2.fields(m)
+    assert np.allclose(s1.dpred(m, f=f1), s2.dpred(m, f=f2), rtol=1e-7, atol=1e-12)
+    rng = np.random.default_rng(42)
+    u = rng.uniform(size=m.shape)
+    v = rng.uniform(size=survey.nD)
+    assert rel(s2.Jvec(m, u, f=f2), s1.Jvec(m, u, f=f1)) < 1e-6
+    assert rel(s2.Jtvec(m, v, f=f2), s1.Jtvec(m, v, f=f1)) < 1e-6
+    assert rel(s2.getJ(m, f=f2), s1.getJ(m, f=f1)) < 1e-6
+    orc = oracle_for(mesh, survey)
+    assert rel(s2.dpred(m, f=f2), orc.dpred(m)) < 1e-6
+
+
+def test_ip_nodal_is_scaled_dc_sensitivity(gpu, tiny):
+    """IP data d = s * J_dc(sigma) eta with sigmaDeriv = -diag(sigma) etaDeriv
+    (induced_polarization/simulation.py:37-39, 117-125); checked against the oracle's DC Jvec."""
+    from simpeg_amd import maps
+    from simpeg_amd.induced_polarization import Simulation3DNodal as IPNodal
+
+    mesh, survey, m = tiny
+    sigma = np.exp(m)
+    rng = np.random.default_rng(8)
+    eta = 0.05 + 0.02 * rng.random(mesh.n_cells)
+    ip = IPNodal(mesh, survey=survey, sigma=sigma, etaMap=maps.IdentityMap(mesh))
+    d = ip.dpred(eta)
+    orc = oracle_for(mesh, survey, mapping="identity")
+    assert rel(d, orc.Jvec(sigma, -sigma * eta)) < 1e-6
+    v = rng.standard_normal(mesh.n_cells)
+    w = rng.standard_normal(survey.nD)
+    lhs = w @ ip.Jvec(eta, v)
+    rhs = v @ ip.Jtvec(eta, w)
+    assert abs(lhs - rhs) < 1e-8 * abs(lhs)
+    assert rel(ip.Jtvec(eta, w), orc.Jtvec(sigma, w) * -sigma) < 1e-6
+
+
+def test_autotuner_picks_converging_parameters(gpu, tiny, tmp_path):
+    from simpeg_amd.autotune import tune
+
+    mesh, survey, m = tiny
+    sim = _sim(mesh, survey)
+    sim.model = m
+    sim._ensure_sigma()
+    q = sim._source_block()
+    cache = str(tmp_path / "tune.json")
+    best, timings = tune(sim.engine, q, survey.nSrc, cache_path=cache)
+    assert len(timings) >= 3 and all(t["seconds"] > 0 for t in timings)
+    best2, _ = tune(sim.engine, q, survey.nSrc, cache_path=cache)
+    assert best2 == best
+    orc = oracle_for(mesh, survey)
+    u_ref = orc.fields(m)
+    f = sim.fields(m)
+    assert rel(f[:, "phiSolution"][:, 0], u_ref[:, 0]) < 1e-7

@@ -158,3 +158,26 @@ def test_grf_statistics():
     assert 0.5 < f.std() < 1.5
     f2 = workloads.gaussian_random_field(mesh, std=1.0, corr=60.0, seed=3)
     assert np.array_equal(f, f2)
+
+
+def test_miniaturize_recombination_with_oracle():
+    """d = AM - AN - BM + BN from the unique pole-pole pairs reproduces the full survey."""
+    from simpeg_amd.resistivity.utils import _mini_pole_pole, mini_survey_data, mini_survey_dataT
+    from tests._helpers import mixed_survey, oracle_for
+
+    mesh, _, m = tiny_problem()
+    survey = mixed_survey()
+    dipoles, invs, mini = _mini_pole_pole(survey)
+    assert mini.nSrc < survey.nSrc + 10 and all(isinstance(s, Src.Pole) for s in mini.source_list)
+    full = oracle_for(mesh, survey)
+    small = oracle_for(mesh, mini)
+    d_full = full.dpred(m)
+    d_mini = small.dpred(m)
+    assert np.allclose(mini_survey_data(dipoles, invs, d_mini), d_full, rtol=1e-10, atol=1e-14)
+    # adjoint of the recombination
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(mini.nD)
+    b = rng.standard_normal(survey.nD)
+    lhs = b @ mini_survey_data(dipoles, invs, a)
+    rhs = a @ mini_survey_dataT(dipoles, invs, mini.nD, b)
+    assert np.isclose(lhs, rhs, rtol=1e-12)
