@@ -81,6 +81,8 @@ typedef struct {
   int32_t check_every;         /* PCG iterations between host convergence polls */
   double reliable_delta;       /* mixed mode: replace the fp32 residual by the fp64 true residual
                                   once it dropped by this factor (0 < delta < 1) */
+  int32_t xfer_rhs_per_thread; /* grid transfers: 0 = as rhs_per_thread, else 1, 2, 4 or 8 */
+  int32_t xcd_remap;           /* 1: give each XCD a contiguous range of work-groups, 0: off */
 } sdc_params;
 
 typedef struct {

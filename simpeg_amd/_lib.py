@@ -54,6 +54,8 @@ class Params(C.Structure):
         ("cheb_lmax", C.c_double), ("cheb_ratio", C.c_double),
         ("rhs_per_thread", C.c_int32), ("check_every", C.c_int32),
         ("reliable_delta", C.c_double),
+        ("xfer_rhs_per_thread", C.c_int32),
+        ("xcd_remap", C.c_int32),
     ]
 
 

@@ -58,12 +58,15 @@ struct Sten {
   uint32_t n1x, n1y, n1z, nN, plane;
   FastDiv fx, fy;
   int pin0;
+  int remap;  // 1: XCD-contiguous work-group order
 };
 
+// Per-node stencil data.  The neighbour indices of a missing neighbour (domain edge) are the
+// node itself and its weight is 0, so stencil loads need no branches and stay in bounds.
 template <typename T>
 struct NodeC {
   T wxm, wxp, wym, wyp, wzm, wzp, d;
-  bool hxm, hxp, hym, hyp, hzm, hzp;
+  uint32_t ixm, ixp, iym, iyp, izm, izp;
 };
 
 template <typename T>
@@ -73,26 +76,33 @@ __device__ __forceinline__ NodeC<T> node_coef(const Sten<T>& s, uint32_t n) {
   const uint32_t i = n - r * s.n1x;
   const uint32_t k = fdiv(r, s.fy);
   const uint32_t j = r - k * s.n1y;
-  c.hxm = i > 0;
-  c.hxp = i + 1 < s.n1x;
-  c.hym = j > 0;
-  c.hyp = j + 1 < s.n1y;
-  c.hzm = k > 0;
-  c.hzp = k + 1 < s.n1z;
-  c.wxp = s.cx[n];
-  c.wyp = s.cy[n];
-  c.wzp = s.cz[n];
-  c.wxm = c.hxm ? s.cx[n - 1] : T(0);
-  c.wym = c.hym ? s.cy[n - s.n1x] : T(0);
-  c.wzm = c.hzm ? s.cz[n - s.plane] : T(0);
+  const bool hxm = i > 0, hxp = i + 1 < s.n1x;
+  const bool hym = j > 0, hyp = j + 1 < s.n1y;
+  const bool hzm = k > 0, hzp = k + 1 < s.n1z;
+  c.ixm = hxm ? n - 1 : n;
+  c.ixp = hxp ? n + 1 : n;
+  c.iym = hym ? n - s.n1x : n;
+  c.iyp = hyp ? n + s.n1x : n;
+  c.izm = hzm ? n - s.plane : n;
+  c.izp = hzp ? n + s.plane : n;
+  const T ax = s.cx[n], ay = s.cy[n], az = s.cz[n];
+  const T bx = s.cx[c.ixm], by = s.cy[c.iym], bz = s.cz[c.izm];
+  c.wxp = hxp ? ax : T(0);
+  c.wyp = hyp ? ay : T(0);
+  c.wzp = hzp ? az : T(0);
+  c.wxm = hxm ? bx : T(0);
+  c.wym = hym ? by : T(0);
+  c.wzm = hzm ? bz : T(0);
   c.d = s.dg[n];
   if (s.pin0) {
-    if (n == 0) {
-      c.wxp = c.wyp = c.wzp = T(0);
-    } else if (s.pin0 == 1) {
-      if (n == 1) c.wxm = T(0);
-      if (n == s.n1x) c.wym = T(0);
-      if (n == s.plane) c.wzm = T(0);
+    const bool z0 = n == 0;
+    c.wxp = z0 ? T(0) : c.wxp;
+    c.wyp = z0 ? T(0) : c.wyp;
+    c.wzp = z0 ? T(0) : c.wzp;
+    if (s.pin0 == 1) {
+      c.wxm = (n == 1) ? T(0) : c.wxm;
+      c.wym = (n == s.n1x) ? T(0) : c.wym;
+      c.wzm = (n == s.plane) ? T(0) : c.wzm;
     }
   }
   return c;
@@ -112,36 +122,36 @@ __device__ __forceinline__ void st_once(T* p, T v) {
 
 // (A x)_n for one column (x points at the column base).
 template <typename T>
-__device__ __forceinline__ T stencil(const NodeC<T>& c, const T* __restrict__ x, uint32_t n,
-                                     uint32_t n1x, uint32_t plane) {
-  T v = c.d * x[n];
-  if (c.hxm) v -= c.wxm * x[n - 1];
-  if (c.hxp) v -= c.wxp * x[n + 1];
-  if (c.hym) v -= c.wym * x[n - n1x];
-  if (c.hyp) v -= c.wyp * x[n + n1x];
-  if (c.hzm) v -= c.wzm * x[n - plane];
-  if (c.hzp) v -= c.wzp * x[n + plane];
-  return v;
+__device__ __forceinline__ T stencil(const NodeC<T>& c, const T* __restrict__ x, uint32_t n) {
+  const T xc = x[n], xw = x[c.ixm], xe = x[c.ixp], xs = x[c.iym], xn = x[c.iyp], xb = x[c.izm],
+          xt = x[c.izp];
+  return c.d * xc - c.wxm * xw - c.wxp * xe - c.wym * xs - c.wyp * xn - c.wzm * xb - c.wzp * xt;
 }
 
-// (A x)_n for R columns starting at column c0 (columns >= ncol are skipped).  All loads are
-// issued before the arithmetic so that a wavefront keeps 7R requests in flight.
+// Column c0 + r of a block, clamped to the last column: loads of columns past ncol read valid
+// memory and their results are discarded, so no load needs a branch.
+template <typename T>
+__device__ __forceinline__ const T* col_ptr(const T* x, size_t ld, int c0, int r, int ncol) {
+  const int c = c0 + r < ncol ? c0 + r : ncol - 1;
+  return x + (size_t)c * ld;
+}
+
+// (A x)_n for R columns starting at column c0.  All 7R loads are issued before the arithmetic
+// so that a wavefront keeps them in flight; the per-lane indices are shared by every column.
 template <typename T, int R>
 __device__ __forceinline__ void stencil_cols(const NodeC<T>& c, const T* __restrict__ x, uint32_t n,
-                                             size_t ld, int c0, int ncol, uint32_t n1x,
-                                             uint32_t plane, T (&xc)[R], T (&v)[R]) {
+                                             size_t ld, int c0, int ncol, T (&xc)[R], T (&v)[R]) {
   T xw[R], xe[R], xs[R], xn[R], xb[R], xt[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const bool ok = c0 + r < ncol;
-    const T* col = x + (size_t)(c0 + r) * ld;
-    xc[r] = ok ? col[n] : T(0);
-    xw[r] = (ok && c.hxm) ? col[n - 1] : T(0);
-    xe[r] = (ok && c.hxp) ? col[n + 1] : T(0);
-    xs[r] = (ok && c.hym) ? col[n - n1x] : T(0);
-    xn[r] = (ok && c.hyp) ? col[n + n1x] : T(0);
-    xb[r] = (ok && c.hzm) ? col[n - plane] : T(0);
-    xt[r] = (ok && c.hzp) ? col[n + plane] : T(0);
+    const T* col = col_ptr(x, ld, c0, r, ncol);
+    xc[r] = col[n];
+    xw[r] = col[c.ixm];
+    xe[r] = col[c.ixp];
+    xs[r] = col[c.iym];
+    xn[r] = col[c.iyp];
+    xb[r] = col[c.izm];
+    xt[r] = col[c.izp];
   }
 #pragma unroll
   for (int r = 0; r < R; ++r)
@@ -151,18 +161,36 @@ __device__ __forceinline__ void stencil_cols(const NodeC<T>& c, const T* __restr
 
 // Sum R per-thread doubles over the work-group and write one partial per
 // column: partial[(c0 + r) * nbx + bx].
+// Inside a wavefront the R sums share one butterfly: at each of the first log2(R) xor steps a
+// lane keeps half of its values and trades the other half with its partner, so R columns cost
+// R - 1 + 6 - log2(R) exchanges instead of 6R.  Afterwards every lane holds the wave sum of
+// column (lane >> (6 - log2 R)).
 template <int R>
 __device__ __forceinline__ void block_partials(double (&acc)[R], double* __restrict__ partial,
                                                int c0, int ncol, uint32_t nbx, uint32_t bx) {
+  constexpr int LOG_R = R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
   __shared__ double red[R][SDC_BLOCK / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double a[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    double v = acc[r];
+  for (int r = 0; r < R; ++r) a[r] = acc[r];
+  int mask = 32;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) red[r][wid] = v;
+  for (int w = R; w > 1; w >>= 1) {
+    const int half = w >> 1;
+    const bool hi = (lane & mask) != 0;
+#pragma unroll
+    for (int k = 0; k < half; ++k) {
+      const double mine = hi ? a[half + k] : a[k];
+      const double other = hi ? a[k] : a[half + k];
+      a[k] = mine + __shfl_xor(other, mask, 64);
+    }
+    mask >>= 1;
   }
+#pragma unroll
+  for (int m = 32 >> LOG_R; m > 0; m >>= 1) a[0] += __shfl_xor(a[0], m, 64);
+  constexpr int GROUP = 64 / R;
+  if ((lane & (GROUP - 1)) == 0) red[lane / GROUP][wid] = a[0];
   __syncthreads();
   if (threadIdx.x < R) {
     const int r = threadIdx.x;

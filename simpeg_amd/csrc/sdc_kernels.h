@@ -15,14 +15,17 @@
 
 // The linear work-group id (x fastest, as the dispatcher numbers them) is remapped so that
 // every XCD works through one contiguous range of (column group, node block) pairs.
-#define SDC_NODE_PROLOGUE(NN)                                                   \
+#define SDC_NODE_PROLOGUE_R(NN, REMAP)                                          \
   const uint32_t nbx = gridDim.x;                                               \
-  const uint32_t lin_ = xcd_remap(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y); \
+  const uint32_t lin0_ = blockIdx.x + nbx * blockIdx.y;                         \
+  const uint32_t lin_ = (REMAP) ? xcd_remap(lin0_, nbx * gridDim.y) : lin0_;    \
   const uint32_t by_ = lin_ / nbx;                                              \
   const uint32_t bx = lin_ - by_ * nbx;                                         \
   const uint32_t n = bx * SDC_BLOCK + threadIdx.x;                              \
   const int c0 = (int)by_ * R;                                                  \
   const bool live = n < (NN);
+
+#define SDC_NODE_PROLOGUE(NN) SDC_NODE_PROLOGUE_R(NN, 1)
 
 // ------------------------------------------------------------------------------------------
 // operator application: y = A x, optional per-column partial sums of x.y
@@ -33,14 +36,14 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> s, const T* __restr
                                                      double* __restrict__ partial,
                                                      const int* __restrict__ done) {
   if (done && *done) return;
-  SDC_NODE_PROLOGUE(s.nN)
+  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
   if (live) {
     const NodeC<T> c = node_coef(s, n);
     T xc[R], v[R];
-    stencil_cols<T, R>(c, x, n, ld, c0, ncol, s.n1x, s.plane, xc, v);
+    stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, v);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (c0 + r < ncol) {
@@ -59,13 +62,13 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const T* __re
                                                         T* __restrict__ res, size_t ld, int ncol,
                                                         const int* __restrict__ done) {
   if (done && *done) return;
-  SDC_NODE_PROLOGUE(s.nN)
+  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   if (!live) return;
   const NodeC<T> c = node_coef(s, n);
   T xc[R], v[R], bv[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) bv[r] = (c0 + r < ncol) ? ld_once(b + (size_t)(c0 + r) * ld + n) : T(0);
-  stencil_cols<T, R>(c, x, n, ld, c0, ncol, s.n1x, s.plane, xc, v);
+  for (int r = 0; r < R; ++r) bv[r] = ld_once(col_ptr(b, ld, c0, r, ncol) + n);
+  stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, v);
 #pragma unroll
   for (int r = 0; r < R; ++r)
     if (c0 + r < ncol) st_once(res + (size_t)(c0 + r) * ld + n, bv[r] - v[r]);
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const T* __res
                                                        T* __restrict__ x, size_t ld, int ncol,
                                                        T coef, const int* __restrict__ done) {
   if (done && *done) return;
-  SDC_NODE_PROLOGUE(s.nN)
+  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   if (!live) return;
   const T w = coef * s.idg[n];
 #pragma unroll
@@ -89,42 +92,42 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const T* __res
   }
 }
 
-// Chebyshev step: res = b - A xin; dn = c1*d + c2*D^-1 res; xout = xin + dn; dout = dn.
-// USE_D = false drops the c1*d term (first step from a non-zero guess).
+// Chebyshev step in three-term form:  xout = x + c1 (x - xprev) + c2 D^-1 (b - A x).
+//   PREV = 0: first step from a non-zero guess (no c1 term);
+//   PREV = 1: xprev read pointwise; xout may be the same buffer (each work-item reads its
+//             xprev entries before it writes xout, and nobody else reads them);
+//   PREV = 2: xprev = cz D^-1 b, the first iterate from a zero guess, recomputed here.
 // DOT adds per-column partial sums of b.xout (the r.z product of PCG).
-template <typename T, int R, bool USE_D, bool DOT>
-__global__ __launch_bounds__(SDC_BLOCK) void k_smooth(Sten<T> s, const T* __restrict__ b,
-                                                      const T* __restrict__ xin,
-                                                      T* __restrict__ xout, const T* dsrc,
-                                                      T* ddst, size_t ld, int ncol, T c1, T c2,
-                                                      double* __restrict__ partial,
-                                                      const int* __restrict__ done) {
+template <typename T, int R, int PREV, bool DOT>
+__global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const T* __restrict__ b,
+                                                    const T* __restrict__ x, const T* xprev,
+                                                    T* xout, size_t ld, int ncol, T c1, T c2,
+                                                    T cz, double* __restrict__ partial,
+                                                    const int* __restrict__ done) {
   if (done && *done) return;
-  SDC_NODE_PROLOGUE(s.nN)
+  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
   if (live) {
     const NodeC<T> c = node_coef(s, n);
-    const T w = c2 * s.idg[n];
-    T xc[R], v[R], bv[R], dv[R];
+    const T g = s.idg[n];
+    const T w = c2 * g;
+    T xc[R], v[R], bv[R], pv[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const size_t o = (size_t)(c0 + r) * ld + n;
-      const bool ok = c0 + r < ncol;
-      bv[r] = ok ? ld_once(b + o) : T(0);
-      dv[r] = (USE_D && ok) ? ld_once(dsrc + o) : T(0);
+      bv[r] = ld_once(col_ptr(b, ld, c0, r, ncol) + n);
+      pv[r] = (PREV == 1) ? col_ptr(xprev, ld, c0, r, ncol)[n] : T(0);
     }
-    stencil_cols<T, R>(c, xin, n, ld, c0, ncol, s.n1x, s.plane, xc, v);
+    stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, v);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (c0 + r < ncol) {
         const size_t o = (size_t)(c0 + r) * ld + n;
-        T dn = w * (bv[r] - v[r]);
-        if (USE_D) dn += c1 * dv[r];
-        const T xn = xc[r] + dn;
+        T xn = xc[r] + w * (bv[r] - v[r]);
+        if (PREV == 1) xn += c1 * (xc[r] - pv[r]);
+        if (PREV == 2) xn += c1 * (xc[r] - cz * g * bv[r]);
         st_once(xout + o, xn);
-        st_once(ddst + o, dn);
         if (DOT) acc[r] += (double)bv[r] * (double)xn;
       }
     }
@@ -134,37 +137,34 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth(Sten<T> s, const T* __rest
 
 // The first two Chebyshev steps from a zero guess in one pass:
 //   x1 = c0 D^-1 b (recomputed at the 7 stencil points, never stored), r1 = b - A x1,
-//   d1 = c1 x1 + c2 D^-1 r1, x2 = x1 + d1.  Writes x2 and d1.
+//   x2 = x1 + c1 x1 + c2 D^-1 r1.  Writes x2 only (a following step recomputes x1 from b).
 template <typename T, int R>
 __global__ __launch_bounds__(SDC_BLOCK) void k_smooth_pair(Sten<T> s, const T* __restrict__ b,
-                                                           T* __restrict__ xout,
-                                                           T* __restrict__ dout, size_t ld,
+                                                           T* __restrict__ xout, size_t ld,
                                                            int ncol, T cz, T c1, T c2,
                                                            const int* __restrict__ done) {
   if (done && *done) return;
-  SDC_NODE_PROLOGUE(s.nN)
+  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   if (!live) return;
   NodeC<T> c = node_coef(s, n);
   const T g = s.idg[n];
   // A x1 = c0 * A' b with the neighbour weights scaled by their 1/diagonal
   c.d *= g;
-  c.wxm *= c.hxm ? s.idg[n - 1] : T(0);
-  c.wxp *= c.hxp ? s.idg[n + 1] : T(0);
-  c.wym *= c.hym ? s.idg[n - s.n1x] : T(0);
-  c.wyp *= c.hyp ? s.idg[n + s.n1x] : T(0);
-  c.wzm *= c.hzm ? s.idg[n - s.plane] : T(0);
-  c.wzp *= c.hzp ? s.idg[n + s.plane] : T(0);
+  c.wxm *= s.idg[c.ixm];
+  c.wxp *= s.idg[c.ixp];
+  c.wym *= s.idg[c.iym];
+  c.wyp *= s.idg[c.iyp];
+  c.wzm *= s.idg[c.izm];
+  c.wzp *= s.idg[c.izp];
   T bc[R], ab[R];
-  stencil_cols<T, R>(c, b, n, ld, c0, ncol, s.n1x, s.plane, bc, ab);
+  stencil_cols<T, R>(c, b, n, ld, c0, ncol, bc, ab);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (c0 + r < ncol) {
       const size_t o = (size_t)(c0 + r) * ld + n;
       const T x1 = cz * g * bc[r];
       const T r1 = bc[r] - cz * ab[r];
-      const T d1 = c1 * x1 + c2 * g * r1;
-      st_once(xout + o, x1 + d1);
-      st_once(dout + o, d1);
+      st_once(xout + o, x1 + c1 * x1 + c2 * g * r1);
     }
   }
 }
@@ -196,6 +196,7 @@ struct Xfer {
   uint32_t fn1x, fn1y, fplane, fN;
   uint32_t cn1x, cn1y, cplane, cN;
   FastDiv ffx, ffy, cfx, cfy;
+  int remap;
 };
 
 // rc = P^T rf: every coarse node gathers the 3x3x3 fine nodes around its fine position with
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __re
                                                         T* __restrict__ rc, size_t ldf, size_t ldc,
                                                         int ncol, const int* __restrict__ done) {
   if (done && *done) return;
-  SDC_NODE_PROLOGUE(t.cN)
+  SDC_NODE_PROLOGUE_R(t.cN, t.remap)
   if (!live) return;
   const uint32_t rr = fdiv(n, t.cfx);
   const uint32_t I = n - rr * t.cn1x;
@@ -228,15 +229,15 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __re
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       const T wyz = wy[b] * wz[c];
-      const size_t row = (size_t)yj[b] * t.fn1x + (size_t)zk[c] * t.fplane;
+      const uint32_t row = (uint32_t)yj[b] * t.fn1x + (uint32_t)zk[c] * t.fplane;
+      const uint32_t i0 = row + xi[0], i1 = row + xi[1], i2 = row + xi[2];
       T v[R][3];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const bool ok = c0 + r < ncol;
-        const T* f = rf + (size_t)(c0 + r) * ldf + row;
-        v[r][0] = ok ? f[xi[0]] : T(0);
-        v[r][1] = ok ? f[xi[1]] : T(0);
-        v[r][2] = ok ? f[xi[2]] : T(0);
+        const T* f = col_ptr(rf, ldf, c0, r, ncol);
+        v[r][0] = f[i0];
+        v[r][1] = f[i1];
+        v[r][2] = f[i2];
       }
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -256,29 +257,28 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const T* _
                                                            size_t ldc, int ncol,
                                                            const int* __restrict__ done) {
   if (done && *done) return;
-  SDC_NODE_PROLOGUE(t.fN)
+  SDC_NODE_PROLOGUE_R(t.fN, t.remap)
   if (!live) return;
   const uint32_t rr = fdiv(n, t.ffx);
   const uint32_t i = n - rr * t.fn1x;
   const uint32_t k = fdiv(rr, t.ffy);
   const uint32_t j = rr - k * t.fn1y;
   const T ux = t.pwx[i], uy = t.pwy[j], uz = t.pwz[k];
-  const size_t c000 = (size_t)t.pix[i] + (size_t)t.piy[j] * t.cn1x + (size_t)t.piz[k] * t.cplane;
-  const size_t sy = t.cn1x, sz = t.cplane;
+  const uint32_t c000 = (uint32_t)t.pix[i] + (uint32_t)t.piy[j] * t.cn1x + (uint32_t)t.piz[k] * t.cplane;
+  const uint32_t c010 = c000 + t.cn1x, c001 = c000 + t.cplane, c011 = c001 + t.cn1x;
   T cv[R][8], fv[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const bool ok = c0 + r < ncol;
-    const T* c = xc + (size_t)(c0 + r) * ldc + c000;
-    cv[r][0] = ok ? c[0] : T(0);
-    cv[r][1] = ok ? c[1] : T(0);
-    cv[r][2] = ok ? c[sy] : T(0);
-    cv[r][3] = ok ? c[sy + 1] : T(0);
-    cv[r][4] = ok ? c[sz] : T(0);
-    cv[r][5] = ok ? c[sz + 1] : T(0);
-    cv[r][6] = ok ? c[sz + sy] : T(0);
-    cv[r][7] = ok ? c[sz + sy + 1] : T(0);
-    fv[r] = ok ? xf[(size_t)(c0 + r) * ldf + n] : T(0);
+    const T* c = col_ptr(xc, ldc, c0, r, ncol);
+    cv[r][0] = c[c000];
+    cv[r][1] = c[c000 + 1];
+    cv[r][2] = c[c010];
+    cv[r][3] = c[c010 + 1];
+    cv[r][4] = c[c001];
+    cv[r][5] = c[c001 + 1];
+    cv[r][6] = c[c011];
+    cv[r][7] = c[c011 + 1];
+    fv[r] = col_ptr((const T*)xf, ldf, c0, r, ncol)[n];
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -327,10 +327,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_dot(const T* __restrict__ a, cons
   if (live) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (c0 + r < ncol) {
-        const size_t o = (size_t)(c0 + r) * ld + n;
-        acc[r] = (double)a[o] * (double)b[o];
-      }
+      if (c0 + r < ncol) acc[r] = (double)col_ptr(a, ld, c0, r, ncol)[n] * (double)col_ptr(b, ld, c0, r, ncol)[n];
   }
   block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
@@ -621,13 +618,17 @@ struct PinFix {
 // with U_ZERO the u loads are skipped (first refinement step).  The rows next to node 0 are
 // shifted into the symmetric (identity row and column 0) form the inner PCG solves.
 // Partial sums of |r|^2 (before the shift) per column.
+// qmask (per node, may be null): with U_ZERO it is set to 1 on every node where some column of
+// q is non-zero (the caller clears it first); otherwise q is read only on flagged nodes.  A
+// right-hand side of point sources touches a few nodes, so the fp64 residual then streams u.
 template <typename TO, int R, bool U_ZERO>
 __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const double* __restrict__ q,
                                                            const double* __restrict__ u,
                                                            TO* __restrict__ rout, size_t ld,
                                                            int ncol, PinFix pf,
-                                                           double* __restrict__ partial) {
-  SDC_NODE_PROLOGUE(s.nN)
+                                                           double* __restrict__ partial,
+                                                           unsigned* __restrict__ qmask) {
+  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -640,12 +641,15 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
       else if (n == pf.n1x) wfix = pf.cy[0];
       else if (n == pf.plane) wfix = pf.cz[0];
     }
+    const bool readq = U_ZERO || qmask == nullptr || qmask[n] != 0u;
+    bool anyq = false;
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (c0 + r < ncol) {
         const size_t o = (size_t)(c0 + r) * ld;
-        double rv = q[o + n];
-        if (!U_ZERO) rv -= stencil(c, u + o, n, s.n1x, s.plane);
+        double rv = readq ? q[o + n] : 0.0;
+        anyq |= rv != 0.0;
+        if (!U_ZERO) rv -= stencil(c, u + o, n);
         acc[r] = rv * rv;
         if (wfix != 0.0) {
           const double r0 = U_ZERO ? q[o] : q[o] - u[o];
@@ -653,6 +657,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
         }
         rout[o + n] = (TO)rv;
       }
+    // every writer stores the same value; the mask is cleared before this pass
+    if (U_ZERO && qmask != nullptr && anyq) qmask[n] = 1u;
   }
   block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
@@ -948,12 +954,12 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_dense_build(Sten<double> s, doubl
   for (size_t m = 0; m < N; ++m) row[m] = 0.0;
   const NodeC<double> c = node_coef(s, n);
   row[n] = c.d;
-  if (c.hxm) row[n - 1] = -c.wxm;
-  if (c.hxp) row[n + 1] = -c.wxp;
-  if (c.hym) row[n - s.n1x] = -c.wym;
-  if (c.hyp) row[n + s.n1x] = -c.wyp;
-  if (c.hzm) row[n - s.plane] = -c.wzm;
-  if (c.hzp) row[n + s.plane] = -c.wzp;
+  if (c.ixm != n) row[c.ixm] = -c.wxm;
+  if (c.ixp != n) row[c.ixp] = -c.wxp;
+  if (c.iym != n) row[c.iym] = -c.wym;
+  if (c.iyp != n) row[c.iyp] = -c.wyp;
+  if (c.izm != n) row[c.izm] = -c.wzm;
+  if (c.izp != n) row[c.izp] = -c.wzp;
 }
 
 // ainv64 <- (ainv64 + ainv64^T)/2 and its fp32 copy

@@ -119,7 +119,7 @@ struct Work {
   // level 0: PCG vectors and smoother scratch
   T *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   // per level smoother / V-cycle vectors (index 0 uses r as rhs and z as solution)
-  std::vector<T*> b, xs, xa, d, res;
+  std::vector<T*> b, xs, xa, res;
   double* partial = nullptr;
   double *alpha = nullptr, *beta = nullptr, *rz = nullptr, *rr = nullptr, *bn2 = nullptr,
          *aux = nullptr;
@@ -129,17 +129,18 @@ struct Work {
   double* orr = nullptr;
   int* omask = nullptr;
   int* ocount = nullptr;
+  // level 0 nodes where some right-hand-side column is non-zero (fp64 residual skips q elsewhere)
+  unsigned* qmask = nullptr;
   void release() {
     dfree(x); dfree(r); dfree(z); dfree(p); dfree(q);
     for (auto* v : b) dfree(v);
     for (auto* v : xs) dfree(v);
     for (auto* v : xa) dfree(v);
-    for (auto* v : d) dfree(v);
     for (auto* v : res) dfree(v);
-    b.clear(); xs.clear(); xa.clear(); d.clear(); res.clear();
+    b.clear(); xs.clear(); xa.clear(); res.clear();
     dfree(partial); dfree(alpha); dfree(beta); dfree(rz); dfree(rr); dfree(bn2); dfree(aux);
     dfree(active); dfree(counters);
-    dfree(orr); dfree(omask); dfree(ocount);
+    dfree(orr); dfree(omask); dfree(ocount); dfree(qmask);
     ncap = 0;
   }
 };
@@ -216,6 +217,7 @@ Sten<double> make_sten<double>(const sdc_handle_s* h, int l, int pin_mode) {
   s.n1x = v.n1x; s.n1y = v.n1y; s.n1z = v.n1z; s.nN = v.nN; s.plane = v.plane;
   s.fx = make_fastdiv(v.n1x); s.fy = make_fastdiv(v.n1y);
   s.pin0 = (h->bc == SDC_BC_NEUMANN) ? pin_mode : 0;
+  s.remap = h->prm.xcd_remap != 0;
   return s;
 }
 
@@ -227,6 +229,7 @@ Sten<float> make_sten<float>(const sdc_handle_s* h, int l, int pin_mode) {
   s.n1x = v.n1x; s.n1y = v.n1y; s.n1z = v.n1z; s.nN = v.nN; s.plane = v.plane;
   s.fx = make_fastdiv(v.n1x); s.fy = make_fastdiv(v.n1y);
   s.pin0 = (h->bc == SDC_BC_NEUMANN) ? pin_mode : 0;
+  s.remap = h->prm.xcd_remap != 0;
   return s;
 }
 
@@ -252,6 +255,7 @@ Xfer<T> make_xfer(const sdc_handle_s* h, int l) {
   t.cn1x = c.n1x; t.cn1y = c.n1y; t.cplane = c.plane; t.cN = c.nN;
   t.ffx = make_fastdiv(f.n1x); t.ffy = make_fastdiv(f.n1y);
   t.cfx = make_fastdiv(c.n1x); t.cfy = make_fastdiv(c.n1y);
+  t.remap = h->prm.xcd_remap != 0;
   return t;
 }
 
@@ -364,7 +368,7 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   CK(dmalloc(&w.p, n0)); CK(dmalloc(&w.q, n0));
   const size_t nl = h->L.size();
   w.b.assign(nl, nullptr); w.xs.assign(nl, nullptr); w.xa.assign(nl, nullptr);
-  w.d.assign(nl, nullptr); w.res.assign(nl, nullptr);
+  w.res.assign(nl, nullptr);
   for (size_t l = 0; l < nl; ++l) {
     const size_t nv = h->L[l].ld * (size_t)cap;
     if (l > 0) {
@@ -372,7 +376,6 @@ int ensure_work(sdc_handle_s* h, int ncol) {
       CK(dmalloc(&w.xs[l], nv));
     }
     CK(dmalloc(&w.xa[l], nv));
-    CK(dmalloc(&w.d[l], nv));
     CK(dmalloc(&w.res[l], nv));
   }
   const size_t nb0 = nblk(h->L[0].nN);
@@ -385,6 +388,7 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   CK(dmalloc(&w.orr, cap));
   CK(dmalloc(&w.omask, cap));
   CK(dmalloc(&w.ocount, 1));
+  CK(dmalloc(&w.qmask, ld0));
   w.ncap = cap;
   return SDC_OK;
 }
@@ -396,6 +400,12 @@ int pick_r(const sdc_handle_s* h, int ncol) {
   if (ncol >= 4) return 4;
   if (ncol >= 2) return 2;
   return 1;
+}
+
+// columns per work-item of the grid-transfer kernels
+int pick_rx(const sdc_handle_s* h, int ncol) {
+  const int r = h->prm.xfer_rhs_per_thread;
+  return (r == 1 || r == 2 || r == 4 || r == 8) ? r : pick_r(h, ncol);
 }
 
 #define DISPATCH_R(Rv, ...)                     \
@@ -458,33 +468,39 @@ int vcycle(sdc_handle_s* h, int l, const T* b, T* x, int ncol, bool dot_out, con
   const Sten<T> s = make_sten<T>(h, l, 1);
   const Cheb cc = cheb_coefs(h->prm);
   const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
-  const int nswitch = (nu1 - 1) + nu2;
-  T* cur = (nswitch % 2 == 0) ? x : W.xa[l];
+  // every smoothing step after the first writes the other buffer of the pair (x, xa[l]); the
+  // first output goes where the last one then lands in x
+  const int swaps = (nu1 >= 2 ? nu1 - 2 : 0) + nu2;
+  T* cur = (swaps % 2 == 0) ? x : W.xa[l];
   T* oth = (cur == x) ? W.xa[l] : x;
   const dim3 g = grid2(lv.nN, ncol, R);
   const dim3 blk(SDC_BLOCK);
   const double nodes = (double)lv.nN, cols = (double)ncol, sT = (double)sizeof(T);
   const int ptag = (l == 0) ? SDC_PROF_SMOOTH : -1;
-  int kstart = 1;
+  const T cz = (T)cc.c2[0];
+  double* const nodot = nullptr;
   if (nu1 >= 2) {
-    // steps 0 and 1 fused; the step-1 output goes where the unfused step 1 would write it
+    // steps 0 and 1 fused
     prof_begin(h, ptag, st);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth_pair<T, R>), g, blk, 0, st, s, b, oth, W.d[l],
-                                     lv.ld, ncol, (T)cc.c2[0], (T)cc.c1[1], (T)cc.c2[1], done));
-    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * 3);
-    std::swap(cur, oth);
-    kstart = 2;
+    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth_pair<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol,
+                                     cz, (T)cc.c1[1], (T)cc.c2[1], done));
+    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * 2);
   } else {
-    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol,
-                                     (T)cc.c2[0], done));
+    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol, cz,
+                                     done));
   }
-  for (int k = kstart; k < nu1; ++k) {
-    const T* dsrc = (k == 1) ? cur : W.d[l];
+  for (int k = 2; k < nu1; ++k) {
     prof_begin(h, ptag, st);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, true, false>), g, blk, 0, st, s, b,
-                                     (const T*)cur, oth, dsrc, W.d[l], lv.ld, ncol, (T)cc.c1[k],
-                                     (T)cc.c2[k], (double*)nullptr, done));
-    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * (dsrc == cur ? 4 : 5));
+    if (k == 2) {
+      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 2, false>), g, blk, 0, st, s, b,
+                                       (const T*)cur, (const T*)nullptr, oth, lv.ld, ncol,
+                                       (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
+    } else {
+      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 1, false>), g, blk, 0, st, s, b,
+                                       (const T*)cur, (const T*)oth, oth, lv.ld, ncol,
+                                       (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
+    }
+    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * (k == 2 ? 3 : 4));
     std::swap(cur, oth);
   }
   prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
@@ -494,39 +510,41 @@ int vcycle(sdc_handle_s* h, int l, const T* b, T* x, int ncol, bool dot_out, con
   const Xfer<T> t = make_xfer<T>(h, l);
   const Level& cl = h->L[l + 1];
   prof_begin(h, l == 0 ? SDC_PROF_RESTRICT : -1, st);
-  DISPATCH_R(R, hipLaunchKernelGGL((k_restrict<T, R>), grid2(cl.nN, ncol, R), blk, 0, st, t,
-                                   (const T*)W.res[l], W.b[l + 1], lv.ld, cl.ld, ncol, done));
+  const int RX = pick_rx(h, ncol);
+  DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict<T, R>), grid2(cl.nN, ncol, RX), blk, 0, st, t,
+                                    (const T*)W.res[l], W.b[l + 1], lv.ld, cl.ld, ncol, done));
   prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, (nodes + (double)cl.nN) * cols * sT);
   CK(vcycle<T>(h, l + 1, W.b[l + 1], W.xs[l + 1], ncol, false, done, st));
   prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
-  DISPATCH_R(R, hipLaunchKernelGGL((k_prolong_add<T, R>), g, blk, 0, st, t,
-                                   (const T*)W.xs[l + 1], cur, lv.ld, cl.ld, ncol, done));
+  DISPATCH_R(RX, hipLaunchKernelGGL((k_prolong_add<T, R>), grid2(lv.nN, ncol, RX), blk, 0, st, t,
+                                    (const T*)W.xs[l + 1], cur, lv.ld, cl.ld, ncol, done));
   prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sT);
   for (int k = 0; k < nu2; ++k) {
     const bool last = (k == nu2 - 1) && dot_out;
+    double* part = last ? W.partial : nodot;
     prof_begin(h, ptag, st);
     if (k == 0) {
       if (last) {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, false, true>), g, blk, 0, st, s, b,
-                                         (const T*)cur, oth, (const T*)W.d[l], W.d[l], lv.ld,
-                                         ncol, (T)0, (T)cc.c2[0], W.partial, done));
+        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 0, true>), g, blk, 0, st, s, b,
+                                         (const T*)cur, (const T*)nullptr, oth, lv.ld, ncol, (T)0,
+                                         (T)cc.c2[0], cz, part, done));
       } else {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, false, false>), g, blk, 0, st, s, b,
-                                         (const T*)cur, oth, (const T*)W.d[l], W.d[l], lv.ld,
-                                         ncol, (T)0, (T)cc.c2[0], (double*)nullptr, done));
+        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 0, false>), g, blk, 0, st, s, b,
+                                         (const T*)cur, (const T*)nullptr, oth, lv.ld, ncol, (T)0,
+                                         (T)cc.c2[0], cz, part, done));
       }
     } else {
       if (last) {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, true, true>), g, blk, 0, st, s, b,
-                                         (const T*)cur, oth, (const T*)W.d[l], W.d[l], lv.ld,
-                                         ncol, (T)cc.c1[k], (T)cc.c2[k], W.partial, done));
+        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 1, true>), g, blk, 0, st, s, b,
+                                         (const T*)cur, (const T*)oth, oth, lv.ld, ncol,
+                                         (T)cc.c1[k], (T)cc.c2[k], cz, part, done));
       } else {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_smooth<T, R, true, false>), g, blk, 0, st, s, b,
-                                         (const T*)cur, oth, (const T*)W.d[l], W.d[l], lv.ld,
-                                         ncol, (T)cc.c1[k], (T)cc.c2[k], (double*)nullptr, done));
+        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 1, false>), g, blk, 0, st, s, b,
+                                         (const T*)cur, (const T*)oth, oth, lv.ld, ncol,
+                                         (T)cc.c1[k], (T)cc.c2[k], cz, part, done));
       }
     }
-    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * (k == 0 ? 4 : 5));
+    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * (k == 0 ? 3 : 4));
     std::swap(cur, oth);
   }
   if (cur != x) return fail(SDC_ERR_STATE, "internal: V-cycle buffer parity");
@@ -624,8 +642,9 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const double delta2 = h->prm.reliable_delta * h->prm.reliable_delta;
   hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
   hipLaunchKernelGGL(k_zero<float>, dim3(nblk(len)), blk, 0, st, W.x, len);
+  hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
   DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
-                                   (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+                                   (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask));
   // aux = |q|^2 and active = |q| > 0; orr = |q|^2
   hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
                      W.active, 1);
@@ -685,8 +704,10 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
         const int R64 = std::min(R, 4);
         DISPATCH_R(R64, hipLaunchKernelGGL((k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64),
                                            blk, 0, st, s2, q, (const double*)u, W.r, l0.ld, ncol,
-                                           pf, W.partial));
-        prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 44);
+                                           pf, W.partial, (unsigned*)W.qmask));
+        // algorithmic bytes: fp64 coefficients + per column fold (x32, u64 read+write) and
+        // residual (u64 read, r32 write); q only on source nodes (not counted)
+        prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 36);
         hipLaunchKernelGGL(k_fin_true, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                            (const double*)W.aux, tol2, W.orr, W.rr, W.active);
         HIPCK(hipMemcpyAsync(h_rr.data(), W.orr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -735,7 +756,9 @@ int sdc_default_params(sdc_params* p) {
   p->cheb_lmax = 2.0;
   p->cheb_ratio = 8.0;
   p->rhs_per_thread = 0;
-  p->check_every = 2;
+  p->xfer_rhs_per_thread = 0;
+  p->xcd_remap = 1;
+  p->check_every = 4;
   p->reliable_delta = 1e-2;
   return SDC_OK;
 }
@@ -864,6 +887,10 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
   if (p->rhs_per_thread != 0 && p->rhs_per_thread != 1 && p->rhs_per_thread != 2 &&
       p->rhs_per_thread != 4 && p->rhs_per_thread != 8)
     return fail(SDC_ERR_ARG, "rhs_per_thread must be 0, 1, 2, 4 or 8");
+  if (p->xfer_rhs_per_thread != 0 && p->xfer_rhs_per_thread != 1 &&
+      p->xfer_rhs_per_thread != 2 && p->xfer_rhs_per_thread != 4 && p->xfer_rhs_per_thread != 8)
+    return fail(SDC_ERR_ARG, "xfer_rhs_per_thread must be 0, 1, 2, 4 or 8");
+  if (p->xcd_remap != 0 && p->xcd_remap != 1) return fail(SDC_ERR_ARG, "xcd_remap must be 0 or 1");
   if (!(p->reliable_delta > 0) || !(p->reliable_delta < 1))
     return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
   h->prm = *p;
@@ -1005,8 +1032,9 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
   } else if (o->mode == SDC_MODE_FP64) {
     CK(ensure_work<double>(h, ncol));
     Work<double>& W = h->w64;
+    hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, true>), g, blk, 0, st, s2, q,
-                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
                        W.active, 0);
     int it = 0;
@@ -1014,7 +1042,8 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     S.iters = it;
     S.outer = 1;
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
-                                     (const double*)u, W.z, l0.ld, ncol, pf, W.partial));
+                                     (const double*)u, W.z, l0.ld, ncol, pf, W.partial,
+                                     W.qmask));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
                        W.active, 0);
     CK(rel_max(W.orr, W.aux));
@@ -1026,8 +1055,9 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     CK(ensure_work<float>(h, ncol));
     Work<float>& W = h->w32;
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
+    hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
-                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
                        W.active, 0);
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
@@ -1037,7 +1067,8 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
       if (outer > 0) {
         prof_begin(h, SDC_PROF_REFINE, st);
         DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, false>), g, blk, 0, st, s2, q,
-                                         (const double*)u, W.r, l0.ld, ncol, pf, W.partial));
+                                         (const double*)u, W.r, l0.ld, ncol, pf, W.partial,
+                                         W.qmask));
         prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 20);
         hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                            W.orr, W.active, 0);
@@ -1090,6 +1121,7 @@ int sdc_jvec_rhs(sdc_handle h, const double* dsig, const double* u, double* out,
   s.n1x = l0.n1x; s.n1y = l0.n1y; s.n1z = l0.n1z; s.nN = l0.nN; s.plane = l0.plane;
   s.fx = make_fastdiv(l0.n1x); s.fy = make_fastdiv(l0.n1y);
   s.pin0 = 0;
+  s.remap = h->prm.xcd_remap != 0;
   const int R = pick_r(h, ncol);
   DISPATCH_R(R, hipLaunchKernelGGL((k_apply<double, R, false>), grid2(l0.nN, ncol, R),
                                    dim3(SDC_BLOCK), 0, st, s, u, out, l0.ld, ncol,
