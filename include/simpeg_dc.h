@@ -83,6 +83,8 @@ typedef struct {
                                   once it dropped by this factor (0 < delta < 1) */
   int32_t xfer_rhs_per_thread; /* grid transfers: 0 = as rhs_per_thread, else 1, 2, 4 or 8 */
   int32_t xcd_remap;           /* 1: give each XCD a contiguous range of work-groups, 0: off */
+  int32_t vcycle_bf16;         /* 1: the fp32 solver keeps its multigrid vectors in bfloat16
+                                  (arithmetic stays fp32); 0: fp32 vectors */
 } sdc_params;
 
 typedef struct {

@@ -56,6 +56,7 @@ class Params(C.Structure):
         ("reliable_delta", C.c_double),
         ("xfer_rhs_per_thread", C.c_int32),
         ("xcd_remap", C.c_int32),
+        ("vcycle_bf16", C.c_int32),
     ]
 
 

@@ -138,20 +138,21 @@ __device__ __forceinline__ const T* col_ptr(const T* x, size_t ld, int c0, int r
 
 // (A x)_n for R columns starting at column c0.  All 7R loads are issued before the arithmetic
 // so that a wavefront keeps them in flight; the per-lane indices are shared by every column.
-template <typename T, int R>
-__device__ __forceinline__ void stencil_cols(const NodeC<T>& c, const T* __restrict__ x, uint32_t n,
+// The block may be stored in a narrower type X (bf16) than the arithmetic type T.
+template <typename T, int R, typename X>
+__device__ __forceinline__ void stencil_cols(const NodeC<T>& c, const X* __restrict__ x, uint32_t n,
                                              size_t ld, int c0, int ncol, T (&xc)[R], T (&v)[R]) {
   T xw[R], xe[R], xs[R], xn[R], xb[R], xt[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const T* col = col_ptr(x, ld, c0, r, ncol);
-    xc[r] = col[n];
-    xw[r] = col[c.ixm];
-    xe[r] = col[c.ixp];
-    xs[r] = col[c.iym];
-    xn[r] = col[c.iyp];
-    xb[r] = col[c.izm];
-    xt[r] = col[c.izp];
+    const X* col = col_ptr(x, ld, c0, r, ncol);
+    xc[r] = (T)col[n];
+    xw[r] = (T)col[c.ixm];
+    xe[r] = (T)col[c.ixp];
+    xs[r] = (T)col[c.iym];
+    xn[r] = (T)col[c.iyp];
+    xb[r] = (T)col[c.izm];
+    xt[r] = (T)col[c.izp];
   }
 #pragma unroll
   for (int r = 0; r < R; ++r)

@@ -55,11 +55,11 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> s, const T* __restr
   if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
 
-// r = b - A x
-template <typename T, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const T* __restrict__ b,
-                                                        const T* __restrict__ x,
-                                                        T* __restrict__ res, size_t ld, int ncol,
+// r = b - A x.  b in type B, x and r in the V-cycle storage type X, arithmetic in T.
+template <typename T, typename B, typename X, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const B* __restrict__ b,
+                                                        const X* __restrict__ x,
+                                                        X* __restrict__ res, size_t ld, int ncol,
                                                         const int* __restrict__ done) {
   if (done && *done) return;
   SDC_NODE_PROLOGUE_R(s.nN, s.remap)
@@ -67,17 +67,17 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const T* __re
   const NodeC<T> c = node_coef(s, n);
   T xc[R], v[R], bv[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) bv[r] = ld_once(col_ptr(b, ld, c0, r, ncol) + n);
+  for (int r = 0; r < R; ++r) bv[r] = (T)ld_once(col_ptr(b, ld, c0, r, ncol) + n);
   stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, v);
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if (c0 + r < ncol) st_once(res + (size_t)(c0 + r) * ld + n, bv[r] - v[r]);
+    if (c0 + r < ncol) st_once(res + (size_t)(c0 + r) * ld + n, (X)(bv[r] - v[r]));
 }
 
 // First Chebyshev step from a zero guess: x = c0 * D^-1 b
-template <typename T, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const T* __restrict__ b,
-                                                       T* __restrict__ x, size_t ld, int ncol,
+template <typename T, typename B, typename X, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const B* __restrict__ b,
+                                                       X* __restrict__ x, size_t ld, int ncol,
                                                        T coef, const int* __restrict__ done) {
   if (done && *done) return;
   SDC_NODE_PROLOGUE_R(s.nN, s.remap)
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const T* __res
   for (int r = 0; r < R; ++r) {
     if (c0 + r < ncol) {
       const size_t o = (size_t)(c0 + r) * ld;
-      st_once(x + o + n, w * ld_once(b + o + n));
+      st_once(x + o + n, (X)(w * (T)ld_once(b + o + n)));
     }
   }
 }
@@ -98,10 +98,12 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const T* __res
 //             xprev entries before it writes xout, and nobody else reads them);
 //   PREV = 2: xprev = cz D^-1 b, the first iterate from a zero guess, recomputed here.
 // DOT adds per-column partial sums of b.xout (the r.z product of PCG).
-template <typename T, int R, int PREV, bool DOT>
-__global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const T* __restrict__ b,
-                                                    const T* __restrict__ x, const T* xprev,
-                                                    T* xout, size_t ld, int ncol, T c1, T c2,
+// b in type B, x and xprev in the storage type X, xout in type O (O = T on the last step of
+// the finest level, which hands z back to PCG).
+template <typename T, typename B, typename X, typename O, int R, int PREV, bool DOT>
+__global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const B* __restrict__ b,
+                                                    const X* __restrict__ x, const X* xprev,
+                                                    O* xout, size_t ld, int ncol, T c1, T c2,
                                                     T cz, double* __restrict__ partial,
                                                     const int* __restrict__ done) {
   if (done && *done) return;
@@ -116,8 +118,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const T* __restri
     T xc[R], v[R], bv[R], pv[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      bv[r] = ld_once(col_ptr(b, ld, c0, r, ncol) + n);
-      pv[r] = (PREV == 1) ? col_ptr(xprev, ld, c0, r, ncol)[n] : T(0);
+      bv[r] = (T)ld_once(col_ptr(b, ld, c0, r, ncol) + n);
+      pv[r] = (PREV == 1) ? (T)col_ptr(xprev, ld, c0, r, ncol)[n] : T(0);
     }
     stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, v);
 #pragma unroll
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const T* __restri
         T xn = xc[r] + w * (bv[r] - v[r]);
         if (PREV == 1) xn += c1 * (xc[r] - pv[r]);
         if (PREV == 2) xn += c1 * (xc[r] - cz * g * bv[r]);
-        st_once(xout + o, xn);
+        st_once(xout + o, (O)xn);
         if (DOT) acc[r] += (double)bv[r] * (double)xn;
       }
     }
@@ -138,9 +140,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const T* __restri
 // The first two Chebyshev steps from a zero guess in one pass:
 //   x1 = c0 D^-1 b (recomputed at the 7 stencil points, never stored), r1 = b - A x1,
 //   x2 = x1 + c1 x1 + c2 D^-1 r1.  Writes x2 only (a following step recomputes x1 from b).
-template <typename T, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_smooth_pair(Sten<T> s, const T* __restrict__ b,
-                                                           T* __restrict__ xout, size_t ld,
+template <typename T, typename B, typename X, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_smooth_pair(Sten<T> s, const B* __restrict__ b,
+                                                           X* __restrict__ xout, size_t ld,
                                                            int ncol, T cz, T c1, T c2,
                                                            const int* __restrict__ done) {
   if (done && *done) return;
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth_pair(Sten<T> s, const T* _
       const size_t o = (size_t)(c0 + r) * ld + n;
       const T x1 = cz * g * bc[r];
       const T r1 = bc[r] - cz * ab[r];
-      st_once(xout + o, x1 + c1 * x1 + c2 * g * r1);
+      st_once(xout + o, (X)(x1 + c1 * x1 + c2 * g * r1));
     }
   }
 }
@@ -202,9 +204,9 @@ struct Xfer {
 // rc = P^T rf: every coarse node gathers the 3x3x3 fine nodes around its fine position with
 // separable weights; a missing neighbour (domain edge or coincident coarse node) has weight 0
 // and its index is clamped into the grid.
-template <typename T, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __restrict__ rf,
-                                                        T* __restrict__ rc, size_t ldf, size_t ldc,
+template <typename T, typename X, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const X* __restrict__ rf,
+                                                        X* __restrict__ rc, size_t ldf, size_t ldc,
                                                         int ncol, const int* __restrict__ done) {
   if (done && *done) return;
   SDC_NODE_PROLOGUE_R(t.cN, t.remap)
@@ -234,10 +236,10 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __re
       T v[R][3];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const T* f = col_ptr(rf, ldf, c0, r, ncol);
-        v[r][0] = f[i0];
-        v[r][1] = f[i1];
-        v[r][2] = f[i2];
+        const X* f = col_ptr(rf, ldf, c0, r, ncol);
+        v[r][0] = (T)f[i0];
+        v[r][1] = (T)f[i1];
+        v[r][2] = (T)f[i2];
       }
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -246,14 +248,14 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const T* __re
   }
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if (c0 + r < ncol) rc[(size_t)(c0 + r) * ldc + n] = acc[r];
+    if (c0 + r < ncol) rc[(size_t)(c0 + r) * ldc + n] = (X)acc[r];
 }
 
 // xf += P xc.  Every fine node reads the 2x2x2 coarse nodes around it; a coincident axis has
 // weight 0 on its second node, whose index still lies inside the coarse grid.
-template <typename T, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const T* __restrict__ xc,
-                                                           T* __restrict__ xf, size_t ldf,
+template <typename T, typename X, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* __restrict__ xc,
+                                                           X* __restrict__ xf, size_t ldf,
                                                            size_t ldc, int ncol,
                                                            const int* __restrict__ done) {
   if (done && *done) return;
@@ -269,16 +271,16 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const T* _
   T cv[R][8], fv[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const T* c = col_ptr(xc, ldc, c0, r, ncol);
-    cv[r][0] = c[c000];
-    cv[r][1] = c[c000 + 1];
-    cv[r][2] = c[c010];
-    cv[r][3] = c[c010 + 1];
-    cv[r][4] = c[c001];
-    cv[r][5] = c[c001 + 1];
-    cv[r][6] = c[c011];
-    cv[r][7] = c[c011 + 1];
-    fv[r] = col_ptr((const T*)xf, ldf, c0, r, ncol)[n];
+    const X* c = col_ptr(xc, ldc, c0, r, ncol);
+    cv[r][0] = (T)c[c000];
+    cv[r][1] = (T)c[c000 + 1];
+    cv[r][2] = (T)c[c010];
+    cv[r][3] = (T)c[c010 + 1];
+    cv[r][4] = (T)c[c001];
+    cv[r][5] = (T)c[c001 + 1];
+    cv[r][6] = (T)c[c011];
+    cv[r][7] = (T)c[c011 + 1];
+    fv[r] = (T)col_ptr((const X*)xf, ldf, c0, r, ncol)[n];
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -289,26 +291,26 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const T* _
       const T a3 = cv[r][6] + ux * (cv[r][7] - cv[r][6]);
       const T b0 = a0 + uy * (a1 - a0);
       const T b1 = a2 + uy * (a3 - a2);
-      xf[(size_t)(c0 + r) * ldf + n] = fv[r] + (b0 + uz * (b1 - b0));
+      xf[(size_t)(c0 + r) * ldf + n] = (X)(fv[r] + (b0 + uz * (b1 - b0)));
     }
   }
 }
 
 // coarsest level: x = Ainv b (dense inverse, n x n; symmetric, so column m is read as row m
 // and consecutive work-items read consecutive addresses)
-template <typename T>
+template <typename T, typename B, typename O>
 __global__ __launch_bounds__(SDC_BLOCK) void k_dense_solve(const T* __restrict__ ainv, int nn,
-                                                           const T* __restrict__ b,
-                                                           T* __restrict__ x, size_t ld, int ncol,
+                                                           const B* __restrict__ b,
+                                                           O* __restrict__ x, size_t ld, int ncol,
                                                            const int* __restrict__ done) {
   if (done && *done) return;
   const int row = blockIdx.x * SDC_BLOCK + threadIdx.x;
   const int col = blockIdx.y;
   if (row >= nn || col >= ncol) return;
-  const T* bc = b + (size_t)col * ld;
+  const B* bc = b + (size_t)col * ld;
   T acc = T(0);
-  for (int m = 0; m < nn; ++m) acc += ainv[(size_t)m * nn + row] * bc[m];
-  x[(size_t)col * ld + row] = acc;
+  for (int m = 0; m < nn; ++m) acc += ainv[(size_t)m * nn + row] * (T)bc[m];
+  x[(size_t)col * ld + row] = (O)acc;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -118,8 +118,12 @@ struct Work {
   int ncap = 0;
   // level 0: PCG vectors and smoother scratch
   T *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
-  // per level smoother / V-cycle vectors (index 0 uses r as rhs and z as solution)
-  std::vector<T*> b, xs, xa, res;
+  // per level V-cycle vectors in the V-cycle storage type (vbytes per entry: T, or bf16 for
+  // the fp32 solver): rhs b and solution xs (levels >= 1; level 0 reads r and writes z),
+  // smoother ping-pong xa (and xb0 on level 0), residual res
+  std::vector<void*> b, xs, xa, res;
+  void* xb0 = nullptr;
+  int vbytes = 0;
   double* partial = nullptr;
   double *alpha = nullptr, *beta = nullptr, *rz = nullptr, *rr = nullptr, *bn2 = nullptr,
          *aux = nullptr;
@@ -138,6 +142,8 @@ struct Work {
     for (auto* v : xa) dfree(v);
     for (auto* v : res) dfree(v);
     b.clear(); xs.clear(); xa.clear(); res.clear();
+    dfree(xb0);
+    vbytes = 0;
     dfree(partial); dfree(alpha); dfree(beta); dfree(rz); dfree(rr); dfree(bn2); dfree(aux);
     dfree(active); dfree(counters);
     dfree(orr); dfree(omask); dfree(ocount); dfree(qmask);
@@ -356,10 +362,18 @@ Work<float>& work_of<float>(sdc_handle_s* h) { return h->w32; }
 template <>
 Work<double>& work_of<double>(sdc_handle_s* h) { return h->w64; }
 
+// bytes per entry of the V-cycle vectors of the solver in precision T
+template <typename T>
+int vcycle_bytes(const sdc_handle_s* h) {
+  return (std::is_same<T, float>::value && h->prm.vcycle_bf16) ? 2 : (int)sizeof(T);
+}
+
 template <typename T>
 int ensure_work(sdc_handle_s* h, int ncol) {
   Work<T>& w = work_of<T>(h);
-  if (w.ncap >= ncol) return SDC_OK;
+  const int vb = vcycle_bytes<T>(h);
+  if (w.ncap >= ncol && w.vbytes == vb) return SDC_OK;
+  ncol = std::max(ncol, w.ncap);
   w.release();
   const int cap = std::max(ncol, 1);
   const size_t ld0 = h->L[0].ld;
@@ -369,14 +383,22 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   const size_t nl = h->L.size();
   w.b.assign(nl, nullptr); w.xs.assign(nl, nullptr); w.xa.assign(nl, nullptr);
   w.res.assign(nl, nullptr);
+  w.vbytes = vb;
+  auto vmalloc = [](void** p, size_t bytes) -> int {
+    unsigned char* q = nullptr;
+    CK(dmalloc(&q, bytes));
+    *p = q;
+    return SDC_OK;
+  };
   for (size_t l = 0; l < nl; ++l) {
-    const size_t nv = h->L[l].ld * (size_t)cap;
+    const size_t nv = h->L[l].ld * (size_t)cap * (size_t)vb;
     if (l > 0) {
-      CK(dmalloc(&w.b[l], nv));
-      CK(dmalloc(&w.xs[l], nv));
+      CK(vmalloc(&w.b[l], nv));
+      CK(vmalloc(&w.xs[l], nv));
     }
-    CK(dmalloc(&w.xa[l], nv));
-    CK(dmalloc(&w.res[l], nv));
+    CK(vmalloc(&w.xa[l], nv));
+    CK(vmalloc(&w.res[l], nv));
+    if (l == 0) CK(vmalloc(&w.xb0, nv));
   }
   const size_t nb0 = nblk(h->L[0].nN);
   CK(dmalloc(&w.partial, nb0 * (size_t)cap));
@@ -446,9 +468,12 @@ Cheb cheb_coefs(const sdc_params& p) {
   return c;
 }
 
-// one V-cycle z = M^-1 b on level l; with dot_out the last kernel writes partials of b.z
-template <typename T>
-int vcycle(sdc_handle_s* h, int l, const T* b, T* x, int ncol, bool dot_out, const int* done,
+// one V-cycle x = M^-1 b on level l; with dot_out the last kernel writes partials of b.x.
+// T: arithmetic and coefficient type; S: storage type of the V-cycle vectors; B, O: types of
+// b and x (T on the finest level, where b is the PCG residual and x the preconditioned one; S
+// below).
+template <typename T, typename S, typename B, typename O>
+int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, const int* done,
            hipStream_t st) {
   Work<T>& W = work_of<T>(h);
   const int R = pick_r(h, ncol);
@@ -457,98 +482,133 @@ int vcycle(sdc_handle_s* h, int l, const T* b, T* x, int ncol, bool dot_out, con
   if (l == nl - 1) {
     const T* ainv = (const T*)(std::is_same<T, float>::value ? (const void*)h->ainv32
                                                               : (const void*)h->ainv64);
-    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(lv.nN), ncol), dim3(SDC_BLOCK), 0, st, ainv,
-                       (int)lv.nN, b, x, lv.ld, ncol, done);
-    if (dot_out) {
-      DISPATCH_R(R, hipLaunchKernelGGL((k_dot<T, R>), grid2(lv.nN, ncol, R), dim3(SDC_BLOCK), 0,
-                                       st, b, (const T*)x, lv.ld, lv.nN, ncol, W.partial, done));
+    hipLaunchKernelGGL((k_dense_solve<T, B, O>), dim3(nblk(lv.nN), ncol), dim3(SDC_BLOCK), 0, st,
+                       ainv, (int)lv.nN, b, x, lv.ld, ncol, done);
+    if constexpr (std::is_same<B, T>::value && std::is_same<O, T>::value) {
+      if (dot_out) {
+        DISPATCH_R(R, hipLaunchKernelGGL((k_dot<T, R>), grid2(lv.nN, ncol, R), dim3(SDC_BLOCK), 0,
+                                         st, b, (const T*)x, lv.ld, lv.nN, ncol, W.partial, done));
+      }
     }
     return SDC_OK;
   }
   const Sten<T> s = make_sten<T>(h, l, 1);
   const Cheb cc = cheb_coefs(h->prm);
   const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
-  // every smoothing step after the first writes the other buffer of the pair (x, xa[l]); the
-  // first output goes where the last one then lands in x
+  // Smoothing steps ping-pong between two S buffers.  When x has the storage type (O == S) it
+  // is one of them and the first output goes where the last one then lands in x; otherwise
+  // (finest level of the fp32 solver with bf16 storage) the last step writes x directly.
+  constexpr bool sep = !std::is_same<O, S>::value;
   const int swaps = (nu1 >= 2 ? nu1 - 2 : 0) + nu2;
-  T* cur = (swaps % 2 == 0) ? x : W.xa[l];
-  T* oth = (cur == x) ? W.xa[l] : x;
+  S* const xa = (S*)W.xa[l];
+  S* cur;
+  S* oth;
+  if constexpr (sep) {
+    cur = xa;
+    oth = (S*)W.xb0;
+  } else {
+    cur = (swaps % 2 == 0) ? (S*)x : xa;
+    oth = (cur == (S*)x) ? xa : (S*)x;
+  }
   const dim3 g = grid2(lv.nN, ncol, R);
   const dim3 blk(SDC_BLOCK);
   const double nodes = (double)lv.nN, cols = (double)ncol, sT = (double)sizeof(T);
+  const double sS = (double)sizeof(S), sB = (double)sizeof(B), sO = (double)sizeof(O);
   const int ptag = (l == 0) ? SDC_PROF_SMOOTH : -1;
   const T cz = (T)cc.c2[0];
   double* const nodot = nullptr;
   if (nu1 >= 2) {
     // steps 0 and 1 fused
     prof_begin(h, ptag, st);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth_pair<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol,
-                                     cz, (T)cc.c1[1], (T)cc.c2[1], done));
-    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * 2);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth_pair<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
+                                     ncol, cz, (T)cc.c1[1], (T)cc.c2[1], done));
+    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * (sB + sS));
   } else {
-    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, R>), g, blk, 0, st, s, b, cur, lv.ld, ncol, cz,
-                                     done));
+    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
+                                     ncol, cz, done));
   }
   for (int k = 2; k < nu1; ++k) {
     prof_begin(h, ptag, st);
     if (k == 2) {
-      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 2, false>), g, blk, 0, st, s, b,
-                                       (const T*)cur, (const T*)nullptr, oth, lv.ld, ncol,
+      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, S, R, 2, false>), g, blk, 0, st, s, b,
+                                       (const S*)cur, (const S*)nullptr, oth, lv.ld, ncol,
                                        (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
     } else {
-      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 1, false>), g, blk, 0, st, s, b,
-                                       (const T*)cur, (const T*)oth, oth, lv.ld, ncol,
+      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, S, R, 1, false>), g, blk, 0, st, s, b,
+                                       (const S*)cur, (const S*)oth, oth, lv.ld, ncol,
                                        (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
     }
-    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * (k == 2 ? 3 : 4));
+    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * (sB + sS * (k == 2 ? 2 : 3)));
     std::swap(cur, oth);
   }
+  S* const res = (S*)W.res[l];
   prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
-  DISPATCH_R(R, hipLaunchKernelGGL((k_residual<T, R>), g, blk, 0, st, s, b, (const T*)cur,
-                                   W.res[l], lv.ld, ncol, done));
-  prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * sT * 3);
+  DISPATCH_R(R, hipLaunchKernelGGL((k_residual<T, B, S, R>), g, blk, 0, st, s, b, (const S*)cur,
+                                   res, lv.ld, ncol, done));
+  prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * (sB + 2 * sS));
   const Xfer<T> t = make_xfer<T>(h, l);
   const Level& cl = h->L[l + 1];
+  S* const cb = (S*)W.b[l + 1];
+  S* const cx = (S*)W.xs[l + 1];
   prof_begin(h, l == 0 ? SDC_PROF_RESTRICT : -1, st);
   const int RX = pick_rx(h, ncol);
-  DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict<T, R>), grid2(cl.nN, ncol, RX), blk, 0, st, t,
-                                    (const T*)W.res[l], W.b[l + 1], lv.ld, cl.ld, ncol, done));
-  prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, (nodes + (double)cl.nN) * cols * sT);
-  CK(vcycle<T>(h, l + 1, W.b[l + 1], W.xs[l + 1], ncol, false, done, st));
+  DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict<T, S, R>), grid2(cl.nN, ncol, RX), blk, 0, st, t,
+                                    (const S*)res, cb, lv.ld, cl.ld, ncol, done));
+  prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, (nodes + (double)cl.nN) * cols * sS);
+  CK((vcycle<T, S, S, S>(h, l + 1, cb, cx, ncol, false, done, st)));
   prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
-  DISPATCH_R(RX, hipLaunchKernelGGL((k_prolong_add<T, R>), grid2(lv.nN, ncol, RX), blk, 0, st, t,
-                                    (const T*)W.xs[l + 1], cur, lv.ld, cl.ld, ncol, done));
-  prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sT);
+  DISPATCH_R(RX, hipLaunchKernelGGL((k_prolong_add<T, S, R>), grid2(lv.nN, ncol, RX), blk, 0, st,
+                                    t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
+  prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sS);
   for (int k = 0; k < nu2; ++k) {
-    const bool last = (k == nu2 - 1) && dot_out;
-    double* part = last ? W.partial : nodot;
+    const bool last = k == nu2 - 1;
+    double* part = (last && dot_out) ? W.partial : nodot;
+    const T c1 = (T)(k == 0 ? 0.0 : cc.c1[k]), c2 = (T)cc.c2[k];
     prof_begin(h, ptag, st);
-    if (k == 0) {
-      if (last) {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 0, true>), g, blk, 0, st, s, b,
-                                         (const T*)cur, (const T*)nullptr, oth, lv.ld, ncol, (T)0,
-                                         (T)cc.c2[0], cz, part, done));
+    // PREV = 0 on the first step (no previous iterate), 1 after
+#define SDC_CHEB_STEP(PREV, OT, OUT)                                                           \
+    if (part) {                                                                                \
+      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, OT, R, PREV, true>), g, blk, 0, st, s, \
+                                       b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol, c1,  \
+                                       c2, cz, part, done));                                   \
+    } else {                                                                                   \
+      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, OT, R, PREV, false>), g, blk, 0, st,   \
+                                       s, b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol,   \
+                                       c1, c2, cz, part, done));                               \
+    }
+    if (sep && last) {
+      if (k == 0) {
+        SDC_CHEB_STEP(0, O, x)
       } else {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 0, false>), g, blk, 0, st, s, b,
-                                         (const T*)cur, (const T*)nullptr, oth, lv.ld, ncol, (T)0,
-                                         (T)cc.c2[0], cz, part, done));
+        SDC_CHEB_STEP(1, O, x)
       }
     } else {
-      if (last) {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 1, true>), g, blk, 0, st, s, b,
-                                         (const T*)cur, (const T*)oth, oth, lv.ld, ncol,
-                                         (T)cc.c1[k], (T)cc.c2[k], cz, part, done));
+      if (k == 0) {
+        SDC_CHEB_STEP(0, S, oth)
       } else {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, R, 1, false>), g, blk, 0, st, s, b,
-                                         (const T*)cur, (const T*)oth, oth, lv.ld, ncol,
-                                         (T)cc.c1[k], (T)cc.c2[k], cz, part, done));
+        SDC_CHEB_STEP(1, S, oth)
       }
     }
-    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * sT * (k == 0 ? 3 : 4));
+#undef SDC_CHEB_STEP
+    prof_end(h, ptag, st,
+             nodes * 5 * sT + nodes * cols * (sB + sS * (k == 0 ? 1 : 2) + (sep && last ? sO : sS)));
     std::swap(cur, oth);
   }
-  if (cur != x) return fail(SDC_ERR_STATE, "internal: V-cycle buffer parity");
+  if constexpr (!sep) {
+    if (cur != (S*)x) return fail(SDC_ERR_STATE, "internal: V-cycle buffer parity");
+  }
   return SDC_OK;
+}
+
+// V-cycle on the finest level: b and x are PCG vectors of type T
+template <typename T>
+int vcycle0(sdc_handle_s* h, const T* b, T* x, int ncol, bool dot_out, const int* done,
+            hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (h->prm.vcycle_bf16) return vcycle<float, __bf16, float, float>(h, 0, b, x, ncol, dot_out,
+                                                                       done, st);
+  }
+  return vcycle<T, T, T, T>(h, 0, b, x, ncol, dot_out, done, st);
 }
 
 // Preconditioned CG on ncol columns: rhs in W.r (overwritten by the residual), solution in x.
@@ -572,7 +632,7 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
                      W.active, 1);
   if (col_mask)
     hipLaunchKernelGGL(k_and_active, dim3(nblk(ncol)), blk, 0, st, W.active, col_mask, ncol);
-  CK(vcycle<T>(h, 0, W.r, W.z, ncol, true, done, st));
+  CK(vcycle0<T>(h, W.r, W.z, ncol, true, done, st));
   hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                      W.beta, (const int*)W.active, 1, (const int*)done);
   hipLaunchKernelGGL((k_update_p4<T, true>), grid4(l0.nN, ncol), blk, 0, st, (const T*)W.z, W.p,
@@ -604,7 +664,7 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
         break;
       }
     }
-    CK(vcycle<T>(h, 0, W.r, W.z, ncol, true, done, st));
+    CK(vcycle0<T>(h, W.r, W.z, ncol, true, done, st));
     hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                        W.beta, (const int*)W.active, 0, (const int*)done);
     hipLaunchKernelGGL((k_update_p4<T, false>), grid4(l0.nN, ncol), blk, 0, st, (const T*)W.z, W.p,
@@ -667,7 +727,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   if (nact == 0) return SDC_OK;
   // the finalisers test the convergence flag counters[3]; a previous solve may have left it set
   hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)4);
-  CK(vcycle<float>(h, 0, W.r, W.z, ncol, true, nullptr, st));
+  CK(vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, st));
   hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                      W.beta, (const int*)W.active, 1, (const int*)(W.counters + 3));
   hipLaunchKernelGGL((k_update_p4<float, true>), grid4(l0.nN, ncol), blk, 0, st, (const float*)W.z,
@@ -725,7 +785,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
         if (nact == 0 || it == maxit) break;
       }
     }
-    CK(vcycle<float>(h, 0, W.r, W.z, ncol, true, nullptr, st));
+    CK(vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, st));
     hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                        W.beta, (const int*)W.active, 0, (const int*)(W.counters + 3));
     hipLaunchKernelGGL((k_update_p4<float, false>), grid4(l0.nN, ncol), blk, 0, st,
@@ -758,6 +818,7 @@ int sdc_default_params(sdc_params* p) {
   p->rhs_per_thread = 0;
   p->xfer_rhs_per_thread = 0;
   p->xcd_remap = 1;
+  p->vcycle_bf16 = 1;
   p->check_every = 4;
   p->reliable_delta = 1e-2;
   return SDC_OK;
@@ -891,6 +952,8 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
       p->xfer_rhs_per_thread != 2 && p->xfer_rhs_per_thread != 4 && p->xfer_rhs_per_thread != 8)
     return fail(SDC_ERR_ARG, "xfer_rhs_per_thread must be 0, 1, 2, 4 or 8");
   if (p->xcd_remap != 0 && p->xcd_remap != 1) return fail(SDC_ERR_ARG, "xcd_remap must be 0 or 1");
+  if (p->vcycle_bf16 != 0 && p->vcycle_bf16 != 1)
+    return fail(SDC_ERR_ARG, "vcycle_bf16 must be 0 or 1");
   if (!(p->reliable_delta > 0) || !(p->reliable_delta < 1))
     return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
   h->prm = *p;
@@ -973,10 +1036,10 @@ int sdc_vcycle(sdc_handle h, int dtype, const void* b, void* z, int ncol, void* 
   hipStream_t st = (hipStream_t)stream;
   if (dtype == SDC_F64) {
     CK(ensure_work<double>(h, ncol));
-    CK(vcycle<double>(h, 0, (const double*)b, (double*)z, ncol, false, nullptr, st));
+    CK(vcycle0<double>(h, (const double*)b, (double*)z, ncol, false, nullptr, st));
   } else if (dtype == SDC_F32) {
     CK(ensure_work<float>(h, ncol));
-    CK(vcycle<float>(h, 0, (const float*)b, (float*)z, ncol, false, nullptr, st));
+    CK(vcycle0<float>(h, (const float*)b, (float*)z, ncol, false, nullptr, st));
   } else {
     return fail(SDC_ERR_ARG, "dtype must be SDC_F32 or SDC_F64");
   }

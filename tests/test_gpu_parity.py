@@ -64,6 +64,33 @@ def test_vcycle_is_symmetric(gpu, tiny):
 
 
 @pytest.mark.parametrize("bc", ["Robin", "Neumann"])
+def test_fp32_vcycle_bf16_storage(gpu, tiny, bc):
+    """The fp32 V-cycle with bfloat16 vector storage stays within bf16 rounding of the fp32-storage
+    V-cycle and of the fp64 one, and is symmetric to that accuracy."""
+    mesh, survey, m = tiny
+    sim = _sim(mesh, survey, bc)
+    sim.model = m
+    sim._ensure_sigma()
+    eng = sim.engine
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((mesh.n_nodes, 3))
+    b = rng.standard_normal((mesh.n_nodes, 3))
+    ref = eng.from_block(eng.vcycle(eng.to_block(a)))
+    out = {}
+    for flag in (0, 1):
+        eng.set_params(vcycle_bf16=flag)
+        out[flag] = eng.from_block(eng.vcycle(eng.to_block(a).float()).double())
+    mb = eng.from_block(eng.vcycle(eng.to_block(b).float()).double())
+    eng.set_params(vcycle_bf16=1)
+    assert rel(out[0], ref) < 1e-5
+    assert 1e-6 < rel(out[1], ref) < 3e-2  # bf16 storage is in effect, and close
+    lhs = np.sum(out[1] * b, axis=0)
+    rhs = np.sum(a * mb, axis=0)
+    assert np.allclose(lhs, rhs, rtol=3e-2)
+    assert np.all(np.sum(out[1] * a, axis=0) > 0)
+
+
+@pytest.mark.parametrize("bc", ["Robin", "Neumann"])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_fields_dpred_match_direct_solver(gpu, tiny, bc, precision):
     mesh, survey, m = tiny
