@@ -85,6 +85,11 @@ typedef struct {
   int32_t xcd_remap;           /* 1: give each XCD a contiguous range of work-groups, 0: off */
   int32_t vcycle_bf16;         /* 1: the fp32 solver keeps its multigrid vectors in bfloat16
                                   (arithmetic stays fp32); 0: fp32 vectors */
+  int32_t xfer_separable;      /* restriction: 1 = one pass per coarsened axis, 0 = one 27-point
+                                  gather pass */
+  int32_t wcycle_level;        /* 0: V-cycle; l >= 1: the coarse correction of level l runs twice
+                                  (W-cycle on that level); -1: the first level with at most 1/16
+                                  of the finest level's nodes */
 } sdc_params;
 
 typedef struct {

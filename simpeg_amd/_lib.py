@@ -57,6 +57,8 @@ class Params(C.Structure):
         ("xfer_rhs_per_thread", C.c_int32),
         ("xcd_remap", C.c_int32),
         ("vcycle_bf16", C.c_int32),
+        ("xfer_separable", C.c_int32),
+        ("wcycle_level", C.c_int32),
     ]
 
 

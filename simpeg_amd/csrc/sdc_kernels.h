@@ -251,6 +251,58 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const X* __re
     if (c0 + r < ncol) rc[(size_t)(c0 + r) * ldc + n] = (X)acc[r];
 }
 
+// One factor of the separable restriction P^T = Pz^T Py^T Px^T: along axis AX the output has
+// the coarse count, every output entry gathers the three input entries around its fine index
+// (weights lo, 1, hi; a weight is 0 where that neighbour is missing or is itself a coarse
+// node, and the index is clamped).  Input dims (a0, a1, a2), output dims (b0, b1, b2), column
+// strides ldi / ldo.  Work-items run over output entries with b0 fastest, so every load and
+// store of a wavefront covers consecutive addresses (stride 2 at most along x).
+struct AxisXfer {
+  uint32_t a0, a1, a2, b0, b1, b2, nout;
+  FastDiv f0, f1;  // division by b0 and b1
+  int remap;
+};
+
+template <typename T, typename X, int AX, int R>
+__global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const int* __restrict__ fi,
+                                                             const T* __restrict__ wlo,
+                                                             const T* __restrict__ whi,
+                                                             const X* __restrict__ in,
+                                                             X* __restrict__ out, size_t ldi,
+                                                             size_t ldo, int ncol,
+                                                             const int* __restrict__ done) {
+  if (done && *done) return;
+  SDC_NODE_PROLOGUE_R(t.nout, t.remap)
+  if (!live) return;
+  const uint32_t q = fdiv(n, t.f0);
+  const uint32_t o0 = n - q * t.b0;
+  const uint32_t o2 = fdiv(q, t.f1);
+  const uint32_t o1 = q - o2 * t.b1;
+  const uint32_t C = AX == 0 ? o0 : AX == 1 ? o1 : o2;
+  const uint32_t na = AX == 0 ? t.a0 : AX == 1 ? t.a1 : t.a2;
+  const uint32_t stride = AX == 0 ? 1u : AX == 1 ? t.a0 : t.a0 * t.a1;
+  const uint32_t f = (uint32_t)fi[C];
+  const T wl = wlo[C], wh = whi[C];
+  const uint32_t fl = f > 0 ? f - 1 : 0, fh = f + 1 < na ? f + 1 : f;
+  const uint32_t c0i = AX == 0 ? 0u : o0;
+  const uint32_t c1i = AX == 1 ? 0u : o1;
+  const uint32_t c2i = AX == 2 ? 0u : o2;
+  const uint32_t base = c0i + t.a0 * (c1i + t.a1 * c2i);
+  const uint32_t il = base + fl * stride, im = base + f * stride, ih = base + fh * stride;
+  T v[R][3];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const X* col = col_ptr(in, ldi, c0, r, ncol);
+    v[r][0] = (T)col[il];
+    v[r][1] = (T)col[im];
+    v[r][2] = (T)col[ih];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (c0 + r < ncol)
+      st_once(out + (size_t)(c0 + r) * ldo + n, (X)(wl * v[r][0] + v[r][1] + wh * v[r][2]));
+}
+
 // xf += P xc.  Every fine node reads the 2x2x2 coarse nodes around it; a coincident axis has
 // weight 0 on its second node, whose index still lies inside the coarse grid.
 template <typename T, typename X, int R>

@@ -170,6 +170,7 @@ struct sdc_handle_s {
          *tidg = nullptr;
   bool sigma_set = false;
   sdc_params prm;
+  int wlevel = 0;  // level whose coarse correction runs twice (0: none)
   Work<float> w32;
   Work<double> w64;
   int* h_flag = nullptr;  // pinned host copy of the convergence flag
@@ -415,6 +416,23 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   return SDC_OK;
 }
 
+// W-cycle level: the requested one, or (wcycle_level = -1) the first level with at most 1/16 of
+// the finest level's nodes -- small enough that a second visit costs little, fine enough that
+// its correction still carries most of the smooth error.  Never the coarsest level.
+int pick_wlevel(const sdc_handle_s* h) {
+  const int nl = (int)h->L.size();
+  int l = h->prm.wcycle_level;
+  if (l < 0) {
+    l = 0;
+    for (int k = 1; k < nl - 1; ++k)
+      if ((size_t)h->L[k].nN * 16 <= (size_t)h->L[0].nN) {
+        l = k;
+        break;
+      }
+  }
+  return (l >= 1 && l < nl - 1) ? l : 0;
+}
+
 int pick_r(const sdc_handle_s* h, int ncol) {
   int r = h->prm.rhs_per_thread;
   if (r == 1 || r == 2 || r == 4 || r == 8) return r;
@@ -499,7 +517,10 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   // is one of them and the first output goes where the last one then lands in x; otherwise
   // (finest level of the fp32 solver with bf16 storage) the last step writes x directly.
   constexpr bool sep = !std::is_same<O, S>::value;
-  const int swaps = (nu1 >= 2 ? nu1 - 2 : 0) + nu2;
+  // W-cycle step on one level: the coarse correction runs twice, with nu_post smoothing steps
+  // between the two (the error propagator stays symmetric: S* C S C S)
+  const int ngam = (h->wlevel > 0 && l == h->wlevel) ? 2 : 1;
+  const int swaps = (nu1 >= 2 ? nu1 - 2 : 0) + nu2 * ngam;
   S* const xa = (S*)W.xa[l];
   S* cur;
   S* oth;
@@ -542,58 +563,113 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
     std::swap(cur, oth);
   }
   S* const res = (S*)W.res[l];
-  prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
-  DISPATCH_R(R, hipLaunchKernelGGL((k_residual<T, B, S, R>), g, blk, 0, st, s, b, (const S*)cur,
-                                   res, lv.ld, ncol, done));
-  prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * (sB + 2 * sS));
   const Xfer<T> t = make_xfer<T>(h, l);
   const Level& cl = h->L[l + 1];
   S* const cb = (S*)W.b[l + 1];
   S* const cx = (S*)W.xs[l + 1];
-  prof_begin(h, l == 0 ? SDC_PROF_RESTRICT : -1, st);
   const int RX = pick_rx(h, ncol);
-  DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict<T, S, R>), grid2(cl.nN, ncol, RX), blk, 0, st, t,
-                                    (const S*)res, cb, lv.ld, cl.ld, ncol, done));
-  prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, (nodes + (double)cl.nN) * cols * sS);
-  CK((vcycle<T, S, S, S>(h, l + 1, cb, cx, ncol, false, done, st)));
-  prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
-  DISPATCH_R(RX, hipLaunchKernelGGL((k_prolong_add<T, S, R>), grid2(lv.nN, ncol, RX), blk, 0, st,
-                                    t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
-  prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sS);
-  for (int k = 0; k < nu2; ++k) {
-    const bool last = k == nu2 - 1;
-    double* part = (last && dot_out) ? W.partial : nodot;
-    const T c1 = (T)(k == 0 ? 0.0 : cc.c1[k]), c2 = (T)cc.c2[k];
-    prof_begin(h, ptag, st);
-    // PREV = 0 on the first step (no previous iterate), 1 after
+  auto post_smooth = [&](bool fin) {
+    for (int k = 0; k < nu2; ++k) {
+      const bool last = k == nu2 - 1 && fin;
+      double* part = (last && dot_out) ? W.partial : nodot;
+      const T c1 = (T)(k == 0 ? 0.0 : cc.c1[k]), c2 = (T)cc.c2[k];
+      prof_begin(h, ptag, st);
+      // PREV = 0 on the first step (no previous iterate), 1 after
 #define SDC_CHEB_STEP(PREV, OT, OUT)                                                           \
-    if (part) {                                                                                \
-      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, OT, R, PREV, true>), g, blk, 0, st, s, \
-                                       b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol, c1,  \
-                                       c2, cz, part, done));                                   \
-    } else {                                                                                   \
-      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, OT, R, PREV, false>), g, blk, 0, st,   \
-                                       s, b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol,   \
-                                       c1, c2, cz, part, done));                               \
-    }
-    if (sep && last) {
-      if (k == 0) {
-        SDC_CHEB_STEP(0, O, x)
-      } else {
-        SDC_CHEB_STEP(1, O, x)
+      if (part) {                                                                                \
+        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, OT, R, PREV, true>), g, blk, 0, st, s, \
+                                         b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol, c1,  \
+                                         c2, cz, part, done));                                   \
+      } else {                                                                                   \
+        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, OT, R, PREV, false>), g, blk, 0, st,   \
+                                         s, b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol,   \
+                                         c1, c2, cz, part, done));                               \
       }
-    } else {
-      if (k == 0) {
-        SDC_CHEB_STEP(0, S, oth)
+      if (sep && last) {
+        if (k == 0) {
+          SDC_CHEB_STEP(0, O, x)
+        } else {
+          SDC_CHEB_STEP(1, O, x)
+        }
       } else {
-        SDC_CHEB_STEP(1, S, oth)
+        if (k == 0) {
+          SDC_CHEB_STEP(0, S, oth)
+        } else {
+          SDC_CHEB_STEP(1, S, oth)
+        }
       }
-    }
 #undef SDC_CHEB_STEP
-    prof_end(h, ptag, st,
-             nodes * 5 * sT + nodes * cols * (sB + sS * (k == 0 ? 1 : 2) + (sep && last ? sO : sS)));
-    std::swap(cur, oth);
+      prof_end(h, ptag, st,
+               nodes * 5 * sT +
+                   nodes * cols * (sB + sS * (k == 0 ? 1 : 2) + (sep && last ? sO : sS)));
+      std::swap(cur, oth);
+    }
+  };
+  for (int gam = 0; gam < ngam; ++gam) {
+    prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_residual<T, B, S, R>), g, blk, 0, st, s, b, (const S*)cur,
+                                     res, lv.ld, ncol, done));
+    prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * (sB + 2 * sS));
+    prof_begin(h, l == 0 ? SDC_PROF_RESTRICT : -1, st);
+    if (h->prm.xfer_separable) {
+      // P^T one axis at a time; the intermediates ping-pong between res and the smoother buffer
+      // that the post-smoothing overwrites anyway
+      uint32_t dims[3] = {lv.n1x, lv.n1y, lv.n1z};
+      const uint32_t cdims[3] = {cl.n1x, cl.n1y, cl.n1z};
+      int axes[3], na = 0;
+      for (int a = 0; a < 3; ++a)
+        if (cdims[a] != dims[a]) axes[na++] = a;
+      const S* src = res;
+      double moved = 0.0;
+      for (int p = 0; p < na; ++p) {
+        const int a = axes[p];
+        const bool lastp = p == na - 1;
+        S* dst = lastp ? cb : (src == res ? oth : res);
+        AxisXfer ax;
+        ax.a0 = dims[0]; ax.a1 = dims[1]; ax.a2 = dims[2];
+        dims[a] = cdims[a];
+        ax.b0 = dims[0]; ax.b1 = dims[1]; ax.b2 = dims[2];
+        ax.nout = ax.b0 * ax.b1 * ax.b2;
+        ax.f0 = make_fastdiv(ax.b0);
+        ax.f1 = make_fastdiv(ax.b1);
+        ax.remap = h->prm.xcd_remap != 0;
+        const size_t ldo = lastp ? cl.ld : lv.ld;
+        const int* fi = a == 0 ? t.rix : a == 1 ? t.riy : t.riz;
+        const T* wl = a == 0 ? t.rlx : a == 1 ? t.rly : t.rlz;
+        const T* wh = a == 0 ? t.rhx : a == 1 ? t.rhy : t.rhz;
+        const dim3 ga = grid2(ax.nout, ncol, RX);
+        switch (a) {
+          case 0:
+            DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict_axis<T, S, 0, R>), ga, blk, 0, st, ax, fi,
+                                              wl, wh, src, dst, lv.ld, ldo, ncol, done));
+            break;
+          case 1:
+            DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict_axis<T, S, 1, R>), ga, blk, 0, st, ax, fi,
+                                              wl, wh, src, dst, lv.ld, ldo, ncol, done));
+            break;
+          default:
+            DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict_axis<T, S, 2, R>), ga, blk, 0, st, ax, fi,
+                                              wl, wh, src, dst, lv.ld, ldo, ncol, done));
+            break;
+        }
+        moved += ((double)ax.a0 * ax.a1 * ax.a2 + (double)ax.nout) * cols * sS;
+        src = dst;
+      }
+      prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, moved);
+    } else {
+      DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict<T, S, R>), grid2(cl.nN, ncol, RX), blk, 0, st,
+                                        t,
+                                        (const S*)res, cb, lv.ld, cl.ld, ncol, done));
+      prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, (nodes + (double)cl.nN) * cols * sS);
+    }
+    CK((vcycle<T, S, S, S>(h, l + 1, cb, cx, ncol, false, done, st)));
+    prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
+    DISPATCH_R(RX, hipLaunchKernelGGL((k_prolong_add<T, S, R>), grid2(lv.nN, ncol, RX), blk, 0, st,
+                                      t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
+    prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sS);
+    if (gam + 1 < ngam) post_smooth(false);
   }
+  post_smooth(true);
   if constexpr (!sep) {
     if (cur != (S*)x) return fail(SDC_ERR_STATE, "internal: V-cycle buffer parity");
   }
@@ -819,8 +895,10 @@ int sdc_default_params(sdc_params* p) {
   p->xfer_rhs_per_thread = 0;
   p->xcd_remap = 1;
   p->vcycle_bf16 = 1;
-  p->check_every = 4;
-  p->reliable_delta = 1e-2;
+  p->xfer_separable = 1;
+  p->wcycle_level = -1;
+  p->check_every = 1;
+  p->reliable_delta = 1e-3;
   return SDC_OK;
 }
 
@@ -898,6 +976,7 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
     h->L.push_back(c);
   }
   if (h->L.back().nN > 4096) return bail(fail(SDC_ERR_ARG, "coarsest multigrid level too large"));
+  h->wlevel = pick_wlevel(h);
   for (auto& v : h->L)
     if ((rc = alloc_level_arrays(v))) return bail(rc);
   h->ncoarse = (int)h->L.back().nN;
@@ -954,10 +1033,14 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
   if (p->xcd_remap != 0 && p->xcd_remap != 1) return fail(SDC_ERR_ARG, "xcd_remap must be 0 or 1");
   if (p->vcycle_bf16 != 0 && p->vcycle_bf16 != 1)
     return fail(SDC_ERR_ARG, "vcycle_bf16 must be 0 or 1");
+  if (p->xfer_separable != 0 && p->xfer_separable != 1)
+    return fail(SDC_ERR_ARG, "xfer_separable must be 0 or 1");
+  if (p->wcycle_level < -1) return fail(SDC_ERR_ARG, "wcycle_level must be >= -1");
   if (!(p->reliable_delta > 0) || !(p->reliable_delta < 1))
     return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
   h->prm = *p;
   if (h->prm.check_every < 1) h->prm.check_every = 1;
+  h->wlevel = pick_wlevel(h);
   return SDC_OK;
 }
 

@@ -45,12 +45,15 @@ def test_apply_matches_assembled_operator(gpu, tiny, bc):
     assert rel(y32, A @ x) < 2e-6
 
 
-def test_vcycle_is_symmetric(gpu, tiny):
+@pytest.mark.parametrize("wlevel", [0, 1])
+def test_vcycle_is_symmetric(gpu, tiny, wlevel):
     mesh, survey, m = tiny
     sim = _sim(mesh, survey)
     sim.model = m
     sim._ensure_sigma()
     eng = sim.engine
+    assert eng.levels >= 3
+    eng.set_params(wcycle_level=wlevel)
     rng = np.random.default_rng(2)
     a = rng.standard_normal((mesh.n_nodes, 2))
     b = rng.standard_normal((mesh.n_nodes, 2))
