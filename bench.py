@@ -126,6 +126,45 @@ class Pipeline:
                     res=rf)
 
 
+# library profiler class -> kernel (finest level, fp32 solver with bf16 multigrid vectors)
+KERNEL_NAMES = {
+    "update": "k_update_xr4<float>", "smooth": "k_cheb<float,float,bf16,float,8,1,true>",
+    "apply": "k_apply<float,8,true>", "pair": "k_smooth_pair<float,float,bf16,8>",
+    "residual": "k_residual<float,float,bf16,8>", "smooth0": "k_cheb<float,float,bf16,bf16,8,0,false>",
+    "prolong": "k_prolong_add<float,bf16,8>", "restrict": "k_restrict_axis<float,bf16,*,8>",
+    "update_p": "k_update_p4<float,false>", "refine": "k_ir_residual<float,4,false>",
+    "fold": "k_ir_fold4", "jtvec": "k_jtvec<0>",
+}
+
+
+def measured_traffic(tag, workload):
+    """HBM bytes per launch of one kernel class from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{workload}.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith(f"traffic[{tag}]"):
+                    return int(line.split(":", 1)[1].split()[0])
+    except OSError:
+        return None
+    return None
+
+
+def calibrate_copy(dev):
+    """A plain device-to-device copy of 256 MiB, three times: the byte count that FETCH_SIZE and
+    WRITE_SIZE should report per launch is 268435456."""
+    import torch
+
+    n = 256 * 1024 * 1024 // 4
+    x = torch.ones(n, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_per_launch": n * 4, "launches": 3}))
+
+
 def rel(a, b):
     a = a.double()
     b = b.double()
@@ -269,8 +308,12 @@ def main():
     ap.add_argument("--gn-beta", type=float, default=1e-2)
     ap.add_argument("--gn-recompute", action="store_true")
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--profile-kernel", default="smooth",
-                    choices=["smooth", "apply", "residual", "refine", "jtvec", "restrict", "prolong"])
+    ap.add_argument("--profile-kernel", default="update",
+                    choices=["smooth", "apply", "residual", "refine", "jtvec", "restrict", "prolong",
+                             "update", "pair", "smooth0", "update_p", "fold"],
+                    help="kernel class timed with HIP events during the timed steps (roofline)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (PMC byte calibration)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain fp64 per-source baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the fp64 reference solve")
@@ -289,6 +332,8 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(2)
     dev = torch.device("cuda", 0)
+    if args.calibrate_copy:
+        return calibrate_copy(dev)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "gn":
@@ -320,7 +365,19 @@ def main():
             f"{r['res'].outer} refinements, rel res {r['res'].max_rel_res:.2e}), "
             f"jtvec {r['t_adj']*1e3:.1f} ms ({r['it_adj']} it)")
     torch.cuda.synchronize()
-    nlaunch, kms, kbytes = eng.profile_read()
+    nlaunch, kms, kbytes = eng.profile_read(args.profile_kernel)
+    eng.profile(None)
+    # per-kernel table from one more (untimed) step with every kernel class bracketed
+    eng.profile("all")
+    pl.step("mixed", args.rtol)
+    torch.cuda.synchronize()
+    kernels = {}
+    for tag in ("update", "smooth", "apply", "pair", "residual", "smooth0", "prolong", "restrict",
+                "update_p", "refine", "fold", "jtvec"):
+        n, ms, by = eng.profile_read(tag)
+        if n:
+            kernels[tag] = {"launches": n, "ms": round(ms, 3), "avg_us": round(1e3 * ms / n, 1),
+                            "GB_per_s": round(by / (ms * 1e-3) / 1e9, 1)}
     eng.profile(None)
     nS = wl.survey.nSrc
     K = args.steps
@@ -350,12 +407,14 @@ def main():
     avg_ms = kms / max(nlaunch, 1)
     avg_bytes = kbytes / max(nlaunch, 1)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if nlaunch else 0.0
+    traffic = measured_traffic(args.profile_kernel, wl.name)
     out["roofline"] = {
-        "bound": "hbm", "kernel": args.profile_kernel, "launches": nlaunch,
+        "bound": "hbm", "kernel": KERNEL_NAMES[args.profile_kernel], "launches": nlaunch,
         "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": avg_bytes,
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
     }
+    out["kernels"] = kernels
 
     if not args.no_accuracy:
         ref_u, ref_res = eng.solve(pl.q, nS, mode="fp64", tol=1e-13, max_iter=2000)

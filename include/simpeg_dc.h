@@ -178,20 +178,26 @@ int sdc_scatter(sdc_handle h, int nrow, const int64_t* dst_dev, const int32_t* r
                 double* out_dev, int ncol, void* stream);
 
 /* Event profiler (no reference counterpart; instrumentation for the roofline report).
- * sdc_profile(h, tag) selects one kernel class; each later launch of it is bracketed by HIP
- * events on its stream.  sdc_profile_read returns the number of launches, their summed
- * duration (ms) and summed algorithmic bytes (the minimum HBM traffic of each launch: every
- * array element it must read or write counted once), then clears the record.  tag 0 = off. */
-#define SDC_PROF_OFF 0
-#define SDC_PROF_SMOOTH 1     /* level-0 Chebyshev smoothing step (stencil + update) */
-#define SDC_PROF_APPLY 2      /* level-0 operator application inside PCG (q = A p, p.q) */
-#define SDC_PROF_RESIDUAL 3   /* level-0 V-cycle residual */
-#define SDC_PROF_REFINE 4     /* fp64 refinement residual q - A u */
+ * sdc_profile(h, mask) selects kernel classes (bit 1 << tag per class); each later launch of a
+ * selected class is bracketed by HIP events on its stream.  sdc_profile_read(h, tag, ...)
+ * returns for one class the number of launches, their summed duration (ms) and summed
+ * algorithmic bytes (the minimum HBM traffic of each launch: every array element it must
+ * read or write counted once), then clears that record.  mask 0 = off.  Finest level only. */
+#define SDC_PROF_SMOOTH 1     /* last post-smoothing step (writes z and the r.z partials) */
+#define SDC_PROF_APPLY 2      /* operator application inside PCG (q = A p, p.q partials) */
+#define SDC_PROF_RESIDUAL 3   /* V-cycle residual */
+#define SDC_PROF_REFINE 4     /* fp64 residual q - A u of a reliable update or refinement step */
 #define SDC_PROF_JTVEC 5      /* adjoint edge-gradient contraction */
-#define SDC_PROF_RESTRICT 6   /* restriction level 0 -> 1 */
-#define SDC_PROF_PROLONG 7    /* interpolation level 1 -> 0 */
-int sdc_profile(sdc_handle h, int tag);
-int sdc_profile_read(sdc_handle h, int64_t* count, double* total_ms, double* total_bytes);
+#define SDC_PROF_RESTRICT 6   /* one per-axis restriction pass (or the 27-point gather) */
+#define SDC_PROF_PROLONG 7    /* interpolation from level 1, added into the smoother iterate */
+#define SDC_PROF_UPDATE 8     /* PCG x += alpha p, r -= alpha q and |r|^2 partials */
+#define SDC_PROF_PAIR 9       /* fused first two pre-smoothing steps */
+#define SDC_PROF_SMOOTH0 10   /* first post-smoothing step */
+#define SDC_PROF_UPDATE_P 11  /* PCG p = z + beta p */
+#define SDC_PROF_FOLD 12      /* reliable update u += x, x = 0 */
+#define SDC_PROF_NTAGS 13
+int sdc_profile(sdc_handle h, uint32_t mask);
+int sdc_profile_read(sdc_handle h, int tag, int64_t* count, double* total_ms, double* total_bytes);
 
 #ifdef __cplusplus
 }

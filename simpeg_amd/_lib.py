@@ -98,13 +98,13 @@ _SIGS = {
     "sdc_jtj_diag_accumulate": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
     "sdc_gather": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sdc_scatter": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
-    "sdc_profile": (C.c_int, [_vp, C.c_int]),
-    "sdc_profile_read": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+    "sdc_profile": (C.c_int, [_vp, C.c_uint32]),
+    "sdc_profile_read": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]),
 }
 
 PROF_TAGS = {"smooth": 1, "apply": 2, "residual": 3, "refine": 4, "jtvec": 5, "restrict": 6,
-             "prolong": 7}
+             "prolong": 7, "update": 8, "pair": 9, "smooth0": 10, "update_p": 11, "fold": 12}
 
 EXPORTED = tuple(_SIGS)
 

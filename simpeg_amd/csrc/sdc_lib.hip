@@ -175,10 +175,12 @@ struct sdc_handle_s {
   Work<double> w64;
   int* h_flag = nullptr;  // pinned host copy of the convergence flag
   // event profiler for one kernel class (sdc_profile)
-  int prof_tag = 0;
-  std::vector<hipEvent_t> prof_ev;
-  size_t prof_used = 0;
-  std::vector<double> prof_bytes;
+  uint32_t prof_mask = 0;
+  struct ProfRec {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    std::vector<double> bytes;
+  } prof[SDC_PROF_NTAGS];
 };
 
 namespace {
@@ -195,22 +197,29 @@ struct DeviceGuard {
   }
 };
 
-// Brackets one launch of the kernel class selected by sdc_profile with events on its stream.
+// Brackets one launch of a kernel class selected by sdc_profile with events on its stream.
+bool prof_on(const sdc_handle_s* h, int tag) {
+  return tag > 0 && tag < SDC_PROF_NTAGS && (h->prof_mask & (1u << tag));
+}
+
 void prof_begin(sdc_handle_s* h, int tag, hipStream_t st) {
-  if (h->prof_tag != tag) return;
-  while (h->prof_ev.size() < h->prof_used + 2) {
+  if (!prof_on(h, tag)) return;
+  auto& p = h->prof[tag];
+  while (p.ev.size() < p.used + 2) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    h->prof_ev.push_back(e);
+    p.ev.push_back(e);
   }
-  (void)hipEventRecord(h->prof_ev[h->prof_used], st);
+  (void)hipEventRecord(p.ev[p.used], st);
 }
 
 void prof_end(sdc_handle_s* h, int tag, hipStream_t st, double bytes) {
-  if (h->prof_tag != tag || h->prof_ev.size() < h->prof_used + 2) return;
-  (void)hipEventRecord(h->prof_ev[h->prof_used + 1], st);
-  h->prof_used += 2;
-  h->prof_bytes.push_back(bytes);
+  if (!prof_on(h, tag)) return;
+  auto& p = h->prof[tag];
+  if (p.ev.size() < p.used + 2) return;
+  (void)hipEventRecord(p.ev[p.used + 1], st);
+  p.used += 2;
+  p.bytes.push_back(bytes);
 }
 
 template <typename T>
@@ -535,21 +544,19 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   const dim3 blk(SDC_BLOCK);
   const double nodes = (double)lv.nN, cols = (double)ncol, sT = (double)sizeof(T);
   const double sS = (double)sizeof(S), sB = (double)sizeof(B), sO = (double)sizeof(O);
-  const int ptag = (l == 0) ? SDC_PROF_SMOOTH : -1;
   const T cz = (T)cc.c2[0];
   double* const nodot = nullptr;
   if (nu1 >= 2) {
     // steps 0 and 1 fused
-    prof_begin(h, ptag, st);
+    prof_begin(h, l == 0 ? SDC_PROF_PAIR : -1, st);
     DISPATCH_R(R, hipLaunchKernelGGL((k_smooth_pair<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
                                      ncol, cz, (T)cc.c1[1], (T)cc.c2[1], done));
-    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * (sB + sS));
+    prof_end(h, l == 0 ? SDC_PROF_PAIR : -1, st, nodes * 5 * sT + nodes * cols * (sB + sS));
   } else {
     DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
                                      ncol, cz, done));
   }
   for (int k = 2; k < nu1; ++k) {
-    prof_begin(h, ptag, st);
     if (k == 2) {
       DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, S, R, 2, false>), g, blk, 0, st, s, b,
                                        (const S*)cur, (const S*)nullptr, oth, lv.ld, ncol,
@@ -559,7 +566,6 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
                                        (const S*)cur, (const S*)oth, oth, lv.ld, ncol,
                                        (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
     }
-    prof_end(h, ptag, st, nodes * 5 * sT + nodes * cols * (sB + sS * (k == 2 ? 2 : 3)));
     std::swap(cur, oth);
   }
   S* const res = (S*)W.res[l];
@@ -573,6 +579,8 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
       const bool last = k == nu2 - 1 && fin;
       double* part = (last && dot_out) ? W.partial : nodot;
       const T c1 = (T)(k == 0 ? 0.0 : cc.c1[k]), c2 = (T)cc.c2[k];
+      const int ptag =
+          (l != 0) ? -1 : last ? SDC_PROF_SMOOTH : (k == 0 && fin) ? SDC_PROF_SMOOTH0 : -1;
       prof_begin(h, ptag, st);
       // PREV = 0 on the first step (no previous iterate), 1 after
 #define SDC_CHEB_STEP(PREV, OT, OUT)                                                           \
@@ -610,7 +618,7 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
     DISPATCH_R(R, hipLaunchKernelGGL((k_residual<T, B, S, R>), g, blk, 0, st, s, b, (const S*)cur,
                                      res, lv.ld, ncol, done));
     prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * (sB + 2 * sS));
-    prof_begin(h, l == 0 ? SDC_PROF_RESTRICT : -1, st);
+    const int rtag = l == 0 ? SDC_PROF_RESTRICT : -1;
     if (h->prm.xfer_separable) {
       // P^T one axis at a time; the intermediates ping-pong between res and the smoother buffer
       // that the post-smoothing overwrites anyway
@@ -620,7 +628,6 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
       for (int a = 0; a < 3; ++a)
         if (cdims[a] != dims[a]) axes[na++] = a;
       const S* src = res;
-      double moved = 0.0;
       for (int p = 0; p < na; ++p) {
         const int a = axes[p];
         const bool lastp = p == na - 1;
@@ -638,6 +645,7 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
         const T* wl = a == 0 ? t.rlx : a == 1 ? t.rly : t.rlz;
         const T* wh = a == 0 ? t.rhx : a == 1 ? t.rhy : t.rhz;
         const dim3 ga = grid2(ax.nout, ncol, RX);
+        prof_begin(h, rtag, st);
         switch (a) {
           case 0:
             DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict_axis<T, S, 0, R>), ga, blk, 0, st, ax, fi,
@@ -652,15 +660,15 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
                                               wl, wh, src, dst, lv.ld, ldo, ncol, done));
             break;
         }
-        moved += ((double)ax.a0 * ax.a1 * ax.a2 + (double)ax.nout) * cols * sS;
+        prof_end(h, rtag, st, ((double)ax.a0 * ax.a1 * ax.a2 + (double)ax.nout) * cols * sS);
         src = dst;
       }
-      prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, moved);
     } else {
+      prof_begin(h, rtag, st);
       DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict<T, S, R>), grid2(cl.nN, ncol, RX), blk, 0, st,
                                         t,
                                         (const S*)res, cb, lv.ld, cl.ld, ncol, done));
-      prof_end(h, l == 0 ? SDC_PROF_RESTRICT : -1, st, (nodes + (double)cl.nN) * cols * sS);
+      prof_end(h, rtag, st, (nodes + (double)cl.nN) * cols * sS);
     }
     CK((vcycle<T, S, S, S>(h, l + 1, cb, cx, ncol, false, done, st)));
     prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
@@ -726,9 +734,11 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
              (double)l0.nN * 4 * sizeof(T) + (double)l0.nN * ncol * 2 * sizeof(T));
     hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                        (const double*)W.rz, W.alpha, (const int*)W.active, (const int*)done);
+    prof_begin(h, SDC_PROF_UPDATE, st);
     hipLaunchKernelGGL(k_update_xr4<T>, grid4(l0.nN, ncol), blk, 0, st, x, (const T*)W.p, W.r,
                        (const T*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
                        (const int*)done);
+    prof_end(h, SDC_PROF_UPDATE, st, (double)l0.nN * ncol * 6 * sizeof(T));
     hipLaunchKernelGGL(k_fin_rr, dim3(ncol), blk, 0, st, (const double*)W.partial, nblk4(l0.nN),
                        (const double*)W.bn2, tol2, W.rr, W.active, W.counters, done,
                        (int*)nullptr, (int)(it == maxit));
@@ -743,8 +753,10 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
     CK(vcycle0<T>(h, W.r, W.z, ncol, true, done, st));
     hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                        W.beta, (const int*)W.active, 0, (const int*)done);
+    prof_begin(h, SDC_PROF_UPDATE_P, st);
     hipLaunchKernelGGL((k_update_p4<T, false>), grid4(l0.nN, ncol), blk, 0, st, (const T*)W.z, W.p,
                        (const double*)W.beta, l0.ld, l0.nN, (const int*)done);
+    prof_end(h, SDC_PROF_UPDATE_P, st, (double)l0.nN * ncol * 3 * sizeof(T));
   }
   HIPCK(hipGetLastError());
   if (iters) *iters = std::min(it, maxit);
@@ -819,9 +831,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                        (const double*)W.rz, W.alpha, (const int*)W.active,
                        (const int*)(W.counters + 3));
+    prof_begin(h, SDC_PROF_UPDATE, st);
     hipLaunchKernelGGL(k_update_xr4<float>, grid4(l0.nN, ncol), blk, 0, st, W.x, (const float*)W.p,
                        W.r, (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
                        (const int*)nullptr);
+    prof_end(h, SDC_PROF_UPDATE, st, (double)l0.nN * ncol * 24);
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nblk4(l0.nN),
                        W.rr, W.active, 0);
     if (it % check == 0 || it == maxit) {
@@ -834,16 +848,18 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
         if (h_rr[c] <= delta2 * h_rmax[c]) drop = true;
       }
       if (cand || drop || it == maxit) {
-        prof_begin(h, SDC_PROF_REFINE, st);
+        prof_begin(h, SDC_PROF_FOLD, st);
         hipLaunchKernelGGL(k_ir_fold4, grid4(l0.nN, ncol), blk, 0, st, u, W.x, l0.ld, l0.nN);
+        prof_end(h, SDC_PROF_FOLD, st, (double)l0.nN * ncol * 24);
+        prof_begin(h, SDC_PROF_REFINE, st);
         // fp64 stencil: 4 columns per work-item keep the neighbouring planes inside L2
         const int R64 = std::min(R, 4);
         DISPATCH_R(R64, hipLaunchKernelGGL((k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64),
                                            blk, 0, st, s2, q, (const double*)u, W.r, l0.ld, ncol,
                                            pf, W.partial, (unsigned*)W.qmask));
-        // algorithmic bytes: fp64 coefficients + per column fold (x32, u64 read+write) and
-        // residual (u64 read, r32 write); q only on source nodes (not counted)
-        prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 36);
+        // algorithmic bytes: fp64 coefficients, per column u64 read and r32 write; q is read
+        // on source nodes only (not counted)
+        prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
         hipLaunchKernelGGL(k_fin_true, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                            (const double*)W.aux, tol2, W.orr, W.rr, W.active);
         HIPCK(hipMemcpyAsync(h_rr.data(), W.orr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -864,9 +880,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     CK(vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, st));
     hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                        W.beta, (const int*)W.active, 0, (const int*)(W.counters + 3));
+    prof_begin(h, SDC_PROF_UPDATE_P, st);
     hipLaunchKernelGGL((k_update_p4<float, false>), grid4(l0.nN, ncol), blk, 0, st,
                        (const float*)W.z, W.p, (const double*)W.beta, l0.ld, l0.nN,
                        (const int*)nullptr);
+    prof_end(h, SDC_PROF_UPDATE_P, st, (double)l0.nN * ncol * 12);
   }
   S.iters = std::min(it, maxit);
   S.converged = nact == 0;
@@ -1008,7 +1026,8 @@ int sdc_destroy(sdc_handle h) {
   h->w32.release();
   h->w64.release();
   if (h->h_flag) (void)hipHostFree(h->h_flag);
-  for (auto e : h->prof_ev) (void)hipEventDestroy(e);
+  for (auto& p : h->prof)
+    for (auto e : p.ev) (void)hipEventDestroy(e);
   delete h;
   return SDC_OK;
 }
@@ -1313,32 +1332,36 @@ int sdc_jtj_diag_accumulate(sdc_handle h, const double* u, const double* lam, in
   return SDC_OK;
 }
 
-int sdc_profile(sdc_handle h, int tag) {
+int sdc_profile(sdc_handle h, uint32_t mask) {
   if (!h) return fail(SDC_ERR_ARG, "null handle");
-  if (tag < 0 || tag > SDC_PROF_PROLONG) return fail(SDC_ERR_ARG, "unknown profile tag");
-  h->prof_tag = tag;
-  h->prof_used = 0;
-  h->prof_bytes.clear();
+  if (mask & ~(((1u << SDC_PROF_NTAGS) - 1u) & ~1u)) return fail(SDC_ERR_ARG, "unknown profile tag");
+  h->prof_mask = mask;
+  for (auto& p : h->prof) {
+    p.used = 0;
+    p.bytes.clear();
+  }
   return SDC_OK;
 }
 
-int sdc_profile_read(sdc_handle h, int64_t* count, double* total_ms, double* total_bytes) {
+int sdc_profile_read(sdc_handle h, int tag, int64_t* count, double* total_ms, double* total_bytes) {
   if (!h || !count || !total_ms || !total_bytes) return fail(SDC_ERR_ARG, "null argument");
+  if (tag <= 0 || tag >= SDC_PROF_NTAGS) return fail(SDC_ERR_ARG, "unknown profile tag");
   DeviceGuard dg(h->device);
+  auto& p = h->prof[tag];
   double ms = 0.0, by = 0.0;
-  const size_t n = h->prof_used / 2;
+  const size_t n = p.used / 2;
   for (size_t i = 0; i < n; ++i) {
-    HIPCK(hipEventSynchronize(h->prof_ev[2 * i + 1]));
+    HIPCK(hipEventSynchronize(p.ev[2 * i + 1]));
     float t = 0.f;
-    HIPCK(hipEventElapsedTime(&t, h->prof_ev[2 * i], h->prof_ev[2 * i + 1]));
+    HIPCK(hipEventElapsedTime(&t, p.ev[2 * i], p.ev[2 * i + 1]));
     ms += t;
-    by += h->prof_bytes[i];
+    by += p.bytes[i];
   }
   *count = (int64_t)n;
   *total_ms = ms;
   *total_bytes = by;
-  h->prof_used = 0;
-  h->prof_bytes.clear();
+  p.used = 0;
+  p.bytes.clear();
   return SDC_OK;
 }
 

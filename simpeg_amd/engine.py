@@ -179,16 +179,28 @@ class NodalEngine:
             _lib.check(rc, "sdc_solve")
         return u, res
 
-    def profile(self, tag):
-        """Bracket launches of one kernel class with events ("smooth", "apply", ...; None = off)."""
-        t = 0 if tag is None else _lib.PROF_TAGS[tag]
-        _lib.check(self.lib.sdc_profile(self.h, t), "sdc_profile")
+    def profile(self, tags):
+        """Bracket launches of kernel classes with events: a name ("smooth", "apply", ...), a
+        list of names, "all", or None (off)."""
+        if tags is None:
+            names = []
+        elif tags == "all":
+            names = list(_lib.PROF_TAGS)
+        elif isinstance(tags, str):
+            names = [tags]
+        else:
+            names = list(tags)
+        mask = 0
+        for n in names:
+            mask |= 1 << _lib.PROF_TAGS[n]
+        _lib.check(self.lib.sdc_profile(self.h, mask), "sdc_profile")
 
-    def profile_read(self):
-        """(launches, total ms, total algorithmic bytes) since profile() or the last read."""
+    def profile_read(self, tag):
+        """(launches, total ms, total algorithmic bytes) of one kernel class since profile() or
+        the last read of that class."""
         n, ms, by = C.c_int64(), C.c_double(), C.c_double()
-        _lib.check(self.lib.sdc_profile_read(self.h, C.byref(n), C.byref(ms), C.byref(by)),
-                   "sdc_profile_read")
+        _lib.check(self.lib.sdc_profile_read(self.h, _lib.PROF_TAGS[tag], C.byref(n), C.byref(ms),
+                                             C.byref(by)), "sdc_profile_read")
         return int(n.value), float(ms.value), float(by.value)
 
     def jvec_rhs(self, dsig, u, ncol=None):
