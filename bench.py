@@ -126,6 +126,13 @@ class Pipeline:
                     res=rf)
 
 
+# settings tried for the plain fp64 per-source baseline (the fastest is reported)
+FP64_BASELINE_CANDIDATES = [
+    {"wcycle_level": 0, "check_every": 1},
+    {"wcycle_level": 0, "check_every": 2},
+    {"wcycle_level": -1, "check_every": 1},
+]
+
 # library profiler class -> kernel (finest level, fp32 solver with bf16 multigrid vectors)
 KERNEL_NAMES = {
     "update": "k_update_xr4<float>", "smooth": "k_cheb<float,float,bf16,float,8,1,true>",
@@ -429,7 +436,19 @@ def main():
         log(f"[bench] accuracy {out['accuracy']}")
 
     if not args.no_fp64:
-        r64 = pl.step("fp64", args.rtol, per_source=True)
+        # The baseline runs one source at a time, where kernel launches dominate: it gets the
+        # multigrid cycle and poll cadence that suit it best (the fastest of a few).
+        base = {f: getattr(eng.params, f) for f, _ in eng.params._fields_}
+        best = None
+        for cand in FP64_BASELINE_CANDIDATES:
+            eng.set_params(**cand)
+            r = pl.step("fp64", args.rtol, per_source=True)
+            log(f"[bench] fp64 per-source {cand}: fwd {1e3*r['t_fwd']:.1f} ms, "
+                f"jtvec {1e3*r['t_adj']:.1f} ms")
+            if best is None or r["t_fwd"] + r["t_adj"] < best[1]["t_fwd"] + best[1]["t_adj"]:
+                best = (cand, r)
+        eng.set_params(**base)
+        cand64, r64 = best
         fwd64 = r64["t_fwd"]
         adj64 = r64["t_adj"]
         out["fp64_per_source"] = {
@@ -437,6 +456,7 @@ def main():
             "forward_ms": round(1e3 * fwd64, 1),
             "jtvec_ms": round(1e3 * adj64, 1),
             "pcg_iterations": {"fwd": r64["it_fwd"], "adj": r64["it_adj"]},
+            "params": cand64,
             "speedup_fwd_plus_jtvec": round((fwd64 + adj64) / ((t_fwd + t_adj) / K), 2),
         }
         if not args.no_accuracy:
