@@ -90,6 +90,8 @@ typedef struct {
   int32_t wcycle_level;        /* 0: V-cycle; l >= 1: the coarse correction of level l runs twice
                                   (W-cycle on that level); -1: the first level with at most 1/16
                                   of the finest level's nodes */
+  int32_t fused_tail;          /* 1: the fp32 solver runs the small coarse levels that fit into
+                                  LDS as one launch (one work-group per column); 0: off */
 } sdc_params;
 
 typedef struct {

@@ -59,6 +59,7 @@ class Params(C.Structure):
         ("vcycle_bf16", C.c_int32),
         ("xfer_separable", C.c_int32),
         ("wcycle_level", C.c_int32),
+        ("fused_tail", C.c_int32),
     ]
 
 

@@ -1,6 +1,7 @@
 // Host side of libsimpeg_dc.so: handle, multigrid hierarchy, PCG / V-cycle / iterative
 // refinement drivers and the C ABI declared in include/simpeg_dc.h.
 #include "sdc_kernels.h"
+#include "sdc_tail.h"
 #include "simpeg_dc.h"
 
 #include <algorithm>
@@ -171,6 +172,8 @@ struct sdc_handle_s {
   bool sigma_set = false;
   sdc_params prm;
   int wlevel = 0;  // level whose coarse correction runs twice (0: none)
+  int tail_level = 0;    // first level of the single-launch coarse tail (0: none)
+  size_t tail_lds = 0;   // its LDS bytes per work-group
   Work<float> w32;
   Work<double> w64;
   int* h_flag = nullptr;  // pinned host copy of the convergence flag
@@ -442,6 +445,30 @@ int pick_wlevel(const sdc_handle_s* h) {
   return (l >= 1 && l < nl - 1) ? l : 0;
 }
 
+// First level of the coarse tail that one work-group per column runs out of LDS: the finest
+// level below the W-cycle level whose remaining levels fit (rhs + solution per level + one shared
+// scratch vector, fp32) into 160 KB, with at least one level above the coarsest.
+void pick_tail(sdc_handle_s* h) {
+  h->tail_level = 0;
+  h->tail_lds = 0;
+  const int nl = (int)h->L.size();
+  if (!h->prm.fused_tail) return;
+  for (int lt = std::max(1, h->wlevel + 1); lt < nl - 1; ++lt) {
+    if (nl - lt > SDC_TAIL_MAX) continue;
+    size_t words = 0, ymax = 0;
+    for (int l = lt; l < nl; ++l) {
+      words += 2 * (size_t)h->L[l].nN;
+      ymax = std::max(ymax, (size_t)h->L[l].nN);
+    }
+    const size_t bytes = (words + ymax) * sizeof(float);
+    if (bytes <= 160 * 1024) {
+      h->tail_level = lt;
+      h->tail_lds = bytes;
+      return;
+    }
+  }
+}
+
 int pick_r(const sdc_handle_s* h, int ncol) {
   int r = h->prm.rhs_per_thread;
   if (r == 1 || r == 2 || r == 4 || r == 8) return r;
@@ -495,6 +522,45 @@ Cheb cheb_coefs(const sdc_params& p) {
   return c;
 }
 
+// The V-cycle of levels tail_level .. L-1 for every column in one launch (one work-group per
+// column, all vectors in LDS); b and x are the tail level's blocks.
+template <typename S>
+int tail_vcycle(sdc_handle_s* h, const S* b, S* x, int ncol, const int* done, hipStream_t st) {
+  const int nl = (int)h->L.size();
+  const int lt = h->tail_level;
+  TailDesc d;
+  d.nlev = nl - lt;
+  uint32_t off = 0, ymax = 0;
+  for (int k = 0; k < d.nlev; ++k) {
+    const int l = lt + k;
+    d.lv[k].s = make_sten<float>(h, l, 1);
+    if (l < nl - 1) d.lv[k].t = make_xfer<float>(h, l);
+    d.lv[k].offB = off;
+    off += h->L[l].nN;
+    d.lv[k].offX = off;
+    off += h->L[l].nN;
+    ymax = std::max(ymax, h->L[l].nN);
+  }
+  d.offY = off;
+  d.ainv = h->ainv32;
+  d.nn = (int)h->L[nl - 1].nN;
+  d.nu_pre = h->prm.nu_pre;
+  d.nu_post = h->prm.nu_post;
+  const Cheb cc = cheb_coefs(h->prm);
+  for (int k = 0; k < 16; ++k) {
+    d.c1[k] = (float)cc.c1[k];
+    d.c2[k] = (float)cc.c2[k];
+  }
+  const size_t bytes = (size_t)(off + ymax) * sizeof(float);
+  if (bytes != h->tail_lds) return fail(SDC_ERR_STATE, "internal: tail LDS size");
+  HIPCK(hipFuncSetAttribute((const void*)k_tail_vcycle<S>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  hipLaunchKernelGGL(k_tail_vcycle<S>, dim3(ncol), dim3(1024), bytes, st, d, b, x, h->L[lt].ld,
+                     done);
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
 // one V-cycle x = M^-1 b on level l; with dot_out the last kernel writes partials of b.x.
 // T: arithmetic and coefficient type; S: storage type of the V-cycle vectors; B, O: types of
 // b and x (T on the finest level, where b is the PCG residual and x the preconditioned one; S
@@ -506,6 +572,10 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   const int R = pick_r(h, ncol);
   const Level& lv = h->L[l];
   const int nl = (int)h->L.size();
+  if constexpr (std::is_same<T, float>::value && std::is_same<B, S>::value &&
+                std::is_same<O, S>::value) {
+    if (h->tail_level > 0 && l == h->tail_level) return tail_vcycle<S>(h, b, x, ncol, done, st);
+  }
   if (l == nl - 1) {
     const T* ainv = (const T*)(std::is_same<T, float>::value ? (const void*)h->ainv32
                                                               : (const void*)h->ainv64);
@@ -915,6 +985,7 @@ int sdc_default_params(sdc_params* p) {
   p->vcycle_bf16 = 1;
   p->xfer_separable = 1;
   p->wcycle_level = -1;
+  p->fused_tail = 1;
   p->check_every = 1;
   p->reliable_delta = 1e-3;
   return SDC_OK;
@@ -995,6 +1066,7 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
   }
   if (h->L.back().nN > 4096) return bail(fail(SDC_ERR_ARG, "coarsest multigrid level too large"));
   h->wlevel = pick_wlevel(h);
+  pick_tail(h);
   for (auto& v : h->L)
     if ((rc = alloc_level_arrays(v))) return bail(rc);
   h->ncoarse = (int)h->L.back().nN;
@@ -1055,11 +1127,13 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
   if (p->xfer_separable != 0 && p->xfer_separable != 1)
     return fail(SDC_ERR_ARG, "xfer_separable must be 0 or 1");
   if (p->wcycle_level < -1) return fail(SDC_ERR_ARG, "wcycle_level must be >= -1");
+  if (p->fused_tail != 0 && p->fused_tail != 1) return fail(SDC_ERR_ARG, "fused_tail must be 0 or 1");
   if (!(p->reliable_delta > 0) || !(p->reliable_delta < 1))
     return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
   h->prm = *p;
   if (h->prm.check_every < 1) h->prm.check_every = 1;
   h->wlevel = pick_wlevel(h);
+  pick_tail(h);
   return SDC_OK;
 }
 
