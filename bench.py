@@ -135,8 +135,8 @@ FP64_BASELINE_CANDIDATES = [
 
 # library profiler class -> kernel (finest level, fp32 solver with bf16 multigrid vectors)
 KERNEL_NAMES = {
-    "update": "k_update_xr4<float>", "smooth": "k_cheb<float,float,bf16,float,8,1,true>",
-    "apply": "k_apply<float,8,true>", "pair": "k_smooth_pair<float,float,bf16,8>",
+    "update": "k_update_r4", "smooth": "k_cheb<float,float,bf16,bf16,8,1,true>",
+    "apply": "k_apply_pq<bf16,8,false>", "pair": "k_smooth_pair<float,float,bf16,8>",
     "residual": "k_residual<float,float,bf16,8>", "smooth0": "k_cheb<float,float,bf16,bf16,8,0,false>",
     "prolong": "k_prolong_add<float,bf16,8>", "restrict": "k_restrict_axis<float,bf16,*,8>",
     "update_p": "k_update_p4<float,false>", "refine": "k_ir_residual<float,4,false>",

@@ -83,8 +83,9 @@ typedef struct {
                                   once it dropped by this factor (0 < delta < 1) */
   int32_t xfer_rhs_per_thread; /* grid transfers: 0 = as rhs_per_thread, else 1, 2, 4 or 8 */
   int32_t xcd_remap;           /* 1: give each XCD a contiguous range of work-groups, 0: off */
-  int32_t vcycle_bf16;         /* 1: the fp32 solver keeps its multigrid vectors in bfloat16
-                                  (arithmetic stays fp32); 0: fp32 vectors */
+  int32_t vcycle_bf16;         /* 1: the fp32 solver keeps its multigrid vectors, including the
+                                  preconditioned residual z, in bfloat16 (arithmetic stays fp32);
+                                  2: the same with z in fp32; 0: fp32 vectors */
   int32_t xfer_separable;      /* restriction: 1 = one pass per coarsened axis, 0 = one 27-point
                                   gather pass */
   int32_t wcycle_level;        /* 0: V-cycle; l >= 1: the coarse correction of level l runs twice

@@ -129,8 +129,10 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const B* __restri
         T xn = xc[r] + w * (bv[r] - v[r]);
         if (PREV == 1) xn += c1 * (xc[r] - pv[r]);
         if (PREV == 2) xn += c1 * (xc[r] - cz * g * bv[r]);
-        st_once(xout + o, (O)xn);
-        if (DOT) acc[r] += (double)bv[r] * (double)xn;
+        const O xs = (O)xn;
+        st_once(xout + o, xs);
+        // the product uses the value stored (z may be rounded to bf16), as the next step does
+        if (DOT) acc[r] += (double)bv[r] * (double)(T)xs;
       }
     }
   }
@@ -486,6 +488,114 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* __restrict__ x, con
   }
   const double t = block_reduce1(acc);
   if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
+}
+
+// The mixed-precision PCG step that replaces "p = z + beta p; q = A p": by linearity
+// q = A z + beta q, so the stencil reads z (the preconditioner output, possibly bf16) instead of
+// p, and p, q are updated pointwise.  The x update of the previous step, x += alpha_prev p_old,
+// rides along (it needs p_old, which is read here anyway); partial sums of p.q per column.
+// FIRST: p = z, q = A z, no x update.
+template <typename Z, int R, bool FIRST>
+__global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* __restrict__ z,
+                                                        float* __restrict__ p,
+                                                        float* __restrict__ q,
+                                                        float* __restrict__ x,
+                                                        const double* __restrict__ beta,
+                                                        const double* __restrict__ alpha_prev,
+                                                        size_t ld, int ncol,
+                                                        double* __restrict__ partial) {
+  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  if (live) {
+    const NodeC<float> c = node_coef(s, n);
+    float zc[R], az[R], po[R], qo[R], xo[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!FIRST) {
+        const size_t o = (size_t)(c0 + r < ncol ? c0 + r : ncol - 1) * ld + n;
+        po[r] = ld_once(p + o);
+        qo[r] = ld_once(q + o);
+        xo[r] = ld_once(x + o);
+      }
+    }
+    stencil_cols<float, R>(c, z, n, ld, c0, ncol, zc, az);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (c0 + r < ncol) {
+        const size_t o = (size_t)(c0 + r) * ld + n;
+        float pn = zc[r], qn = az[r];
+        if (!FIRST) {
+          const float be = (float)beta[c0 + r];
+          pn += be * po[r];
+          qn += be * qo[r];
+          st_once(x + o, xo[r] + (float)alpha_prev[c0 + r] * po[r]);
+        }
+        st_once(p + o, pn);
+        st_once(q + o, qn);
+        acc[r] += (double)pn * (double)qn;
+      }
+    }
+  }
+  block_partials<R>(acc, partial, c0, ncol, nbx, bx);
+}
+
+// r -= alpha q and per-column |r|^2 partials (4 nodes per work-item)
+__global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
+                                                         const float* __restrict__ q,
+                                                         const double* __restrict__ alpha,
+                                                         size_t ld, uint32_t nN,
+                                                         double* __restrict__ partial) {
+  const int c = blockIdx.y;
+  const size_t o = (size_t)c * ld;
+  const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
+  const float al = (float)alpha[c];
+  double acc = 0.0;
+  if (n0 + 3 < nN) {
+    vec4_t<float> rv = *(const vec4_t<float>*)(r + o + n0);
+    const vec4_t<float> qv = *(const vec4_t<float>*)(q + o + n0);
+    rv -= al * qv;
+    *(vec4_t<float>*)(r + o + n0) = rv;
+    acc = (double)rv.x * rv.x + (double)rv.y * rv.y + (double)rv.z * rv.z + (double)rv.w * rv.w;
+  } else {
+    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
+      const float rn = r[o + n] - al * q[o + n];
+      r[o + n] = rn;
+      acc += (double)rn * rn;
+    }
+  }
+  const double t = block_reduce1(acc);
+  if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
+}
+
+// u += x + alpha p, x = 0 (fold the fp32 partial solution, including the pending x update of
+// the last step, into the fp64 solution)
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_p4(double* __restrict__ u,
+                                                          float* __restrict__ x,
+                                                          const float* __restrict__ p,
+                                                          const double* __restrict__ alpha,
+                                                          size_t ld, uint32_t nN) {
+  const int c = blockIdx.y;
+  const size_t o = (size_t)c * ld;
+  const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
+  const float al = (float)alpha[c];
+  if (n0 + 3 < nN) {
+    const vec4_t<float> pv = *(const vec4_t<float>*)(p + o + n0);
+    const vec4_t<float> xv = *(const vec4_t<float>*)(x + o + n0) + al * pv;
+    vec4_t<double> uv = *(const vec4_t<double>*)(u + o + n0);
+    uv.x += (double)xv.x;
+    uv.y += (double)xv.y;
+    uv.z += (double)xv.z;
+    uv.w += (double)xv.w;
+    *(vec4_t<double>*)(u + o + n0) = uv;
+    *(vec4_t<float>*)(x + o + n0) = vec4_t<float>(0.0f);
+  } else {
+    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
+      u[o + n] += (double)(x[o + n] + al * p[o + n]);
+      x[o + n] = 0.0f;
+    }
+  }
 }
 
 // p = z + beta p (FIRST: p = z)

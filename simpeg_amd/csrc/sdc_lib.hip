@@ -885,27 +885,50 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   if (nact == 0) return SDC_OK;
   // the finalisers test the convergence flag counters[3]; a previous solve may have left it set
   hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)4);
-  CK(vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, st));
-  hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
-                     W.beta, (const int*)W.active, 1, (const int*)(W.counters + 3));
-  hipLaunchKernelGGL((k_update_p4<float, true>), grid4(l0.nN, ncol), blk, 0, st, (const float*)W.z,
-                     W.p, (const double*)W.beta, l0.ld, l0.nN, (const int*)nullptr);
+  // Iteration: z = M r (z kept in the V-cycle storage type), beta, p = z + beta p and
+  // q = A z + beta q in one pass (plus the lagged x += alpha p), alpha, r -= alpha q.  A reliable
+  // update folds x + alpha p into u, so the lagged update is cleared with alpha.
+  const bool zb = h->prm.vcycle_bf16 == 1 && h->L.size() > 1;
+  __bf16* const z16 = (__bf16*)W.xb0;
+  const int* done = W.counters + 3;
   const int check = std::max(1, (int)h->prm.check_every);
   const int maxit = o->max_iter;
+  const double bz = zb ? 2.0 : 4.0;
   int it;
   for (it = 1; it <= maxit; ++it) {
+    const bool first = it == 1;
+    if (zb) {
+      CK((vcycle<float, __bf16, float, __bf16>(h, 0, W.r, z16, ncol, true, nullptr, st)));
+    } else {
+      CK(vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, st));
+    }
+    hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
+                       W.beta, (const int*)W.active, first ? 1 : 0, done);
     prof_begin(h, SDC_PROF_APPLY, st);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_apply<float, R, true>), g, blk, 0, st, s1, (const float*)W.p,
-                                     W.q, l0.ld, ncol, W.partial, (const int*)nullptr));
-    prof_end(h, SDC_PROF_APPLY, st, (double)l0.nN * 16 + (double)l0.nN * ncol * 8);
+#define SDC_APPLY_PQ(ZT, ZP)                                                                   \
+    if (first) {                                                                               \
+      DISPATCH_R(R, hipLaunchKernelGGL((k_apply_pq<ZT, R, true>), g, blk, 0, st, s1, ZP, W.p,  \
+                                       W.q, W.x, (const double*)W.beta,                        \
+                                       (const double*)W.alpha, l0.ld, ncol, W.partial));       \
+    } else {                                                                                   \
+      DISPATCH_R(R, hipLaunchKernelGGL((k_apply_pq<ZT, R, false>), g, blk, 0, st, s1, ZP, W.p, \
+                                       W.q, W.x, (const double*)W.beta,                        \
+                                       (const double*)W.alpha, l0.ld, ncol, W.partial));       \
+    }
+    if (zb) {
+      SDC_APPLY_PQ(__bf16, (const __bf16*)z16)
+    } else {
+      SDC_APPLY_PQ(float, (const float*)W.z)
+    }
+#undef SDC_APPLY_PQ
+    prof_end(h, SDC_PROF_APPLY, st,
+             (double)l0.nN * 16 + (double)l0.nN * ncol * (bz + (first ? 8.0 : 24.0)));
     hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
-                       (const double*)W.rz, W.alpha, (const int*)W.active,
-                       (const int*)(W.counters + 3));
+                       (const double*)W.rz, W.alpha, (const int*)W.active, done);
     prof_begin(h, SDC_PROF_UPDATE, st);
-    hipLaunchKernelGGL(k_update_xr4<float>, grid4(l0.nN, ncol), blk, 0, st, W.x, (const float*)W.p,
-                       W.r, (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
-                       (const int*)nullptr);
-    prof_end(h, SDC_PROF_UPDATE, st, (double)l0.nN * ncol * 24);
+    hipLaunchKernelGGL(k_update_r4, grid4(l0.nN, ncol), blk, 0, st, W.r, (const float*)W.q,
+                       (const double*)W.alpha, l0.ld, l0.nN, W.partial);
+    prof_end(h, SDC_PROF_UPDATE, st, (double)l0.nN * ncol * 12);
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nblk4(l0.nN),
                        W.rr, W.active, 0);
     if (it % check == 0 || it == maxit) {
@@ -919,8 +942,10 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       }
       if (cand || drop || it == maxit) {
         prof_begin(h, SDC_PROF_FOLD, st);
-        hipLaunchKernelGGL(k_ir_fold4, grid4(l0.nN, ncol), blk, 0, st, u, W.x, l0.ld, l0.nN);
-        prof_end(h, SDC_PROF_FOLD, st, (double)l0.nN * ncol * 24);
+        hipLaunchKernelGGL(k_ir_fold_p4, grid4(l0.nN, ncol), blk, 0, st, u, W.x,
+                           (const float*)W.p, (const double*)W.alpha, l0.ld, l0.nN);
+        prof_end(h, SDC_PROF_FOLD, st, (double)l0.nN * ncol * 28);
+        hipLaunchKernelGGL(k_zero<double>, dim3(nblk(ncol)), blk, 0, st, W.alpha, (size_t)ncol);
         prof_begin(h, SDC_PROF_REFINE, st);
         // fp64 stencil: 4 columns per work-item keep the neighbouring planes inside L2
         const int R64 = std::min(R, 4);
@@ -947,14 +972,6 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
         if (nact == 0 || it == maxit) break;
       }
     }
-    CK(vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, st));
-    hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
-                       W.beta, (const int*)W.active, 0, (const int*)(W.counters + 3));
-    prof_begin(h, SDC_PROF_UPDATE_P, st);
-    hipLaunchKernelGGL((k_update_p4<float, false>), grid4(l0.nN, ncol), blk, 0, st,
-                       (const float*)W.z, W.p, (const double*)W.beta, l0.ld, l0.nN,
-                       (const int*)nullptr);
-    prof_end(h, SDC_PROF_UPDATE_P, st, (double)l0.nN * ncol * 12);
   }
   S.iters = std::min(it, maxit);
   S.converged = nact == 0;
@@ -982,7 +999,7 @@ int sdc_default_params(sdc_params* p) {
   p->rhs_per_thread = 0;
   p->xfer_rhs_per_thread = 0;
   p->xcd_remap = 1;
-  p->vcycle_bf16 = 1;
+  p->vcycle_bf16 = 2;
   p->xfer_separable = 1;
   p->wcycle_level = -1;
   p->fused_tail = 1;
@@ -1122,8 +1139,8 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
       p->xfer_rhs_per_thread != 2 && p->xfer_rhs_per_thread != 4 && p->xfer_rhs_per_thread != 8)
     return fail(SDC_ERR_ARG, "xfer_rhs_per_thread must be 0, 1, 2, 4 or 8");
   if (p->xcd_remap != 0 && p->xcd_remap != 1) return fail(SDC_ERR_ARG, "xcd_remap must be 0 or 1");
-  if (p->vcycle_bf16 != 0 && p->vcycle_bf16 != 1)
-    return fail(SDC_ERR_ARG, "vcycle_bf16 must be 0 or 1");
+  if (p->vcycle_bf16 < 0 || p->vcycle_bf16 > 2)
+    return fail(SDC_ERR_ARG, "vcycle_bf16 must be 0, 1 or 2");
   if (p->xfer_separable != 0 && p->xfer_separable != 1)
     return fail(SDC_ERR_ARG, "xfer_separable must be 0 or 1");
   if (p->wcycle_level < -1) return fail(SDC_ERR_ARG, "wcycle_level must be >= -1");

@@ -84,13 +84,28 @@ def test_fp32_vcycle_bf16_storage(gpu, tiny, bc):
         eng.set_params(vcycle_bf16=flag)
         out[flag] = eng.from_block(eng.vcycle(eng.to_block(a).float()).double())
     mb = eng.from_block(eng.vcycle(eng.to_block(b).float()).double())
-    eng.set_params(vcycle_bf16=1)
+    eng.set_params(vcycle_bf16=2)
     assert rel(out[0], ref) < 1e-5
     assert 1e-6 < rel(out[1], ref) < 3e-2  # bf16 storage is in effect, and close
     lhs = np.sum(out[1] * b, axis=0)
     rhs = np.sum(a * mb, axis=0)
     assert np.allclose(lhs, rhs, rtol=3e-2)
     assert np.all(np.sum(out[1] * a, axis=0) > 0)
+
+
+@pytest.mark.parametrize("storage", [0, 1, 2])
+def test_mixed_solve_with_each_vcycle_storage(gpu, tiny, storage):
+    """Mixed-precision solves with fp32 multigrid vectors (0), bf16 vectors and a bf16
+    preconditioned residual (1), and bf16 vectors with an fp32 one (2) all reach the fp64
+    tolerance and the direct solver's potentials."""
+    mesh, survey, m = tiny
+    sim = _sim(mesh, survey, gpu_params={"vcycle_bf16": storage})
+    orc = oracle_for(mesh, survey)
+    u_ref = orc.fields(m)
+    f = sim.fields(m)
+    assert sim.last_solve.converged and sim.last_solve.max_rel_res <= 1e-10
+    for s in range(survey.nSrc):
+        assert rel(f[:, "phiSolution"][:, s], u_ref[:, s]) < 1e-7
 
 
 @pytest.mark.parametrize("bc", ["Robin", "Neumann"])
