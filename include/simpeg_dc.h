@@ -85,7 +85,8 @@ typedef struct {
   int32_t xcd_remap;           /* 1: give each XCD a contiguous range of work-groups, 0: off */
   int32_t vcycle_bf16;         /* 1: the fp32 solver keeps its multigrid vectors, including the
                                   preconditioned residual z, in bfloat16 (arithmetic stays fp32);
-                                  2: the same with z in fp32; 0: fp32 vectors */
+                                  2: the same with z in fp32; 3: as 2, and the finest level
+                                  reads a bfloat16 copy of the residual; 0: fp32 vectors */
   int32_t xfer_separable;      /* restriction: 1 = one pass per coarsened axis, 0 = one 27-point
                                   gather pass */
   int32_t wcycle_level;        /* 0: V-cycle; l >= 1: the coarse correction of level l runs twice
@@ -93,6 +94,8 @@ typedef struct {
                                   of the finest level's nodes */
   int32_t fused_tail;          /* 1: the fp32 solver runs the small coarse levels that fit into
                                   LDS as one launch (one work-group per column); 0: off */
+  int32_t refine_rhs_per_thread; /* fp64 true-residual kernel: 0 = 8, else 1, 2, 4 or 8 columns
+                                    per work-item */
 } sdc_params;
 
 typedef struct {

@@ -60,6 +60,7 @@ class Params(C.Structure):
         ("xfer_separable", C.c_int32),
         ("wcycle_level", C.c_int32),
         ("fused_tail", C.c_int32),
+        ("refine_rhs_per_thread", C.c_int32),
     ]
 
 

@@ -542,11 +542,14 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* 
 }
 
 // r -= alpha q and per-column |r|^2 partials (4 nodes per work-item)
+// (R16: also write a bfloat16 copy of r, the multigrid input)
+template <bool R16>
 __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
                                                          const float* __restrict__ q,
                                                          const double* __restrict__ alpha,
                                                          size_t ld, uint32_t nN,
-                                                         double* __restrict__ partial) {
+                                                         double* __restrict__ partial,
+                                                         __bf16* __restrict__ r16) {
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
   const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
@@ -557,11 +560,21 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
     const vec4_t<float> qv = *(const vec4_t<float>*)(q + o + n0);
     rv -= al * qv;
     *(vec4_t<float>*)(r + o + n0) = rv;
+    if (R16) {
+      typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+      bf4 h;
+      h.x = (__bf16)rv.x;
+      h.y = (__bf16)rv.y;
+      h.z = (__bf16)rv.z;
+      h.w = (__bf16)rv.w;
+      *(bf4*)(r16 + o + n0) = h;
+    }
     acc = (double)rv.x * rv.x + (double)rv.y * rv.y + (double)rv.z * rv.z + (double)rv.w * rv.w;
   } else {
     for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
       const float rn = r[o + n] - al * q[o + n];
       r[o + n] = rn;
+      if (R16) r16[o + n] = (__bf16)rn;
       acc += (double)rn * rn;
     }
   }
@@ -791,7 +804,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
                                                            TO* __restrict__ rout, size_t ld,
                                                            int ncol, PinFix pf,
                                                            double* __restrict__ partial,
-                                                           unsigned* __restrict__ qmask) {
+                                                           unsigned* __restrict__ qmask,
+                                                           __bf16* __restrict__ r16) {
   SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   double acc[R];
 #pragma unroll
@@ -820,6 +834,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
           rv += wfix * r0;
         }
         rout[o + n] = (TO)rv;
+        if (r16) r16[o + n] = (__bf16)rv;
       }
     // every writer stores the same value; the mask is cleared before this pass
     if (U_ZERO && qmask != nullptr && anyq) qmask[n] = 1u;

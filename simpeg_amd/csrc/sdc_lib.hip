@@ -136,8 +136,10 @@ struct Work {
   int* ocount = nullptr;
   // level 0 nodes where some right-hand-side column is non-zero (fp64 residual skips q elsewhere)
   unsigned* qmask = nullptr;
+  // bfloat16 copy of r, the finest-level multigrid input (vcycle_bf16 = 3)
+  __bf16* r16 = nullptr;
   void release() {
-    dfree(x); dfree(r); dfree(z); dfree(p); dfree(q);
+    dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(r16);
     for (auto* v : b) dfree(v);
     for (auto* v : xs) dfree(v);
     for (auto* v : xa) dfree(v);
@@ -385,7 +387,8 @@ template <typename T>
 int ensure_work(sdc_handle_s* h, int ncol) {
   Work<T>& w = work_of<T>(h);
   const int vb = vcycle_bytes<T>(h);
-  if (w.ncap >= ncol && w.vbytes == vb) return SDC_OK;
+  const bool want16 = std::is_same<T, float>::value && h->prm.vcycle_bf16 == 3;
+  if (w.ncap >= ncol && w.vbytes == vb && (w.r16 != nullptr) == want16) return SDC_OK;
   ncol = std::max(ncol, w.ncap);
   w.release();
   const int cap = std::max(ncol, 1);
@@ -424,6 +427,7 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   CK(dmalloc(&w.omask, cap));
   CK(dmalloc(&w.ocount, 1));
   CK(dmalloc(&w.qmask, ld0));
+  if (std::is_same<T, float>::value && h->prm.vcycle_bf16 == 3) CK(dmalloc(&w.r16, n0));
   w.ncap = cap;
   return SDC_OK;
 }
@@ -861,8 +865,10 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
   hipLaunchKernelGGL(k_zero<float>, dim3(nblk(len)), blk, 0, st, W.x, len);
   hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
+  // vcycle_bf16 = 3: the multigrid reads a bfloat16 copy of r, written with r
+  __bf16* const r16 = (h->prm.vcycle_bf16 == 3 && h->L.size() > 1) ? W.r16 : nullptr;
   DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
-                                   (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask));
+                                   (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask, r16));
   // aux = |q|^2 and active = |q| > 0; orr = |q|^2
   hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
                      W.active, 1);
@@ -899,6 +905,8 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     const bool first = it == 1;
     if (zb) {
       CK((vcycle<float, __bf16, float, __bf16>(h, 0, W.r, z16, ncol, true, nullptr, st)));
+    } else if (r16) {
+      CK((vcycle<float, __bf16, __bf16, float>(h, 0, r16, W.z, ncol, true, nullptr, st)));
     } else {
       CK(vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, st));
     }
@@ -926,9 +934,14 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                        (const double*)W.rz, W.alpha, (const int*)W.active, done);
     prof_begin(h, SDC_PROF_UPDATE, st);
-    hipLaunchKernelGGL(k_update_r4, grid4(l0.nN, ncol), blk, 0, st, W.r, (const float*)W.q,
-                       (const double*)W.alpha, l0.ld, l0.nN, W.partial);
-    prof_end(h, SDC_PROF_UPDATE, st, (double)l0.nN * ncol * 12);
+    if (r16) {
+      hipLaunchKernelGGL(k_update_r4<true>, grid4(l0.nN, ncol), blk, 0, st, W.r,
+                         (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16);
+    } else {
+      hipLaunchKernelGGL(k_update_r4<false>, grid4(l0.nN, ncol), blk, 0, st, W.r,
+                         (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16);
+    }
+    prof_end(h, SDC_PROF_UPDATE, st, (double)l0.nN * ncol * (r16 ? 14 : 12));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nblk4(l0.nN),
                        W.rr, W.active, 0);
     if (it % check == 0 || it == maxit) {
@@ -947,11 +960,12 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
         prof_end(h, SDC_PROF_FOLD, st, (double)l0.nN * ncol * 28);
         hipLaunchKernelGGL(k_zero<double>, dim3(nblk(ncol)), blk, 0, st, W.alpha, (size_t)ncol);
         prof_begin(h, SDC_PROF_REFINE, st);
-        // fp64 stencil: 4 columns per work-item keep the neighbouring planes inside L2
-        const int R64 = std::min(R, 4);
+        // fp64 stencil: refine_rhs_per_thread columns per work-item (default 8)
+        const int R64 =
+            std::min(R, h->prm.refine_rhs_per_thread > 0 ? h->prm.refine_rhs_per_thread : 8);
         DISPATCH_R(R64, hipLaunchKernelGGL((k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64),
                                            blk, 0, st, s2, q, (const double*)u, W.r, l0.ld, ncol,
-                                           pf, W.partial, (unsigned*)W.qmask));
+                                           pf, W.partial, (unsigned*)W.qmask, r16));
         // algorithmic bytes: fp64 coefficients, per column u64 read and r32 write; q is read
         // on source nodes only (not counted)
         prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
@@ -1003,6 +1017,7 @@ int sdc_default_params(sdc_params* p) {
   p->xfer_separable = 1;
   p->wcycle_level = -1;
   p->fused_tail = 1;
+  p->refine_rhs_per_thread = 0;
   p->check_every = 1;
   p->reliable_delta = 1e-3;
   return SDC_OK;
@@ -1139,12 +1154,16 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
       p->xfer_rhs_per_thread != 2 && p->xfer_rhs_per_thread != 4 && p->xfer_rhs_per_thread != 8)
     return fail(SDC_ERR_ARG, "xfer_rhs_per_thread must be 0, 1, 2, 4 or 8");
   if (p->xcd_remap != 0 && p->xcd_remap != 1) return fail(SDC_ERR_ARG, "xcd_remap must be 0 or 1");
-  if (p->vcycle_bf16 < 0 || p->vcycle_bf16 > 2)
-    return fail(SDC_ERR_ARG, "vcycle_bf16 must be 0, 1 or 2");
+  if (p->vcycle_bf16 < 0 || p->vcycle_bf16 > 3)
+    return fail(SDC_ERR_ARG, "vcycle_bf16 must be 0, 1, 2 or 3");
   if (p->xfer_separable != 0 && p->xfer_separable != 1)
     return fail(SDC_ERR_ARG, "xfer_separable must be 0 or 1");
   if (p->wcycle_level < -1) return fail(SDC_ERR_ARG, "wcycle_level must be >= -1");
   if (p->fused_tail != 0 && p->fused_tail != 1) return fail(SDC_ERR_ARG, "fused_tail must be 0 or 1");
+  if (p->refine_rhs_per_thread != 0 && p->refine_rhs_per_thread != 1 &&
+      p->refine_rhs_per_thread != 2 && p->refine_rhs_per_thread != 4 &&
+      p->refine_rhs_per_thread != 8)
+    return fail(SDC_ERR_ARG, "refine_rhs_per_thread must be 0, 1, 2, 4 or 8");
   if (!(p->reliable_delta > 0) || !(p->reliable_delta < 1))
     return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
   h->prm = *p;
@@ -1290,7 +1309,8 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     Work<double>& W = h->w64;
     hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, true>), g, blk, 0, st, s2, q,
-                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask));
+                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask,
+                                     (__bf16*)nullptr));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
                        W.active, 0);
     int it = 0;
@@ -1299,7 +1319,7 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     S.outer = 1;
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
                                      (const double*)u, W.z, l0.ld, ncol, pf, W.partial,
-                                     W.qmask));
+                                     W.qmask, (__bf16*)nullptr));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
                        W.active, 0);
     CK(rel_max(W.orr, W.aux));
@@ -1313,7 +1333,8 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
     hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
-                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask));
+                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask,
+                                     (__bf16*)nullptr));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
                        W.active, 0);
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
@@ -1324,7 +1345,7 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
         prof_begin(h, SDC_PROF_REFINE, st);
         DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, false>), g, blk, 0, st, s2, q,
                                          (const double*)u, W.r, l0.ld, ncol, pf, W.partial,
-                                         W.qmask));
+                                         W.qmask, (__bf16*)nullptr));
         prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 20);
         hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                            W.orr, W.active, 0);
