@@ -494,7 +494,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* __restrict__ x, con
 // q = A z + beta q, so the stencil reads z (the preconditioner output, possibly bf16) instead of
 // p, and p, q are updated pointwise.  The x update of the previous step, x += alpha_prev p_old,
 // rides along (it needs p_old, which is read here anyway); partial sums of p.q per column.
-// FIRST: p = z, q = A z, no x update.
+// FIRST: p = z, q = A z, no x update.  xfresh: x was folded away and counts as zero (not read).
 template <typename Z, int R, bool FIRST>
 __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* __restrict__ z,
                                                         float* __restrict__ p,
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* 
                                                         const double* __restrict__ beta,
                                                         const double* __restrict__ alpha_prev,
                                                         size_t ld, int ncol,
-                                                        double* __restrict__ partial) {
+                                                        double* __restrict__ partial, int xfresh) {
   SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   double acc[R];
 #pragma unroll
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* 
         const size_t o = (size_t)(c0 + r < ncol ? c0 + r : ncol - 1) * ld + n;
         po[r] = ld_once(p + o);
         qo[r] = ld_once(q + o);
-        xo[r] = ld_once(x + o);
+        xo[r] = xfresh ? 0.0f : ld_once(x + o);
       }
     }
     stencil_cols<float, R>(c, z, n, ld, c0, ncol, zc, az);
@@ -582,32 +582,31 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
   if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
 }
 
-// u += x + alpha p, x = 0 (fold the fp32 partial solution, including the pending x update of
-// the last step, into the fp64 solution)
+// u += x + alpha p (fold the fp32 partial solution, including the pending x update of the last
+// step, into the fp64 solution).  x is not cleared: the next k_apply_pq treats it as zero.
+// xfresh: x counts as zero (not read).
 __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_p4(double* __restrict__ u,
-                                                          float* __restrict__ x,
+                                                          const float* __restrict__ x,
                                                           const float* __restrict__ p,
                                                           const double* __restrict__ alpha,
-                                                          size_t ld, uint32_t nN) {
+                                                          size_t ld, uint32_t nN, int xfresh) {
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
   const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
   const float al = (float)alpha[c];
   if (n0 + 3 < nN) {
     const vec4_t<float> pv = *(const vec4_t<float>*)(p + o + n0);
-    const vec4_t<float> xv = *(const vec4_t<float>*)(x + o + n0) + al * pv;
+    const vec4_t<float> xv =
+        (xfresh ? vec4_t<float>(0.0f) : *(const vec4_t<float>*)(x + o + n0)) + al * pv;
     vec4_t<double> uv = *(const vec4_t<double>*)(u + o + n0);
     uv.x += (double)xv.x;
     uv.y += (double)xv.y;
     uv.z += (double)xv.z;
     uv.w += (double)xv.w;
     *(vec4_t<double>*)(u + o + n0) = uv;
-    *(vec4_t<float>*)(x + o + n0) = vec4_t<float>(0.0f);
   } else {
-    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
-      u[o + n] += (double)(x[o + n] + al * p[o + n]);
-      x[o + n] = 0.0f;
-    }
+    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n)
+      u[o + n] += (double)((xfresh ? 0.0f : x[o + n]) + al * p[o + n]);
   }
 }
 
@@ -820,20 +819,25 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
       else if (n == pf.plane) wfix = pf.cz[0];
     }
     const bool readq = U_ZERO || qmask == nullptr || qmask[n] != 0u;
+    // all loads of the R columns first (clamped columns, results discarded), then arithmetic
+    double qv[R], uc[R], au[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) qv[r] = readq ? ld_once(col_ptr(q, ld, c0, r, ncol) + n) : 0.0;
+    if (!U_ZERO) stencil_cols<double, R>(c, u, n, ld, c0, ncol, uc, au);
     bool anyq = false;
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (c0 + r < ncol) {
         const size_t o = (size_t)(c0 + r) * ld;
-        double rv = readq ? q[o + n] : 0.0;
+        double rv = qv[r];
         anyq |= rv != 0.0;
-        if (!U_ZERO) rv -= stencil(c, u + o, n);
+        if (!U_ZERO) rv -= au[r];
         acc[r] = rv * rv;
         if (wfix != 0.0) {
           const double r0 = U_ZERO ? q[o] : q[o] - u[o];
           rv += wfix * r0;
         }
-        rout[o + n] = (TO)rv;
+        st_once(rout + o + n, (TO)rv);
         if (r16) r16[o + n] = (__bf16)rv;
       }
     // every writer stores the same value; the mask is cleared before this pass

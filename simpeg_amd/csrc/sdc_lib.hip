@@ -863,7 +863,6 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const double tol2 = o->tol * o->tol;
   const double delta2 = h->prm.reliable_delta * h->prm.reliable_delta;
   hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
-  hipLaunchKernelGGL(k_zero<float>, dim3(nblk(len)), blk, 0, st, W.x, len);
   hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
   // vcycle_bf16 = 3: the multigrid reads a bfloat16 copy of r, written with r
   __bf16* const r16 = (h->prm.vcycle_bf16 == 3 && h->L.size() > 1) ? W.r16 : nullptr;
@@ -900,6 +899,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const int check = std::max(1, (int)h->prm.check_every);
   const int maxit = o->max_iter;
   const double bz = zb ? 2.0 : 4.0;
+  bool xfresh = true;  // x holds nothing that is not yet in u (its memory is not read)
   int it;
   for (it = 1; it <= maxit; ++it) {
     const bool first = it == 1;
@@ -917,11 +917,12 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     if (first) {                                                                               \
       DISPATCH_R(R, hipLaunchKernelGGL((k_apply_pq<ZT, R, true>), g, blk, 0, st, s1, ZP, W.p,  \
                                        W.q, W.x, (const double*)W.beta,                        \
-                                       (const double*)W.alpha, l0.ld, ncol, W.partial));       \
+                                       (const double*)W.alpha, l0.ld, ncol, W.partial, 1));    \
     } else {                                                                                   \
       DISPATCH_R(R, hipLaunchKernelGGL((k_apply_pq<ZT, R, false>), g, blk, 0, st, s1, ZP, W.p, \
                                        W.q, W.x, (const double*)W.beta,                        \
-                                       (const double*)W.alpha, l0.ld, ncol, W.partial));       \
+                                       (const double*)W.alpha, l0.ld, ncol, W.partial,         \
+                                       xfresh ? 1 : 0));                                       \
     }
     if (zb) {
       SDC_APPLY_PQ(__bf16, (const __bf16*)z16)
@@ -930,7 +931,9 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     }
 #undef SDC_APPLY_PQ
     prof_end(h, SDC_PROF_APPLY, st,
-             (double)l0.nN * 16 + (double)l0.nN * ncol * (bz + (first ? 8.0 : 24.0)));
+             (double)l0.nN * 16 +
+                 (double)l0.nN * ncol * (bz + (first ? 8.0 : (xfresh ? 20.0 : 24.0))));
+    if (!first) xfresh = false;
     hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
                        (const double*)W.rz, W.alpha, (const int*)W.active, done);
     prof_begin(h, SDC_PROF_UPDATE, st);
@@ -955,10 +958,12 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       }
       if (cand || drop || it == maxit) {
         prof_begin(h, SDC_PROF_FOLD, st);
-        hipLaunchKernelGGL(k_ir_fold_p4, grid4(l0.nN, ncol), blk, 0, st, u, W.x,
-                           (const float*)W.p, (const double*)W.alpha, l0.ld, l0.nN);
-        prof_end(h, SDC_PROF_FOLD, st, (double)l0.nN * ncol * 28);
+        hipLaunchKernelGGL(k_ir_fold_p4, grid4(l0.nN, ncol), blk, 0, st, u, (const float*)W.x,
+                           (const float*)W.p, (const double*)W.alpha, l0.ld, l0.nN,
+                           xfresh ? 1 : 0);
+        prof_end(h, SDC_PROF_FOLD, st, (double)l0.nN * ncol * 24);
         hipLaunchKernelGGL(k_zero<double>, dim3(nblk(ncol)), blk, 0, st, W.alpha, (size_t)ncol);
+        xfresh = true;
         prof_begin(h, SDC_PROF_REFINE, st);
         // fp64 stencil: refine_rhs_per_thread columns per work-item (default 8)
         const int R64 =
