@@ -135,12 +135,12 @@ FP64_BASELINE_CANDIDATES = [
 
 # library profiler class -> kernel (finest level, fp32 solver with bf16 multigrid vectors)
 KERNEL_NAMES = {
-    "update": "k_update_r4", "smooth": "k_cheb<float,float,bf16,bf16,8,1,true>",
-    "apply": "k_apply_pq<bf16,8,false>", "pair": "k_smooth_pair<float,float,bf16,8>",
+    "update": "k_update_r4<false>", "smooth": "k_cheb<float,float,bf16,float,8,1,true>",
+    "apply": "k_apply_pq<float,8,false>", "pair": "k_smooth_pair<float,float,bf16,8>",
     "residual": "k_residual<float,float,bf16,8>", "smooth0": "k_cheb<float,float,bf16,bf16,8,0,false>",
     "prolong": "k_prolong_add<float,bf16,8>", "restrict": "k_restrict_axis<float,bf16,*,8>",
-    "update_p": "k_update_p4<float,false>", "refine": "k_ir_residual<float,4,false>",
-    "fold": "k_ir_fold4", "jtvec": "k_jtvec<0>",
+    "update_p": "k_update_p4<float,false>", "refine": "k_ir_residual<float,8,false>",
+    "fold": "k_ir_fold_p4", "jtvec": "k_jtvec<0>",
 }
 
 
@@ -314,8 +314,9 @@ def main():
     ap.add_argument("--workload", default="large", choices=["large", "small", "batch", "gn"])
     ap.add_argument("--gn-beta", type=float, default=1e-2)
     ap.add_argument("--gn-recompute", action="store_true")
-    ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--profile-kernel", default="update",
+    ap.add_argument("--rtol", type=float, default=1e-8,
+                    help="relative fp64 residual per source (the workload's stated 1e-8)")
+    ap.add_argument("--profile-kernel", default="apply",
                     choices=["smooth", "apply", "residual", "refine", "jtvec", "restrict", "prolong",
                              "update", "pair", "smooth0", "update_p", "fold"],
                     help="kernel class timed with HIP events during the timed steps (roofline)")
@@ -427,12 +428,16 @@ def main():
         ref_u, ref_res = eng.solve(pl.q, nS, mode="fp64", tol=1e-13, max_iter=2000)
         ref_d = eng.gather(pl.proj["gather"], ref_u)
         ref_jtv, _ = pl.adjoint(ref_u, "fp64", 1e-13)
-        out["accuracy"] = {
+        acc = {
             "reference": "fp64 batched multigrid PCG on the GPU to relative residual 1e-13",
             "potential_max_rel": rel_cols(last["u"], ref_u, eng.nN),
             "data_rel": rel(last["d"], ref_d),
             "jtvec_rel": rel(last["jtv"], ref_jtv),
         }
+        # the bar: potentials within 1e-7, data and Jtvec within 1e-6 (relative l2)
+        acc["within_bar"] = bool(acc["potential_max_rel"] <= 1e-7 and acc["data_rel"] <= 1e-6
+                                 and acc["jtvec_rel"] <= 1e-6)
+        out["accuracy"] = acc
         log(f"[bench] accuracy {out['accuracy']}")
 
     if not args.no_fp64:
