@@ -226,7 +226,9 @@ def cpu_baseline(budget_s=12.0):
 
 def bench_batch(args):
     """48 independent problems (uneven sizes / source counts) behind MetaSimulation: fields +
-    Jtvec for all of them, one stream after the other vs concurrently on 4 streams."""
+    Jtvec for all of them, one stream after the other vs concurrently on 4 streams.  Multigrid
+    settings come from the autotuner (tuned during the one-stream warm-up, cached per mesh shape
+    and column bucket)."""
     import torch
 
     from simpeg_amd import maps, workloads
@@ -240,7 +242,7 @@ def bench_batch(args):
     res = {}
     for ns in (1, 4):
         sims = [Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
-                                  rtol=args.rtol) for wl in batch]
+                                  rtol=args.rtol, autotune=True) for wl in batch]
         maps_ = [maps.Projection(int(offs[-1]), slice(int(offs[k]), int(offs[k + 1])))
                  for k in range(len(batch))]
         meta = MetaSimulation(sims, maps_, n_streams=ns)
@@ -262,6 +264,7 @@ def bench_batch(args):
                       "node_columns": int(sum(wl.mesh.n_nodes * wl.survey.nSrc for wl in batch))},
            "fields_ms": {str(k): round(1e3 * v[0], 1) for k, v in res.items()},
            "jtvec_ms": {str(k): round(1e3 * v[1], 1) for k, v in res.items()},
+           "autotuned": True,
            "concurrency_speedup": round(sum(res[1]) / sum(res[4]), 2)}
     print(json.dumps(out), flush=True)
 
@@ -290,7 +293,7 @@ def bench_gn(args):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         m, h = gauss_newton(sim, dobs, wd, m0, beta=args.gn_beta, n_iter=3, cg_iter=10,
-                            recompute=args.gn_recompute, verbose=True)
+                            recompute=not args.gn_store_fields, verbose=True)
         torch.cuda.synchronize()
         runs[prec] = (m, h, time.perf_counter() - t0)
         log(f"[bench] GN {prec}: {runs[prec][2]:.1f} s, phi {h.phi}")
@@ -299,7 +302,7 @@ def bench_gn(args):
     out = {"metric": "Gauss-Newton (3 x 10 CG) wall time, 176x176x104 cells / 256 sources",
            "unit": "s", "value": round(tm, 2), "n_gpus": 1, "higher_is_better": False,
            "config": {"workload": wl.name, "nodes": wl.mesh.n_nodes, "sources": wl.survey.nSrc,
-                      "data": wl.survey.nD, "recompute_fields": args.gn_recompute},
+                      "data": wl.survey.nD, "recompute_fields": not args.gn_store_fields},
            "fp64_seconds": round(t64, 2),
            "phi_mixed": hm.phi, "phi_fp64": h64.phi,
            "model_update_rel_diff": float(np.linalg.norm(mm - m64) / np.linalg.norm(m64 - m0))}
@@ -313,7 +316,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large", choices=["large", "small", "batch", "gn"])
     ap.add_argument("--gn-beta", type=float, default=1e-2)
-    ap.add_argument("--gn-recompute", action="store_true")
+    ap.add_argument("--gn-store-fields", action="store_true",
+                    help="keep the source fields between Hessian products (default: recompute "
+                         "them, the checkpointing the Gauss-Newton workload asks for)")
     ap.add_argument("--rtol", type=float, default=1e-8,
                     help="relative fp64 residual per source (the workload's stated 1e-8)")
     ap.add_argument("--profile-kernel", default="apply",

@@ -4,8 +4,8 @@ For a given problem (engine with sigma set, right-hand-side block) it times comp
 solves for a small grid of candidates -- smoothing schedule (nu_pre, nu_post), Chebyshev
 interval ratio, W-cycle level, poll cadence, columns per work-item -- and keeps the fastest that
 converges.  Results are
-cached in a JSON file keyed by mesh shape, column count, boundary condition and precision, so a
-later run with the same shape reuses the choice without timing.
+cached in a JSON file keyed by mesh shape, column count (rounded up to a power of two), boundary
+condition and precision, so a later run with the same shape reuses the choice without timing.
 """
 
 from __future__ import annotations
@@ -31,7 +31,12 @@ CANDIDATES = [
 
 
 def _key(engine, ncol, mode):
-    return f"{engine.shape_cells}|ncol={ncol}|bc={engine.bc}|mode={mode}|levels={engine.levels}"
+    # column counts share an entry up to the next power of two (the kernels' columns per
+    # work-item and the relative cost of the coarse levels change at that granularity)
+    bucket = 1
+    while bucket < ncol:
+        bucket *= 2
+    return f"{engine.shape_cells}|ncol<={bucket}|bc={engine.bc}|mode={mode}|levels={engine.levels}"
 
 
 def load_cache(path=DEFAULT_CACHE):
