@@ -1025,7 +1025,7 @@ int sdc_default_params(sdc_params* p) {
   p->fused_tail = 1;
   p->refine_rhs_per_thread = 0;
   p->check_every = 1;
-  p->reliable_delta = 1e-3;
+  p->reliable_delta = 1e-4;
   return SDC_OK;
 }
 
