@@ -388,52 +388,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_dot(const T* __restrict__ a, cons
   block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
 
-// x += alpha p ; r -= alpha q ; partial |r|^2
-template <typename T, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_update_xr(T* __restrict__ x, const T* __restrict__ p,
-                                                         T* __restrict__ r_,
-                                                         const T* __restrict__ q,
-                                                         const double* __restrict__ alpha,
-                                                         size_t ld, uint32_t nN, int ncol,
-                                                         double* __restrict__ partial,
-                                                         const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE(nN)
-  double acc[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.0;
-  if (live) {
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (c0 + r < ncol) {
-        const T al = (T)alpha[c0 + r];
-        const size_t o = (size_t)(c0 + r) * ld + n;
-        x[o] += al * p[o];
-        const T rn = r_[o] - al * q[o];
-        r_[o] = rn;
-        acc[r] = (double)rn * (double)rn;
-      }
-  }
-  block_partials<R>(acc, partial, c0, ncol, nbx, bx);
-}
-
-// p = z + beta p  (FIRST: p = z)
-template <typename T, int R, bool FIRST>
-__global__ __launch_bounds__(SDC_BLOCK) void k_update_p(const T* __restrict__ z, T* __restrict__ p,
-                                                        const double* __restrict__ beta, size_t ld,
-                                                        uint32_t nN, int ncol,
-                                                        const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE(nN)
-  if (!live) return;
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (c0 + r < ncol) {
-      const size_t o = (size_t)(c0 + r) * ld + n;
-      p[o] = FIRST ? z[o] : z[o] + (T)beta[c0 + r] * p[o];
-    }
-}
-
 // ------------------------------------------------------------------------------------------
 // streaming PCG updates, 4 consecutive nodes per work-item (one 16-byte access per array for
 // fp32), one column per blockIdx.y; the last, ragged group of nodes takes a scalar path.
@@ -635,29 +589,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_p4(const T* __restrict__ z
   }
 }
 
-// u += x, x = 0
-__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold4(double* __restrict__ u, float* __restrict__ x,
-                                                        size_t ld, uint32_t nN) {
-  const int c = blockIdx.y;
-  const size_t o = (size_t)c * ld;
-  const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
-  if (n0 + 3 < nN) {
-    const vec4_t<float> xv = *(const vec4_t<float>*)(x + o + n0);
-    vec4_t<double> uv = *(const vec4_t<double>*)(u + o + n0);
-    uv.x += (double)xv.x;
-    uv.y += (double)xv.y;
-    uv.z += (double)xv.z;
-    uv.w += (double)xv.w;
-    *(vec4_t<double>*)(u + o + n0) = uv;
-    *(vec4_t<float>*)(x + o + n0) = vec4_t<float>(0.0f);
-  } else {
-    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
-      u[o + n] += (double)x[o + n];
-      x[o + n] = 0.0f;
-    }
-  }
-}
-
 template <typename T>
 __global__ __launch_bounds__(SDC_BLOCK) void k_zero(T* __restrict__ x, size_t len) {
   const size_t i = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
@@ -758,8 +689,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_beta(const double* __restrict
   }
 }
 
-__global__ void k_set_int(int* p, int v) { *p = v; }
-
 // active[c] &= mask[c]
 __global__ __launch_bounds__(SDC_BLOCK) void k_and_active(int* __restrict__ active,
                                                           const int* __restrict__ mask, int ncol) {
@@ -844,21 +773,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
     if (U_ZERO && qmask != nullptr && anyq) qmask[n] = 1u;
   }
   block_partials<R>(acc, partial, c0, ncol, nbx, bx);
-}
-
-// u += x, x = 0 (fold the fp32 partial solution into the fp64 one)
-template <int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold(double* __restrict__ u, float* __restrict__ x,
-                                                       size_t ld, uint32_t nN, int ncol) {
-  SDC_NODE_PROLOGUE(nN)
-  if (!live) return;
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (c0 + r < ncol) {
-      const size_t o = (size_t)(c0 + r) * ld + n;
-      u[o] += (double)x[o];
-      x[o] = 0.0f;
-    }
 }
 
 // true-residual check after a reliable update: rr = |q - A u|^2, rmax = rr,
