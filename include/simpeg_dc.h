@@ -138,6 +138,15 @@ int sdc_set_params(sdc_handle h, const sdc_params* p);
  * `self.Ainv = self.solver(A, **self.solver_opts)` (resistivity/simulation.py:108). */
 int sdc_set_sigma(sdc_handle h, const double* sigma_dev, void* stream);
 
+/* Set the finest level's operator directly (device fp64, nN each): cx/cy/cz = conductance of the
+ * +x/+y/+z edge of every node (0 where there is no such edge), bd = the rest of the diagonal
+ * (Robin term).  For the solver plugin point, which receives an assembled operator instead of
+ * sigma: `self.solver(A, **self.solver_opts)` (resistivity/simulation.py:108; solver contract
+ * base/pde_simulation.py:493-511).  Coarse levels and the coarse inverse are rebuilt as in
+ * sdc_set_sigma; Neumann handles keep node 0's identity row. */
+int sdc_set_coefficients(sdc_handle h, const double* cx_dev, const double* cy_dev,
+                         const double* cz_dev, const double* bd_dev, void* stream);
+
 /* y = A x for ncol columns (dtype SDC_F32 or SDC_F64), with A exactly the reference operator
  * getA() (Neumann: identity row 0).  resistivity/simulation.py:523-555. */
 int sdc_apply(sdc_handle h, int dtype, const void* x_dev, void* y_dev, int ncol, void* stream);

@@ -1001,6 +1001,10 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
 
 }  // namespace
 
+namespace {
+int build_levels(sdc_handle_s* h, hipStream_t st);
+}  // namespace
+
 // ============================================================================================
 // C ABI
 
@@ -1187,6 +1191,30 @@ int sdc_set_sigma(sdc_handle h, const double* sigma, void* stream) {
   Level& l0 = h->L[0];
   hipLaunchKernelGGL(k_coef_from_sigma, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, g, sigma,
                      l0.cx64, l0.cy64, l0.cz64, l0.bd64, 1.0);
+  return build_levels(h, st);
+}
+
+int sdc_set_coefficients(sdc_handle h, const double* cx, const double* cy, const double* cz,
+                         const double* bd, void* stream) {
+  if (!h || !cx || !cy || !cz || !bd) return fail(SDC_ERR_ARG, "null argument");
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  Level& l0 = h->L[0];
+  const size_t bytes = (size_t)l0.nN * sizeof(double);
+  HIPCK(hipMemcpyAsync(l0.cx64, cx, bytes, hipMemcpyDeviceToDevice, st));
+  HIPCK(hipMemcpyAsync(l0.cy64, cy, bytes, hipMemcpyDeviceToDevice, st));
+  HIPCK(hipMemcpyAsync(l0.cz64, cz, bytes, hipMemcpyDeviceToDevice, st));
+  HIPCK(hipMemcpyAsync(l0.bd64, bd, bytes, hipMemcpyDeviceToDevice, st));
+  return build_levels(h, st);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Everything that follows the finest level's edge conductances and Robin diagonal: coarse
+// conductances, diagonals, fp32 copies and the coarsest dense inverse.
+int build_levels(sdc_handle_s* h, hipStream_t st) {
   for (size_t l = 0; l + 1 < h->L.size(); ++l) {
     Level& f = h->L[l];
     Level& c = h->L[l + 1];
@@ -1222,9 +1250,13 @@ int sdc_set_sigma(sdc_handle h, const double* sigma, void* stream) {
   return SDC_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 int sdc_apply(sdc_handle h, int dtype, const void* x, void* y, int ncol, void* stream) {
   if (!h || !x || !y || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
-  if (!h->sigma_set) return fail(SDC_ERR_STATE, "sdc_set_sigma has not been called");
+  if (!h->sigma_set) return fail(SDC_ERR_STATE, "no operator set: call sdc_set_sigma or sdc_set_coefficients first");
   DeviceGuard dg(h->device);
   hipStream_t st = (hipStream_t)stream;
   const Level& l0 = h->L[0];
@@ -1249,7 +1281,7 @@ int sdc_apply(sdc_handle h, int dtype, const void* x, void* y, int ncol, void* s
 
 int sdc_vcycle(sdc_handle h, int dtype, const void* b, void* z, int ncol, void* stream) {
   if (!h || !b || !z || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
-  if (!h->sigma_set) return fail(SDC_ERR_STATE, "sdc_set_sigma has not been called");
+  if (!h->sigma_set) return fail(SDC_ERR_STATE, "no operator set: call sdc_set_sigma or sdc_set_coefficients first");
   DeviceGuard dg(h->device);
   hipStream_t st = (hipStream_t)stream;
   if (dtype == SDC_F64) {
@@ -1268,7 +1300,7 @@ int sdc_vcycle(sdc_handle h, int dtype, const void* b, void* z, int ncol, void* 
 int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solve_opts* o,
               sdc_solve_stats* stats, void* stream) {
   if (!h || !q || !u || !o || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
-  if (!h->sigma_set) return fail(SDC_ERR_STATE, "sdc_set_sigma has not been called");
+  if (!h->sigma_set) return fail(SDC_ERR_STATE, "no operator set: call sdc_set_sigma or sdc_set_coefficients first");
   if (o->mode != SDC_MODE_MIXED && o->mode != SDC_MODE_FP64 && o->mode != SDC_MODE_MIXED_IR)
     return fail(SDC_ERR_ARG, "mode must be SDC_MODE_MIXED, SDC_MODE_MIXED_IR or SDC_MODE_FP64");
   if (!(o->tol > 0) || o->max_iter < 1) return fail(SDC_ERR_ARG, "tol must be > 0, max_iter >= 1");

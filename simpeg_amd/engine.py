@@ -146,6 +146,20 @@ class NodalEngine:
         _lib.check(self.lib.sdc_set_sigma(self.h, _lib.ptr(s), self._stream()), "sdc_set_sigma")
         self._sigma = s
 
+    def set_coefficients(self, cx, cy, cz, bd):
+        """Set the finest-level operator from its edge conductances (+x/+y/+z edge of every node)
+        and the remaining diagonal instead of from sigma (host or device arrays of nN)."""
+        arrs = []
+        for a in (cx, cy, cz, bd):
+            t = a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a, np.float64))
+            t = t.to(self.device, torch.float64).contiguous()
+            if t.numel() != self.nN:
+                raise ValueError(f"coefficient arrays must have {self.nN} entries, got {t.numel()}")
+            arrs.append(t)
+        _lib.check(self.lib.sdc_set_coefficients(self.h, *[_lib.ptr(t) for t in arrs],
+                                                 self._stream()), "sdc_set_coefficients")
+        self._coef = arrs
+
     def apply(self, x, ncol=None, dtype=None):
         ncol = x.shape[0] if ncol is None else ncol
         y = torch.empty_like(x)

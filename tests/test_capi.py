@@ -73,4 +73,5 @@ def test_null_handle_calls():
     assert lib.sdc_destroy(None) == _lib.SDC_OK
     assert lib.sdc_ld(None) == -1
     assert lib.sdc_set_sigma(None, None, None) == _lib.SDC_ERR_ARG
+    assert lib.sdc_set_coefficients(None, None, None, None, None, None) == _lib.SDC_ERR_ARG
     assert lib.sdc_solve(None, None, None, 1, None, None, None) == _lib.SDC_ERR_ARG

@@ -315,3 +315,31 @@ def test_autotuner_picks_converging_parameters(gpu, tiny, tmp_path):
     u_ref = orc.fields(m)
     f = sim.fields(m)
     assert rel(f[:, "phiSolution"][:, 0], u_ref[:, 0]) < 1e-7
+
+
+@pytest.mark.parametrize("bc", ["Robin", "Neumann"])
+def test_solver_wrapper_matches_direct_factorisation(gpu, tiny, bc):
+    """Ainv = NodalGPUSolver(A, mesh=mesh); Ainv * b against splu of the same assembled A, for
+    the simulation's source term (several columns) and for one random vector (which, for the
+    Neumann operator, has a non-zero entry in the identity row 0)."""
+    from simpeg_amd.solvers import NodalGPUSolver
+
+    mesh, survey, m = tiny
+    orc = oracle_for(mesh, survey, bc)
+    orc.set_model(m)
+    A = orc.getA()
+    Ainv = NodalGPUSolver(A, mesh=mesh)
+    try:
+        q = orc.source_term()
+        u = Ainv * q
+        u_ref = orc.solve(q)
+        for s in range(q.shape[1]):
+            assert rel(u[:, s], u_ref[:, s]) < 1e-7, s
+        assert Ainv.last_solve.converged
+        b = np.random.default_rng(4).standard_normal(mesh.n_nodes)
+        x = Ainv * b
+        assert x.shape == b.shape
+        assert rel(x, orc.solve(b)) < 1e-7
+        assert rel(A @ x, b) < 1e-9
+    finally:
+        Ainv.clean()

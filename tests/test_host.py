@@ -181,3 +181,57 @@ def test_miniaturize_recombination_with_oracle():
     lhs = b @ mini_survey_data(dipoles, invs, a)
     rhs = a @ mini_survey_dataT(dipoles, invs, mini.nD, b)
     assert np.isclose(lhs, rhs, rtol=1e-12)
+
+
+@pytest.mark.parametrize("bc", ["Robin", "Neumann"])
+def test_solver_reads_edge_conductances_from_assembled_operator(bc):
+    """NodalGPUSolver's structure extraction: the +x/+y/+z conductances read from getA's
+    sub-diagonals are the oracle's edge inner product over h^2 (G = diag(1/h) D), and the rest
+    of the diagonal is AvgBC @ sigma (Robin) or zero (Neumann)."""
+    from simpeg_amd.solvers import nodal_coefficients
+    from tests._helpers import oracle_for
+
+    mesh, survey, m = tiny_problem()
+    orc = oracle_for(mesh, survey, bc)
+    orc.set_model(m)
+    A = orc.getA()
+    cx, cy, cz, bd, neumann = nodal_coefficients(A, mesh)
+    assert neumann == (bc == "Neumann")
+    hx, hy, hz = (np.asarray(h) for h in mesh.h)
+    nx, ny, nz = mesh.shape_cells
+    n1x, n1y, n1z = nx + 1, ny + 1, nz + 1
+    me = orc.mesh.get_edge_inner_product(orc.sigma).diagonal()
+    nex, ney = nx * n1y * n1z, n1x * ny * n1z
+    mex = me[:nex].reshape((nx, n1y, n1z), order="F") / hx[:, None, None] ** 2
+    mey = me[nex:nex + ney].reshape((n1x, ny, n1z), order="F") / hy[None, :, None] ** 2
+    mez = me[nex + ney:].reshape((n1x, n1y, nz), order="F") / hz[None, None, :] ** 2
+    gx = cx.reshape((n1x, n1y, n1z), order="F")
+    gy = cy.reshape((n1x, n1y, n1z), order="F")
+    gz = cz.reshape((n1x, n1y, n1z), order="F")
+    assert rel(gx[:nx], mex) < 1e-13 and not gx[nx].any()
+    assert rel(gy[:, :ny], mey) < 1e-13 and not gy[:, ny].any()
+    assert rel(gz[:, :, :nz], mez) < 1e-13 and not gz[:, :, nz].any()
+    if bc == "Robin":
+        assert rel(bd, orc.AvgBC @ orc.sigma) < 1e-12
+    else:
+        assert np.abs(bd).max() < 1e-12 * np.abs(A.diagonal()).max()
+
+
+def test_solver_rejects_operators_that_are_not_the_nodal_stencil():
+    from simpeg_amd.solvers import nodal_coefficients
+    from tests._helpers import oracle_for
+
+    mesh, survey, m = tiny_problem()
+    orc = oracle_for(mesh, survey)
+    orc.set_model(m)
+    A = orc.getA().tolil()
+    A[5, 200] = -1.0  # a coupling outside the 7-point pattern
+    with pytest.raises(ValueError, match="7-point"):
+        nodal_coefficients(A.tocsr(), mesh)
+    A = orc.getA().tolil()
+    A[3, 4] *= 2.0  # nonsymmetric edge
+    with pytest.raises(ValueError, match="7-point"):
+        nodal_coefficients(A.tocsr(), mesh)
+    small = TensorMesh([[(10.0, 4)], [(10.0, 4)], [(10.0, 4)]])
+    with pytest.raises(ValueError, match="nodes"):
+        nodal_coefficients(orc.getA(), small)
