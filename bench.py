@@ -60,7 +60,7 @@ def build_problem(name, seed=0):
 class Pipeline:
     """The hot path of Simulation3DNodal.fields/dpred/Jtvec at the engine level."""
 
-    def __init__(self, wl, device):
+    def __init__(self, wl, device, gpu_params=None):
         import torch
 
         from simpeg_amd import maps
@@ -69,7 +69,7 @@ class Pipeline:
         self.torch = torch
         self.wl = wl
         self.sim = Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
-                                     device=device)
+                                     device=device, gpu_params=gpu_params)
         self.sim.model = wl.model
         self.eng = self.sim.engine
         self.nS = wl.survey.nSrc
@@ -242,7 +242,8 @@ def bench_batch(args):
     res = {}
     for ns in (1, 4):
         sims = [Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
-                                  rtol=args.rtol, autotune=True) for wl in batch]
+                                  rtol=args.rtol, autotune=True,
+                                  gpu_params=json.loads(args.gpu_params)) for wl in batch]
         maps_ = [maps.Projection(int(offs[-1]), slice(int(offs[k]), int(offs[k + 1])))
                  for k in range(len(batch))]
         meta = MetaSimulation(sims, maps_, n_streams=ns)
@@ -289,7 +290,8 @@ def bench_gn(args):
     runs = {}
     for prec in ("mixed", "fp64"):
         sim = Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
-                                precision=prec, rtol=args.rtol)
+                                precision=prec, rtol=args.rtol,
+                                gpu_params=json.loads(args.gpu_params))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         m, h = gauss_newton(sim, dobs, wd, m0, beta=args.gn_beta, n_iter=3, cg_iter=10,
@@ -327,6 +329,9 @@ def main():
                     help="kernel class timed with HIP events during the timed steps (roofline)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (PMC byte calibration)")
+    ap.add_argument("--gpu-params", default="{}",
+                    help="JSON dict of solver parameters for the mixed-precision runs, "
+                         "e.g. '{\"graph\": 0}'")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain fp64 per-source baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the fp64 reference solve")
@@ -354,7 +359,7 @@ def main():
     wl = build_problem(args.workload)
     log(f"[bench] workload {wl.name}: cells {wl.mesh.shape_cells}, nodes {wl.mesh.n_nodes}, "
         f"sources {wl.survey.nSrc}, data {wl.survey.nD}")
-    pl = Pipeline(wl, dev)
+    pl = Pipeline(wl, dev, json.loads(args.gpu_params))
     eng = pl.eng
     log(f"[bench] multigrid levels {eng.levels}, ld {eng.ld}")
 

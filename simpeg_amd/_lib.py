@@ -61,6 +61,7 @@ class Params(C.Structure):
         ("wcycle_level", C.c_int32),
         ("fused_tail", C.c_int32),
         ("refine_rhs_per_thread", C.c_int32),
+        ("graph", C.c_int32),
     ]
 
 

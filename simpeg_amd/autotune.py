@@ -64,14 +64,16 @@ def tune(engine, q, ncol=None, mode="mixed", tol=1e-10, candidates=None, repeats
     ncol = q.shape[0] if ncol is None else ncol
     key = _key(engine, ncol, mode)
     cache = load_cache(cache_path) if use_cache else {}
+    # the candidate's fields override the engine's current parameters; the rest (e.g. graph,
+    # vcycle storage) stay as the caller set them, also when the choice comes from the cache
+    base = {f: getattr(engine.params, f) for f, _ in engine.params._fields_}
     if use_cache and key in cache:
-        best = cache[key]["params"]
+        best = {**base, **cache[key]["params"]}
         engine.set_params(**best)
         return best, cache[key].get("timings", [])
-    base = {f: getattr(engine.params, f) for f, _ in engine.params._fields_}
     cands = candidates or CANDIDATES
     timings = []
-    best, best_t = None, float("inf")
+    best, best_cand, best_t = None, None, float("inf")
     u = engine.block(ncol, zero=False)
     for cand in cands:
         p = {**base, **cand}
@@ -88,12 +90,12 @@ def tune(engine, q, ncol=None, mode="mixed", tol=1e-10, candidates=None, repeats
             continue
         timings.append(dict(params=cand, seconds=dt, iters=res.iters))
         if dt < best_t:
-            best, best_t = p, dt
+            best, best_cand, best_t = p, cand, dt
     if best is None:
         engine.set_params(**base)
         raise RuntimeError("no autotuner candidate converged")
     engine.set_params(**best)
     if use_cache:
-        cache[key] = dict(params=best, timings=timings)
+        cache[key] = dict(params=best_cand, timings=timings)
         save_cache(cache, cache_path)
     return best, timings

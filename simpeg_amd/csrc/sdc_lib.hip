@@ -8,7 +8,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -179,6 +181,23 @@ struct sdc_handle_s {
   Work<float> w32;
   Work<double> w64;
   int* h_flag = nullptr;  // pinned host copy of the convergence flag
+  // recorded finest-level V-cycles (sdc_params.graph): one executable graph per input, output,
+  // column count and storage variant, launched on the caller's stream
+  struct GraphRec {
+    const void* b;
+    void* x;
+    int ncol;
+    int variant;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphRec> graphs;
+  // while a V-cycle is being recorded: the graph under construction and its last node
+  struct Recorder {
+    hipGraph_t g = nullptr;
+    hipGraphNode_t last = nullptr;
+    hipError_t err = hipSuccess;
+    std::vector<std::shared_ptr<void>> args;  // argument values, alive until instantiation
+  }* rec = nullptr;
   // event profiler for one kernel class (sdc_profile)
   uint32_t prof_mask = 0;
   struct ProfRec {
@@ -201,6 +220,50 @@ struct DeviceGuard {
     if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
   }
 };
+
+// Forget the captured V-cycles (their kernel arguments name workspaces and parameters).
+void drop_graphs(sdc_handle_s* h) {
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+  h->graphs.clear();
+}
+
+// Every kernel of the V-cycle starts here: a launch on st, or, while h->rec is set, a kernel
+// node appended to the recorded graph after the previous one (the same order as on a stream).
+// The graph is built node by node rather than by stream capture, so recording does not interact
+// with the null-stream operations other host threads may issue meanwhile.
+template <typename... P, typename... A>
+void launch(sdc_handle_s* h, void (*k)(P...), dim3 grid, dim3 block, size_t shmem,
+            hipStream_t st, A&&... a) {
+  auto* rec = h->rec;
+  if (!rec) {
+    hipLaunchKernelGGL(k, grid, block, shmem, st, std::forward<A>(a)...);
+    return;
+  }
+  auto vals = std::make_shared<std::tuple<std::decay_t<P>...>>(static_cast<std::decay_t<P>>(a)...);
+  rec->args.push_back(vals);
+  void* ptrs[sizeof...(P) > 0 ? sizeof...(P) : 1];
+  std::apply(
+      [&](auto&... v) {
+        size_t i = 0;
+        ((ptrs[i++] = (void*)&v), ...);
+      },
+      *vals);
+  hipKernelNodeParams prm = {};
+  prm.func = (void*)k;
+  prm.gridDim = grid;
+  prm.blockDim = block;
+  prm.sharedMemBytes = (unsigned)shmem;
+  prm.kernelParams = ptrs;
+  prm.extra = nullptr;
+  hipGraphNode_t node = nullptr;
+  const hipError_t e = hipGraphAddKernelNode(&node, rec->g, rec->last ? &rec->last : nullptr,
+                                             rec->last ? 1 : 0, &prm);
+  if (e != hipSuccess) {
+    if (rec->err == hipSuccess) rec->err = e;
+    return;
+  }
+  rec->last = node;
+}
 
 // Brackets one launch of a kernel class selected by sdc_profile with events on its stream.
 bool prof_on(const sdc_handle_s* h, int tag) {
@@ -390,6 +453,7 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   const bool want16 = std::is_same<T, float>::value && h->prm.vcycle_bf16 == 3;
   if (w.ncap >= ncol && w.vbytes == vb && (w.r16 != nullptr) == want16) return SDC_OK;
   ncol = std::max(ncol, w.ncap);
+  drop_graphs(h);
   w.release();
   const int cap = std::max(ncol, 1);
   const size_t ld0 = h->L[0].ld;
@@ -557,9 +621,7 @@ int tail_vcycle(sdc_handle_s* h, const S* b, S* x, int ncol, const int* done, hi
   }
   const size_t bytes = (size_t)(off + ymax) * sizeof(float);
   if (bytes != h->tail_lds) return fail(SDC_ERR_STATE, "internal: tail LDS size");
-  HIPCK(hipFuncSetAttribute((const void*)k_tail_vcycle<S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-  hipLaunchKernelGGL(k_tail_vcycle<S>, dim3(ncol), dim3(1024), bytes, st, d, b, x, h->L[lt].ld,
+  launch(h, k_tail_vcycle<S>, dim3(ncol), dim3(1024), bytes, st, d, b, x, h->L[lt].ld,
                      done);
   HIPCK(hipGetLastError());
   return SDC_OK;
@@ -583,11 +645,11 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   if (l == nl - 1) {
     const T* ainv = (const T*)(std::is_same<T, float>::value ? (const void*)h->ainv32
                                                               : (const void*)h->ainv64);
-    hipLaunchKernelGGL((k_dense_solve<T, B, O>), dim3(nblk(lv.nN), ncol), dim3(SDC_BLOCK), 0, st,
+    launch(h, (k_dense_solve<T, B, O>), dim3(nblk(lv.nN), ncol), dim3(SDC_BLOCK), 0, st,
                        ainv, (int)lv.nN, b, x, lv.ld, ncol, done);
     if constexpr (std::is_same<B, T>::value && std::is_same<O, T>::value) {
       if (dot_out) {
-        DISPATCH_R(R, hipLaunchKernelGGL((k_dot<T, R>), grid2(lv.nN, ncol, R), dim3(SDC_BLOCK), 0,
+        DISPATCH_R(R, launch(h, (k_dot<T, R>), grid2(lv.nN, ncol, R), dim3(SDC_BLOCK), 0,
                                          st, b, (const T*)x, lv.ld, lv.nN, ncol, W.partial, done));
       }
     }
@@ -623,20 +685,20 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   if (nu1 >= 2) {
     // steps 0 and 1 fused
     prof_begin(h, l == 0 ? SDC_PROF_PAIR : -1, st);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth_pair<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
+    DISPATCH_R(R, launch(h, (k_smooth_pair<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
                                      ncol, cz, (T)cc.c1[1], (T)cc.c2[1], done));
     prof_end(h, l == 0 ? SDC_PROF_PAIR : -1, st, nodes * 5 * sT + nodes * cols * (sB + sS));
   } else {
-    DISPATCH_R(R, hipLaunchKernelGGL((k_smooth0<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
+    DISPATCH_R(R, launch(h, (k_smooth0<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
                                      ncol, cz, done));
   }
   for (int k = 2; k < nu1; ++k) {
     if (k == 2) {
-      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, S, R, 2, false>), g, blk, 0, st, s, b,
+      DISPATCH_R(R, launch(h, (k_cheb<T, B, S, S, R, 2, false>), g, blk, 0, st, s, b,
                                        (const S*)cur, (const S*)nullptr, oth, lv.ld, ncol,
                                        (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
     } else {
-      DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, S, R, 1, false>), g, blk, 0, st, s, b,
+      DISPATCH_R(R, launch(h, (k_cheb<T, B, S, S, R, 1, false>), g, blk, 0, st, s, b,
                                        (const S*)cur, (const S*)oth, oth, lv.ld, ncol,
                                        (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
     }
@@ -659,11 +721,11 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
       // PREV = 0 on the first step (no previous iterate), 1 after
 #define SDC_CHEB_STEP(PREV, OT, OUT)                                                           \
       if (part) {                                                                                \
-        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, OT, R, PREV, true>), g, blk, 0, st, s, \
+        DISPATCH_R(R, launch(h, (k_cheb<T, B, S, OT, R, PREV, true>), g, blk, 0, st, s, \
                                          b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol, c1,  \
                                          c2, cz, part, done));                                   \
       } else {                                                                                   \
-        DISPATCH_R(R, hipLaunchKernelGGL((k_cheb<T, B, S, OT, R, PREV, false>), g, blk, 0, st,   \
+        DISPATCH_R(R, launch(h, (k_cheb<T, B, S, OT, R, PREV, false>), g, blk, 0, st,   \
                                          s, b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol,   \
                                          c1, c2, cz, part, done));                               \
       }
@@ -689,7 +751,7 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   };
   for (int gam = 0; gam < ngam; ++gam) {
     prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_residual<T, B, S, R>), g, blk, 0, st, s, b, (const S*)cur,
+    DISPATCH_R(R, launch(h, (k_residual<T, B, S, R>), g, blk, 0, st, s, b, (const S*)cur,
                                      res, lv.ld, ncol, done));
     prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * (sB + 2 * sS));
     const int rtag = l == 0 ? SDC_PROF_RESTRICT : -1;
@@ -722,15 +784,15 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
         prof_begin(h, rtag, st);
         switch (a) {
           case 0:
-            DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict_axis<T, S, 0, R>), ga, blk, 0, st, ax, fi,
+            DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 0, R>), ga, blk, 0, st, ax, fi,
                                               wl, wh, src, dst, lv.ld, ldo, ncol, done));
             break;
           case 1:
-            DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict_axis<T, S, 1, R>), ga, blk, 0, st, ax, fi,
+            DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 1, R>), ga, blk, 0, st, ax, fi,
                                               wl, wh, src, dst, lv.ld, ldo, ncol, done));
             break;
           default:
-            DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict_axis<T, S, 2, R>), ga, blk, 0, st, ax, fi,
+            DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 2, R>), ga, blk, 0, st, ax, fi,
                                               wl, wh, src, dst, lv.ld, ldo, ncol, done));
             break;
         }
@@ -739,14 +801,14 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
       }
     } else {
       prof_begin(h, rtag, st);
-      DISPATCH_R(RX, hipLaunchKernelGGL((k_restrict<T, S, R>), grid2(cl.nN, ncol, RX), blk, 0, st,
+      DISPATCH_R(RX, launch(h, (k_restrict<T, S, R>), grid2(cl.nN, ncol, RX), blk, 0, st,
                                         t,
                                         (const S*)res, cb, lv.ld, cl.ld, ncol, done));
       prof_end(h, rtag, st, (nodes + (double)cl.nN) * cols * sS);
     }
     CK((vcycle<T, S, S, S>(h, l + 1, cb, cx, ncol, false, done, st)));
     prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
-    DISPATCH_R(RX, hipLaunchKernelGGL((k_prolong_add<T, S, R>), grid2(lv.nN, ncol, RX), blk, 0, st,
+    DISPATCH_R(RX, launch(h, (k_prolong_add<T, S, R>), grid2(lv.nN, ncol, RX), blk, 0, st,
                                       t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
     prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sS);
     if (gam + 1 < ngam) post_smooth(false);
@@ -837,6 +899,41 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   return SDC_OK;
 }
 
+// Runs fn(stream) -- V-cycle launches only, no host synchronisation -- on st, through an executable
+// graph recorded on the first call with the same (b, x, ncol, variant) when sdc_params.graph is
+// set and none of the V-cycle's kernel classes is being profiled.
+template <typename F>
+int graph_run(sdc_handle_s* h, const void* b, void* x, int ncol, int variant, hipStream_t st,
+              F&& fn) {
+  // events bracketing the V-cycle's own kernel classes cannot sit inside the graph
+  constexpr uint32_t vcycle_tags = (1u << SDC_PROF_SMOOTH) | (1u << SDC_PROF_SMOOTH0) |
+                                   (1u << SDC_PROF_PAIR) | (1u << SDC_PROF_RESIDUAL) |
+                                   (1u << SDC_PROF_RESTRICT) | (1u << SDC_PROF_PROLONG);
+  if (!h->prm.graph || (h->prof_mask & vcycle_tags)) return fn(st);
+  for (const auto& g : h->graphs)
+    if (g.b == b && g.x == x && g.ncol == ncol && g.variant == variant) {
+      HIPCK(hipGraphLaunch(g.exec, st));
+      return SDC_OK;
+    }
+  sdc_handle_s::Recorder rec;
+  HIPCK(hipGraphCreate(&rec.g, 0));
+  h->rec = &rec;
+  const int rc = fn(st);
+  h->rec = nullptr;
+  if (rc != SDC_OK || rec.err != hipSuccess) {
+    (void)hipGraphDestroy(rec.g);
+    if (rc != SDC_OK) return rc;
+    HIPCK(rec.err);
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, rec.g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(rec.g);
+  HIPCK(ei);
+  h->graphs.push_back({b, x, ncol, variant, exec});
+  HIPCK(hipGraphLaunch(exec, st));
+  return SDC_OK;
+}
+
 // fp32 PCG with reliable updates: the fp32 partial solution is folded into the fp64 solution u
 // and the fp32 residual is replaced by the fp64 true residual q - A u whenever every active
 // column's fp32 residual passed the tolerance, or some column's residual dropped by
@@ -904,11 +1001,17 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   for (it = 1; it <= maxit; ++it) {
     const bool first = it == 1;
     if (zb) {
-      CK((vcycle<float, __bf16, float, __bf16>(h, 0, W.r, z16, ncol, true, nullptr, st)));
+      CK(graph_run(h, W.r, z16, ncol, 1, st, [&](hipStream_t s) {
+        return vcycle<float, __bf16, float, __bf16>(h, 0, W.r, z16, ncol, true, nullptr, s);
+      }));
     } else if (r16) {
-      CK((vcycle<float, __bf16, __bf16, float>(h, 0, r16, W.z, ncol, true, nullptr, st)));
+      CK(graph_run(h, r16, W.z, ncol, 3, st, [&](hipStream_t s) {
+        return vcycle<float, __bf16, __bf16, float>(h, 0, r16, W.z, ncol, true, nullptr, s);
+      }));
     } else {
-      CK(vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, st));
+      CK(graph_run(h, W.r, W.z, ncol, h->prm.vcycle_bf16 ? 2 : 0, st, [&](hipStream_t s) {
+        return vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, s);
+      }));
     }
     hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                        W.beta, (const int*)W.active, first ? 1 : 0, done);
@@ -1028,6 +1131,7 @@ int sdc_default_params(sdc_params* p) {
   p->wcycle_level = -1;
   p->fused_tail = 1;
   p->refine_rhs_per_thread = 0;
+  p->graph = 1;
   p->check_every = 1;
   p->reliable_delta = 1e-4;
   return SDC_OK;
@@ -1124,6 +1228,12 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
     h->h_flag = nullptr;
     return bail(fail(SDC_ERR_ALLOC, "hipHostMalloc failed"));
   }
+  // the fused coarse tail may use all 160 KB of LDS (set once, outside any graph capture)
+  if (hipFuncSetAttribute((const void*)k_tail_vcycle<__bf16>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_tail_vcycle<float>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+    return bail(fail(SDC_ERR_HIP, "hipFuncSetAttribute failed"));
   *out = h;
   return SDC_OK;
 }
@@ -1140,6 +1250,7 @@ int sdc_destroy(sdc_handle h) {
   h->w32.release();
   h->w64.release();
   if (h->h_flag) (void)hipHostFree(h->h_flag);
+  drop_graphs(h);
   for (auto& p : h->prof)
     for (auto e : p.ev) (void)hipEventDestroy(e);
   delete h;
@@ -1176,6 +1287,9 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
     return fail(SDC_ERR_ARG, "refine_rhs_per_thread must be 0, 1, 2, 4 or 8");
   if (!(p->reliable_delta > 0) || !(p->reliable_delta < 1))
     return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
+  if (p->graph != 0 && p->graph != 1) return fail(SDC_ERR_ARG, "graph must be 0 or 1");
+  DeviceGuard dg(h->device);
+  drop_graphs(h);
   h->prm = *p;
   if (h->prm.check_every < 1) h->prm.check_every = 1;
   h->wlevel = pick_wlevel(h);

@@ -127,6 +127,8 @@ class Projection(IdentityMap):
     def __init__(self, nP, index, **kwargs):
         super().__init__(nP=nP)
         self.index = np.atleast_1d(np.arange(nP)[index] if isinstance(index, slice) else index)
+        # each model entry selected at most once: P^T v is a scatter into a zero vector
+        self.unique_index = bool(np.unique(self.index).size == self.index.size)
         self._P = sp.csr_matrix((np.ones(self.index.size), (np.arange(self.index.size), self.index)),
                                 shape=(self.index.size, nP))
 

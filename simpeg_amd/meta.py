@@ -15,6 +15,20 @@ import concurrent.futures as cf
 import numpy as np
 import torch
 
+from .maps import Projection
+
+
+def sum_adjoint(mappings, model, parts):
+    """sum_i mapping_i'(m)^T part_i, accumulated in place: a Projection with distinct indices
+    adds its part into its own entries (no full-length temporary per problem)."""
+    out = np.zeros(np.asarray(model).size)
+    for mp, part in zip(mappings, parts):
+        if isinstance(mp, Projection) and mp.unique_index:
+            out[mp.index] += part
+        else:
+            out += mp.deriv(model).T @ part
+    return out
+
 
 class MetaSimulation:
     def __init__(self, simulations, mappings, n_streams=4):
@@ -102,10 +116,7 @@ class MetaSimulation:
         v = np.asarray(v, dtype=float)
         parts = self._map(lambda i: self.simulations[i].Jtvec(
             self.simulations[i].model, v[self._data_offsets[i]:self._data_offsets[i + 1]], f=f[i]))
-        out = 0
-        for i, part in enumerate(parts):
-            out = out + self.mappings[i].deriv(self._model).T @ part
-        return out
+        return sum_adjoint(self.mappings, self._model, parts)
 
     def getJtJdiag(self, m, W=None, f=None):
         self.model = m

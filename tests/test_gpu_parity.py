@@ -311,6 +311,12 @@ def test_autotuner_picks_converging_parameters(gpu, tiny, tmp_path):
     assert len(timings) >= 3 and all(t["seconds"] > 0 for t in timings)
     best2, _ = tune(sim.engine, q, survey.nSrc, cache_path=cache)
     assert best2 == best
+    # a cached choice overrides only the tuned fields
+    sim.engine.set_params(graph=1 - best["graph"])
+    best3, _ = tune(sim.engine, q, survey.nSrc, cache_path=cache)
+    assert best3["graph"] == 1 - best["graph"] and sim.engine.params.graph == best3["graph"]
+    assert {k: v for k, v in best3.items() if k != "graph"} == \
+        {k: v for k, v in best.items() if k != "graph"}
     orc = oracle_for(mesh, survey)
     u_ref = orc.fields(m)
     f = sim.fields(m)
@@ -343,3 +349,32 @@ def test_solver_wrapper_matches_direct_factorisation(gpu, tiny, bc):
         assert rel(A @ x, b) < 1e-9
     finally:
         Ainv.clean()
+
+
+def test_graph_replay_matches_launched_cycle(gpu, tiny):
+    """With gpu_params graph=1 the finest-level V-cycle is captured once per workspace and
+    column count and replayed every PCG iteration, with the solver's own launches and host
+    polls between replays.  Every result of a sequence of solves -- fields, Jtvec, fields for a
+    new model (same graph, new coefficients), a 2-column solve (a second graph), the full batch
+    again -- must equal the launched cycle's bit for bit."""
+    mesh, survey, m = tiny
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(survey.nD)
+    m2 = m + 0.2 * rng.standard_normal(m.size)
+    res = []
+    for g in (0, 1):
+        sim = _sim(mesh, survey, gpu_params={"graph": g})
+        f = sim.fields(m)
+        out = [f[:, "phiSolution"].copy(), sim.Jtvec(m, v, f=f)]
+        f2 = sim.fields(m2)
+        u2 = f2[:, "phiSolution"].copy()
+        out.append(u2)
+        eng = sim.engine
+        q = eng.to_block(u2[:, :2])
+        u, _ = eng.solve(q, 2, mode="mixed", tol=1e-8)
+        out.append(eng.from_block(u, 2))
+        out.append(sim.fields(m)[:, "phiSolution"].copy())
+        res.append(out)
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+    assert rel(res[1][0], oracle_for(mesh, survey).fields(m)) < 1e-7

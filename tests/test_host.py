@@ -235,3 +235,19 @@ def test_solver_rejects_operators_that_are_not_the_nodal_stencil():
     small = TensorMesh([[(10.0, 4)], [(10.0, 4)], [(10.0, 4)]])
     with pytest.raises(ValueError, match="nodes"):
         nodal_coefficients(orc.getA(), small)
+
+
+def test_meta_adjoint_sum_matches_sparse_transpose():
+    """MetaSimulation.Jtvec's accumulation equals sum_i P_i^T part_i for disjoint projections,
+    a projection that selects an entry twice, and a non-projection map."""
+    from simpeg_amd.meta import sum_adjoint
+
+    rng = np.random.default_rng(0)
+    n = 50
+    model = rng.standard_normal(n)
+    mps = [maps.Projection(n, slice(0, 20)), maps.Projection(n, slice(20, 50)),
+           maps.Projection(n, np.array([3, 7, 3, 49])), maps.ExpMap(nP=n)]
+    assert [mp.unique_index for mp in mps[:3]] == [True, True, False]
+    parts = [rng.standard_normal(mp.shape[0]) for mp in mps]
+    ref = sum(np.asarray(mp.deriv(model).T @ p).ravel() for mp, p in zip(mps, parts))
+    assert np.allclose(sum_adjoint(mps, model, parts), ref, rtol=1e-14, atol=1e-14)
