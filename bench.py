@@ -297,15 +297,18 @@ def bench_gn(args):
         m, h = gauss_newton(sim, dobs, wd, m0, beta=args.gn_beta, n_iter=3, cg_iter=10,
                             recompute=not args.gn_store_fields, verbose=True)
         torch.cuda.synchronize()
-        runs[prec] = (m, h, time.perf_counter() - t0)
-        log(f"[bench] GN {prec}: {runs[prec][2]:.1f} s, phi {h.phi}")
-    mm, hm, tm = runs["mixed"]
-    m64, h64, t64 = runs["fp64"]
+        runs[prec] = (m, h, time.perf_counter() - t0, sim.solve_count, sim.solve_iterations)
+        log(f"[bench] GN {prec}: {runs[prec][2]:.1f} s, phi {h.phi}, {sim.solve_count} batched "
+            f"solves, {sim.solve_iterations / max(sim.solve_count, 1):.1f} PCG iterations each")
+    mm, hm, tm, nsm, itm = runs["mixed"]
+    m64, h64, t64, ns64, it64 = runs["fp64"]
     out = {"metric": "Gauss-Newton (3 x 10 CG) wall time, 176x176x104 cells / 256 sources",
            "unit": "s", "value": round(tm, 2), "n_gpus": 1, "higher_is_better": False,
            "config": {"workload": wl.name, "nodes": wl.mesh.n_nodes, "sources": wl.survey.nSrc,
                       "data": wl.survey.nD, "recompute_fields": not args.gn_store_fields},
            "fp64_seconds": round(t64, 2),
+           "batched_solves": nsm, "pcg_iterations_per_solve": round(itm / max(nsm, 1), 2),
+           "fp64_pcg_iterations_per_solve": round(it64 / max(ns64, 1), 2),
            "phi_mixed": hm.phi, "phi_fp64": h64.phi,
            "model_update_rel_diff": float(np.linalg.norm(mm - m64) / np.linalg.norm(m64 - m0))}
     print(json.dumps(out), flush=True)

@@ -5,8 +5,11 @@ Gauss-Newton step solves (J^T Wd^T Wd J + beta Wm^T Wm) dm = -grad with a fixed 
 conjugate-gradient steps (the inexact Gauss-Newton of simpeg/optimization.py:1345-1416, without
 its BFGS preconditioner), every Hessian product being one batched Jvec and one batched Jtvec,
 followed by an Armijo backtracking line search.  With ``recompute=True`` the source fields are not
-kept between products: every Jvec / Jtvec solves the forward problem again, trading solves for
-memory (checkpointing).
+kept between Hessian products (checkpointing): each product solves the forward problem once and
+shares those fields between its Jvec and its Jtvec, then drops them, so the fields never
+outlive one product; the gradient reuses the fields of the line search's accepted model.  The
+peak device memory is that of one product (fields, adjoint block and solver workspace) either
+way; recomputing costs one extra batched solve per product.
 """
 
 from __future__ import annotations
@@ -53,12 +56,15 @@ def gauss_newton(sim, dobs, wd, m0, beta, mref=None, wm=None, n_iter=3, cg_iter=
     hist.phi_m.append(pm)
     hist.phi.append(pd + pm)
     for it in range(n_iter):
-        fk = None if recompute else f
-        g = sim.Jtvec(m, wd2 * r, f=fk) + beta * wm2 * (m - mref)
+        # f: the fields at m from the last objective evaluation
+        g = sim.Jtvec(m, wd2 * r, f=f) + beta * wm2 * (m - mref)
+        if recompute:
+            f = None
 
         def H(v):
-            jv = sim.Jvec(m, v, f=None if recompute else f)
-            return sim.Jtvec(m, wd2 * jv, f=None if recompute else f) + beta * wm2 * v
+            fh = sim.fields(m) if recompute else f
+            jv = sim.Jvec(m, v, f=fh)
+            return sim.Jtvec(m, wd2 * jv, f=fh) + beta * wm2 * v
 
         # conjugate gradients on H dm = -g, fixed number of steps
         dm = np.zeros_like(m)

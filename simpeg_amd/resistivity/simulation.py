@@ -108,6 +108,9 @@ class Simulation3DNodal:
         self.max_iter = int(max_iter)
         self.jchunk = int(jchunk)
         self._device = device
+        # batched solves issued and their PCG iterations (instrumentation)
+        self.solve_count = 0
+        self.solve_iterations = 0
         self._gpu_params = dict(gpu_params or {})
         self._coarse_max = int(coarse_max)
         self._engine = None
@@ -251,6 +254,8 @@ class Simulation3DNodal:
                                    inner_tol=self.inner_rtol, max_outer=self.max_outer,
                                    max_iter=self.max_iter)
         self.last_solve = res
+        self.solve_count += 1
+        self.solve_iterations += res.iters
         return u
 
     # ------------------------------------------------------------------ survey -> sparse maps

@@ -92,6 +92,11 @@ class NodalGPUSolver:
         b = np.asarray(rhs, dtype=np.float64)
         vec = b.ndim == 1
         B = b[:, None] if vec else b
+        if B.ndim != 2 or B.shape[0] != self.engine.nN:
+            raise ValueError(f"rhs must have shape ({self.engine.nN},) or ({self.engine.nN}, k), "
+                             f"got {b.shape}")
+        if B.shape[1] == 0:
+            return np.zeros_like(b)
         eng = self.engine
         q = eng.to_block(B)
         u, res = eng.solve(q, B.shape[1], mode=self.precision, tol=self.rtol,

@@ -378,3 +378,36 @@ def test_graph_replay_matches_launched_cycle(gpu, tiny):
     for a, b in zip(*res):
         assert np.array_equal(a, b)
     assert rel(res[1][0], oracle_for(mesh, survey).fields(m)) < 1e-7
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_ragged_batches_and_zero_columns(gpu, tiny, precision):
+    """Column counts that are not a multiple of the columns per work-item (1, 3, 13, 33), a
+    zero right-hand side among them (stays exactly zero, counts as converged), an empty block,
+    and two blocks of different widths on one handle, against splu of the same operator."""
+    from simpeg_amd.solvers import NodalGPUSolver
+
+    mesh, survey, m = tiny
+    orc = oracle_for(mesh, survey)
+    orc.set_model(m)
+    A = orc.getA()
+    Ainv = NodalGPUSolver(A, mesh=mesh, precision=precision)
+    rng = np.random.default_rng(11)
+    try:
+        assert Ainv.solve(np.zeros((mesh.n_nodes, 0))).shape == (mesh.n_nodes, 0)
+        for k in (1, 3, 13, 33, 3):
+            b = rng.standard_normal((mesh.n_nodes, k))
+            if k > 1:
+                b[:, 1] = 0.0
+            x = Ainv * b
+            assert Ainv.last_solve.converged
+            ref = orc.solve(b)
+            for c in range(k):
+                if k > 1 and c == 1:
+                    assert not x[:, c].any()
+                else:
+                    assert rel(x[:, c], ref[:, c]) < 1e-7, (k, c)
+        with pytest.raises(ValueError):
+            Ainv * np.ones(mesh.n_nodes + 1)
+    finally:
+        Ainv.clean()
