@@ -90,7 +90,7 @@ typedef struct {
   int32_t xfer_separable;      /* restriction: 1 = one pass per coarsened axis, 0 = one 27-point
                                   gather pass */
   int32_t wcycle_level;        /* 0: V-cycle; l >= 1: the coarse correction of level l runs twice
-                                  (W-cycle on that level); -1: the first level with at most 1/16
+                                  (W-cycle on that level); -1: the first level with at most 1/4
                                   of the finest level's nodes */
   int32_t fused_tail;          /* 1: the fp32 solver runs the small coarse levels that fit into
                                   LDS as one launch (one work-group per column); 0: off */
@@ -100,6 +100,9 @@ typedef struct {
                                   once per (workspace, column count) as a HIP graph and replays
                                   it every iteration (not while sdc_profile brackets one of its
                                   kernel classes); 0: launches */
+  int32_t interp;              /* multigrid interpolation: 1 = operator-dependent (a node between
+                                  two coarse nodes weights them by the conductances of its two
+                                  edges on its grid line), 0 = geometric (linear in position) */
 } sdc_params;
 
 typedef struct {

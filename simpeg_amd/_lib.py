@@ -62,6 +62,7 @@ class Params(C.Structure):
         ("fused_tail", C.c_int32),
         ("refine_rhs_per_thread", C.c_int32),
         ("graph", C.c_int32),
+        ("interp", C.c_int32),
     ]
 
 

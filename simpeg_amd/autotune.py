@@ -2,8 +2,8 @@
 
 For a given problem (engine with sigma set, right-hand-side block) it times complete batched
 solves for a small grid of candidates -- smoothing schedule (nu_pre, nu_post), Chebyshev
-interval ratio, W-cycle level, poll cadence, columns per work-item -- and keeps the fastest that
-converges.  Results are
+interval ratio, W-cycle level, interpolation, poll cadence, columns per work-item -- and keeps
+the fastest that converges.  Results are
 cached in a JSON file keyed by mesh shape, column count (rounded up to a power of two), boundary
 condition and precision, so a later run with the same shape reuses the choice without timing.
 """
@@ -21,7 +21,8 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "simpeg_amd", "a
 CANDIDATES = [
     dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=-1),
     dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=0),
-    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=1),
+    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=2),
+    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=-1, interp=0),
     dict(nu_pre=2, nu_post=2, cheb_ratio=4.0, rhs_per_thread=0, wcycle_level=-1),
     dict(nu_pre=2, nu_post=3, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=-1),
     dict(nu_pre=3, nu_post=3, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=0),

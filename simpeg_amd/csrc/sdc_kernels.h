@@ -201,10 +201,130 @@ struct Xfer {
   uint32_t cn1x, cn1y, cplane, cN;
   FastDiv ffx, ffy, cfx, cfy;
   int remap;
+  // operator-dependent interpolation (null: the geometric tables above).  P = Px Py Pz, each
+  // factor a 1D interpolation along its axis whose weight of the right coarse node is stored
+  // per point of the factor's grid: wx over the fine grid (fn1x, fn1y, fn1z), wy over
+  // (cn1x, fn1y, fn1z), wz over (cn1x, cn1y, fn1z)
+  const T* wx;
+  const T* wy;
+  const T* wz;
 };
 
-// rc = P^T rf: every coarse node gathers the 3x3x3 fine nodes around its fine position with
-// separable weights; a missing neighbour (domain edge or coincident coarse node) has weight 0
+// Weights of the 8 coarse nodes around fine node n = (i, j, k); corner a + 2b + 4c is
+// (I0 + a, J0 + b, K0 + c).  P = Px Py Pz makes the corner weight the product
+// wx_a(i, j, k) * wy_b(I0 + a, j, k) * wz_c(I0 + a, J0 + b, k), each factor read on its own grid;
+// with the geometric tables the three factors depend on i, j and k only (trilinear).
+template <typename T>
+__device__ __forceinline__ void prolong_weights(const Xfer<T>& t, uint32_t n, uint32_t i,
+                                                uint32_t j, uint32_t k, uint32_t I0, uint32_t J0,
+                                                T w[8]) {
+  T fx, fy[2], fz[2][2];
+  if (t.wx) {
+    fx = t.wx[n];
+    const uint32_t py = I0 + t.cn1x * (j + t.fn1y * k);
+    fy[0] = t.wy[py];
+    fy[1] = t.wy[py + 1];
+    const uint32_t pz = I0 + t.cn1x * J0 + t.cplane * k;
+    fz[0][0] = t.wz[pz];
+    fz[1][0] = t.wz[pz + 1];
+    fz[0][1] = t.wz[pz + t.cn1x];
+    fz[1][1] = t.wz[pz + t.cn1x + 1];
+  } else {
+    fx = t.pwx[i];
+    fy[0] = fy[1] = t.pwy[j];
+    fz[0][0] = fz[1][0] = fz[0][1] = fz[1][1] = t.pwz[k];
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        w[a + 2 * b + 4 * c] = (a ? fx : T(1) - fx) * (b ? fy[a] : T(1) - fy[a]) *
+                               (c ? fz[a][b] : T(1) - fz[a][b]);
+}
+
+// Weights of P^T for coarse node (I, J, K) at fine (fi, fj, fk) over the 3x3x3 fine nodes
+// (xi[a], yj[b], zk[c]) (index a + 3b + 9c), the transpose of prolong_weights: a fine neighbour
+// halfway to the next coarse node on the left contributes its weight of the right coarse node,
+// one on the right 1 minus that; 0 where the neighbour is missing or is itself a coarse node
+// (the geometric tables' zeros), the index then being clamped into the grid.
+template <typename T>
+__device__ __forceinline__ void restrict_weights(const Xfer<T>& t, uint32_t I, uint32_t J,
+                                                 uint32_t K, const int xi[3], const int yj[3],
+                                                 const int zk[3], T w[27]) {
+  const T gx[3] = {t.rlx[I], T(1), t.rhx[I]};
+  const T gy[3] = {t.rly[J], T(1), t.rhy[J]};
+  const T gz[3] = {t.rlz[K], T(1), t.rhz[K]};
+  if (!t.wx) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) w[a + 3 * b + 9 * c] = gx[a] * gy[b] * gz[c];
+    return;
+  }
+  const uint32_t pz = I + t.cn1x * J;
+  const T z[3] = {gz[0] != T(0) ? t.wz[pz + t.cplane * (uint32_t)zk[0]] : T(0), T(1),
+                  gz[2] != T(0) ? T(1) - t.wz[pz + t.cplane * (uint32_t)zk[2]] : T(0)};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const uint32_t py = I + t.cn1x * t.fn1y * (uint32_t)zk[c];
+    const T y[3] = {gy[0] != T(0) ? t.wy[py + t.cn1x * (uint32_t)yj[0]] : T(0), T(1),
+                    gy[2] != T(0) ? T(1) - t.wy[py + t.cn1x * (uint32_t)yj[2]] : T(0)};
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const uint32_t row = (uint32_t)yj[b] * t.fn1x + (uint32_t)zk[c] * t.fplane;
+      const T yz = y[b] * z[c];
+      w[3 * b + 9 * c] = gx[0] != T(0) ? t.wx[row + (uint32_t)xi[0]] * yz : T(0);
+      w[1 + 3 * b + 9 * c] = yz;
+      w[2 + 3 * b + 9 * c] = gx[2] != T(0) ? (T(1) - t.wx[row + (uint32_t)xi[2]]) * yz : T(0);
+    }
+  }
+}
+
+// Operator-dependent interpolation weights of the factor of P = Px Py Pz along axis AX, over
+// the factor's grid (g0, g1, g2) (see Xfer::wx).  A fine node halfway between two coarse nodes
+// along AX gets the weight cR / (cL + cR) for the right one, cL and cR being the conductances of
+// its two edges along AX on the fine line through it: the exact interpolation of the 1D
+// operator, equal to the geometric weight in a homogeneous medium.  Coincident nodes keep their
+// 0 / 1 and nodes of longer gaps the geometric weight.  t: the geometric tables (fp64);
+// c: the fine level's conductances along AX.
+template <int AX>
+__global__ __launch_bounds__(SDC_BLOCK) void k_interp_weights(Xfer<double> t, uint32_t g0,
+                                                              uint32_t g1, uint32_t g2,
+                                                              const double* __restrict__ c,
+                                                              double* __restrict__ w64,
+                                                              float* __restrict__ w32) {
+  const size_t n = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (n >= (size_t)g0 * g1 * g2) return;
+  const uint32_t p0 = (uint32_t)(n % g0);
+  const size_t q = n / g0;
+  const uint32_t p1 = (uint32_t)(q % g1);
+  const uint32_t p2 = (uint32_t)(q / g1);
+  // fine indices of the grid point: coarse x on the Py and Pz grids, coarse y on the Pz grid
+  const uint32_t fi = AX == 0 ? p0 : (uint32_t)t.rix[p0];
+  const uint32_t fj = AX == 2 ? (uint32_t)t.riy[p1] : p1;
+  const uint32_t fk = p2;
+  const uint32_t f = AX == 0 ? fi : AX == 1 ? fj : fk;
+  const int* pi = AX == 0 ? t.pix : AX == 1 ? t.piy : t.piz;
+  const int* ri = AX == 0 ? t.rix : AX == 1 ? t.riy : t.riz;
+  const double* pw = AX == 0 ? t.pwx : AX == 1 ? t.pwy : t.pwz;
+  const size_t stride = AX == 0 ? 1 : AX == 1 ? (size_t)t.fn1x : (size_t)t.fplane;
+  const int I0 = pi[f];
+  double w = pw[f];
+  if (ri[I0] != (int)f && ri[I0 + 1] != (int)f && ri[I0 + 1] - ri[I0] == 2) {
+    const size_t node = fi + (size_t)fj * t.fn1x + (size_t)fk * t.fplane;
+    const double cl = c[node - stride], cr = c[node];
+    if (cl + cr > 0.0) w = cr / (cl + cr);
+  }
+  w64[n] = w;
+  w32[n] = (float)w;
+}
+
+// rc = P^T rf: every coarse node gathers the 3x3x3 fine nodes around its fine position
+// (restrict_weights); a missing neighbour (domain edge or coincident coarse node) has weight 0
 // and its index is clamped into the grid.
 template <typename T, typename X, int R>
 __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const X* __restrict__ rf,
@@ -218,13 +338,12 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const X* __re
   const uint32_t K = fdiv(rr, t.cfy);
   const uint32_t J = rr - K * t.cn1y;
   const int fi = t.rix[I], fj = t.riy[J], fk = t.riz[K];
-  const T wx[3] = {t.rlx[I], T(1), t.rhx[I]};
-  const T wy[3] = {t.rly[J], T(1), t.rhy[J]};
-  const T wz[3] = {t.rlz[K], T(1), t.rhz[K]};
   const int fn1z = (int)(t.fN / t.fplane);
   const int xi[3] = {fi > 0 ? fi - 1 : 0, fi, fi + 1 < (int)t.fn1x ? fi + 1 : fi};
   const int yj[3] = {fj > 0 ? fj - 1 : 0, fj, fj + 1 < (int)t.fn1y ? fj + 1 : fj};
   const int zk[3] = {fk > 0 ? fk - 1 : 0, fk, fk + 1 < fn1z ? fk + 1 : fk};
+  T w[27];
+  restrict_weights(t, I, J, K, xi, yj, zk, w);
   T acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = T(0);
@@ -232,7 +351,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const X* __re
   for (int c = 0; c < 3; ++c) {
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const T wyz = wy[b] * wz[c];
+      const T* wb = w + 3 * b + 9 * c;
       const uint32_t row = (uint32_t)yj[b] * t.fn1x + (uint32_t)zk[c] * t.fplane;
       const uint32_t i0 = row + xi[0], i1 = row + xi[1], i2 = row + xi[2];
       T v[R][3];
@@ -244,8 +363,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const X* __re
         v[r][2] = (T)f[i2];
       }
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        acc[r] += wyz * (wx[0] * v[r][0] + v[r][1] + wx[2] * v[r][2]);
+      for (int r = 0; r < R; ++r) acc[r] += wb[0] * v[r][0] + wb[1] * v[r][1] + wb[2] * v[r][2];
     }
   }
 #pragma unroll
@@ -255,8 +373,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const X* __re
 
 // One factor of the separable restriction P^T = Pz^T Py^T Px^T: along axis AX the output has
 // the coarse count, every output entry gathers the three input entries around its fine index
-// (weights lo, 1, hi; a weight is 0 where that neighbour is missing or is itself a coarse
-// node, and the index is clamped).  Input dims (a0, a1, a2), output dims (b0, b1, b2), column
+// (weights lo, 1, hi, or from wop for the operator-dependent P; a weight is 0 where that
+// neighbour is missing or is itself a coarse node, and the index is clamped).  Input dims (a0, a1, a2), output dims (b0, b1, b2), column
 // strides ldi / ldo.  Work-items run over output entries with b0 fastest, so every load and
 // store of a wavefront covers consecutive addresses (stride 2 at most along x).
 struct AxisXfer {
@@ -269,6 +387,7 @@ template <typename T, typename X, int AX, int R>
 __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const int* __restrict__ fi,
                                                              const T* __restrict__ wlo,
                                                              const T* __restrict__ whi,
+                                                             const T* __restrict__ wop,
                                                              const X* __restrict__ in,
                                                              X* __restrict__ out, size_t ldi,
                                                              size_t ldo, int ncol,
@@ -284,13 +403,19 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const i
   const uint32_t na = AX == 0 ? t.a0 : AX == 1 ? t.a1 : t.a2;
   const uint32_t stride = AX == 0 ? 1u : AX == 1 ? t.a0 : t.a0 * t.a1;
   const uint32_t f = (uint32_t)fi[C];
-  const T wl = wlo[C], wh = whi[C];
   const uint32_t fl = f > 0 ? f - 1 : 0, fh = f + 1 < na ? f + 1 : f;
   const uint32_t c0i = AX == 0 ? 0u : o0;
   const uint32_t c1i = AX == 1 ? 0u : o1;
   const uint32_t c2i = AX == 2 ? 0u : o2;
   const uint32_t base = c0i + t.a0 * (c1i + t.a1 * c2i);
   const uint32_t il = base + fl * stride, im = base + f * stride, ih = base + fh * stride;
+  // operator-dependent weights (wop, over this pass's input grid): the left neighbour gives
+  // its weight of the right coarse node, the right neighbour 1 minus its own
+  T wl = wlo[C], wh = whi[C];
+  if (wop) {
+    wl = wl != T(0) ? wop[il] : T(0);
+    wh = wh != T(0) ? T(1) - wop[ih] : T(0);
+  }
   T v[R][3];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -305,8 +430,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const i
       st_once(out + (size_t)(c0 + r) * ldo + n, (X)(wl * v[r][0] + v[r][1] + wh * v[r][2]));
 }
 
-// xf += P xc.  Every fine node reads the 2x2x2 coarse nodes around it; a coincident axis has
-// weight 0 on its second node, whose index still lies inside the coarse grid.
+// xf += P xc.  Every fine node reads the 2x2x2 coarse nodes around it (prolong_weights); a
+// coincident axis has weight 0 on its second node, whose index still lies inside the coarse
+// grid.
 template <typename T, typename X, int R>
 __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* __restrict__ xc,
                                                            X* __restrict__ xf, size_t ldf,
@@ -319,8 +445,10 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* _
   const uint32_t i = n - rr * t.fn1x;
   const uint32_t k = fdiv(rr, t.ffy);
   const uint32_t j = rr - k * t.fn1y;
-  const T ux = t.pwx[i], uy = t.pwy[j], uz = t.pwz[k];
-  const uint32_t c000 = (uint32_t)t.pix[i] + (uint32_t)t.piy[j] * t.cn1x + (uint32_t)t.piz[k] * t.cplane;
+  const uint32_t I0 = (uint32_t)t.pix[i], J0 = (uint32_t)t.piy[j];
+  T w[8];
+  prolong_weights(t, n, i, j, k, I0, J0, w);
+  const uint32_t c000 = I0 + J0 * t.cn1x + (uint32_t)t.piz[k] * t.cplane;
   const uint32_t c010 = c000 + t.cn1x, c001 = c000 + t.cplane, c011 = c001 + t.cn1x;
   T cv[R][8], fv[R];
 #pragma unroll
@@ -339,13 +467,10 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* _
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (c0 + r < ncol) {
-      const T a0 = cv[r][0] + ux * (cv[r][1] - cv[r][0]);
-      const T a1 = cv[r][2] + ux * (cv[r][3] - cv[r][2]);
-      const T a2 = cv[r][4] + ux * (cv[r][5] - cv[r][4]);
-      const T a3 = cv[r][6] + ux * (cv[r][7] - cv[r][6]);
-      const T b0 = a0 + uy * (a1 - a0);
-      const T b1 = a2 + uy * (a3 - a2);
-      xf[(size_t)(c0 + r) * ldf + n] = (X)(fv[r] + (b0 + uz * (b1 - b0)));
+      T acc = fv[r];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc += w[q] * cv[r][q];
+      xf[(size_t)(c0 + r) * ldf + n] = (X)acc;
     }
   }
 }

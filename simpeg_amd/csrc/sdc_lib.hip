@@ -89,11 +89,14 @@ struct XferDev {
   T *rlx = nullptr, *rly = nullptr, *rlz = nullptr, *rhx = nullptr, *rhy = nullptr, *rhz = nullptr;
   int *pix = nullptr, *piy = nullptr, *piz = nullptr;
   T *pwx = nullptr, *pwy = nullptr, *pwz = nullptr;
+  // operator-dependent interpolation weights (Xfer::wx), recomputed with the conductances
+  T *wx = nullptr, *wy = nullptr, *wz = nullptr;
   void release() {
     dfree(rix); dfree(riy); dfree(riz);
     dfree(rlx); dfree(rly); dfree(rlz); dfree(rhx); dfree(rhy); dfree(rhz);
     dfree(pix); dfree(piy); dfree(piz);
     dfree(pwx); dfree(pwy); dfree(pwz);
+    dfree(wx); dfree(wy); dfree(wz);
   }
 };
 
@@ -340,6 +343,9 @@ Xfer<T> make_xfer(const sdc_handle_s* h, int l) {
   t.ffx = make_fastdiv(f.n1x); t.ffy = make_fastdiv(f.n1y);
   t.cfx = make_fastdiv(c.n1x); t.cfy = make_fastdiv(c.n1y);
   t.remap = h->prm.xcd_remap != 0;
+  t.wx = h->prm.interp ? d.wx : nullptr;
+  t.wy = h->prm.interp ? d.wy : nullptr;
+  t.wz = h->prm.interp ? d.wz : nullptr;
   return t;
 }
 
@@ -429,6 +435,12 @@ int make_transfer(Level& f, Level& c, std::vector<int> (&fidx)[3]) {
   fill_level_dims(c);
   CK(upload_xfer(f.x64, ri, rl, rh, pi, pw));
   CK(upload_xfer(f.x32, ri, rl, rh, pi, pw));
+  // operator-dependent weights: Px over the fine grid, Py over (coarse x, fine y, fine z),
+  // Pz over (coarse x, coarse y, fine z)
+  const size_t nwx = f.nN, nwy = (size_t)c.n1x * f.n1y * f.n1z,
+               nwz = (size_t)c.n1x * c.n1y * f.n1z;
+  CK(dmalloc(&f.x64.wx, nwx)); CK(dmalloc(&f.x64.wy, nwy)); CK(dmalloc(&f.x64.wz, nwz));
+  CK(dmalloc(&f.x32.wx, nwx)); CK(dmalloc(&f.x32.wy, nwy)); CK(dmalloc(&f.x32.wz, nwz));
   f.has_coarse = true;
   return SDC_OK;
 }
@@ -496,16 +508,17 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   return SDC_OK;
 }
 
-// W-cycle level: the requested one, or (wcycle_level = -1) the first level with at most 1/16 of
-// the finest level's nodes -- small enough that a second visit costs little, fine enough that
-// its correction still carries most of the smooth error.  Never the coarsest level.
+// W-cycle level: the requested one, or (wcycle_level = -1) the first level with at most 1/4 of
+// the finest level's nodes (level 1 unless the first coarsening step only touched a small core
+// of a padded mesh) -- a second visit there costs ~15 % of a cycle and removes an iteration on
+// padded meshes with strong contrasts.  Never the coarsest level.
 int pick_wlevel(const sdc_handle_s* h) {
   const int nl = (int)h->L.size();
   int l = h->prm.wcycle_level;
   if (l < 0) {
     l = 0;
     for (int k = 1; k < nl - 1; ++k)
-      if ((size_t)h->L[k].nN * 16 <= (size_t)h->L[0].nN) {
+      if ((size_t)h->L[k].nN * 4 <= (size_t)h->L[0].nN) {
         l = k;
         break;
       }
@@ -780,20 +793,21 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
         const int* fi = a == 0 ? t.rix : a == 1 ? t.riy : t.riz;
         const T* wl = a == 0 ? t.rlx : a == 1 ? t.rly : t.rlz;
         const T* wh = a == 0 ? t.rhx : a == 1 ? t.rhy : t.rhz;
+        const T* wop = a == 0 ? t.wx : a == 1 ? t.wy : t.wz;
         const dim3 ga = grid2(ax.nout, ncol, RX);
         prof_begin(h, rtag, st);
         switch (a) {
           case 0:
             DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 0, R>), ga, blk, 0, st, ax, fi,
-                                              wl, wh, src, dst, lv.ld, ldo, ncol, done));
+                                              wl, wh, wop, src, dst, lv.ld, ldo, ncol, done));
             break;
           case 1:
             DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 1, R>), ga, blk, 0, st, ax, fi,
-                                              wl, wh, src, dst, lv.ld, ldo, ncol, done));
+                                              wl, wh, wop, src, dst, lv.ld, ldo, ncol, done));
             break;
           default:
             DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 2, R>), ga, blk, 0, st, ax, fi,
-                                              wl, wh, src, dst, lv.ld, ldo, ncol, done));
+                                              wl, wh, wop, src, dst, lv.ld, ldo, ncol, done));
             break;
         }
         prof_end(h, rtag, st, ((double)ax.a0 * ax.a1 * ax.a2 + (double)ax.nout) * cols * sS);
@@ -1132,6 +1146,7 @@ int sdc_default_params(sdc_params* p) {
   p->fused_tail = 1;
   p->refine_rhs_per_thread = 0;
   p->graph = 1;
+  p->interp = 1;
   p->check_every = 1;
   p->reliable_delta = 1e-4;
   return SDC_OK;
@@ -1288,6 +1303,7 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
   if (!(p->reliable_delta > 0) || !(p->reliable_delta < 1))
     return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
   if (p->graph != 0 && p->graph != 1) return fail(SDC_ERR_ARG, "graph must be 0 or 1");
+  if (p->interp != 0 && p->interp != 1) return fail(SDC_ERR_ARG, "interp must be 0 or 1");
   DeviceGuard dg(h->device);
   drop_graphs(h);
   h->prm = *p;
@@ -1336,6 +1352,20 @@ int build_levels(sdc_handle_s* h, hipStream_t st) {
     hipLaunchKernelGGL(k_coarsen_coef, dim3(nblk(c.nN)), dim3(SDC_BLOCK), 0, st, t,
                        (const double*)f.cx64, (const double*)f.cy64, (const double*)f.cz64,
                        (const double*)f.bd64, c.cx64, c.cy64, c.cz64, c.bd64, c.n1z);
+  }
+  // operator-dependent interpolation weights of every transfer (used when sdc_params.interp = 1)
+  for (size_t l = 0; l + 1 < h->L.size(); ++l) {
+    Level& f = h->L[l];
+    const Level& c = h->L[l + 1];
+    const Xfer<double> t = make_xfer<double>(h, (int)l);
+    const size_t nwx = f.nN, nwy = (size_t)c.n1x * f.n1y * f.n1z,
+                 nwz = (size_t)c.n1x * c.n1y * f.n1z;
+    hipLaunchKernelGGL(k_interp_weights<0>, dim3(nblk(nwx)), dim3(SDC_BLOCK), 0, st, t, f.n1x,
+                       f.n1y, f.n1z, (const double*)f.cx64, f.x64.wx, f.x32.wx);
+    hipLaunchKernelGGL(k_interp_weights<1>, dim3(nblk(nwy)), dim3(SDC_BLOCK), 0, st, t, c.n1x,
+                       f.n1y, f.n1z, (const double*)f.cy64, f.x64.wy, f.x32.wy);
+    hipLaunchKernelGGL(k_interp_weights<2>, dim3(nblk(nwz)), dim3(SDC_BLOCK), 0, st, t, c.n1x,
+                       c.n1y, f.n1z, (const double*)f.cz64, f.x64.wz, f.x32.wz);
   }
   const int pin = h->bc == SDC_BC_NEUMANN ? 1 : 0;
   for (auto& v : h->L) {

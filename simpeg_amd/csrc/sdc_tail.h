@@ -1,12 +1,13 @@
 // The coarse end of the V-cycle in one launch.
 //
-// Levels l_t .. L-1 of the hierarchy are small (cfg 2: 7,406 + 1,352 + 245 + 48 nodes), so the
-// separate launches for their smoothing steps, residuals and transfers are bound by launch
-// latency, not by memory.  Here one work-group owns one right-hand-side column and runs the
-// whole V-cycle below level l_t - 1 out of LDS (gfx950: up to 160 KB per work-group): the
-// column's rhs of level l_t is read once from HBM, the solution written once.  The arithmetic is
-// the multi-launch V-cycle's: Chebyshev steps on the same interval (three-term form), P^T as the
-// 27-point gather, trilinear P, and the dense inverse on the coarsest level.  Coefficients and
+// Levels l_t .. L-1 of the hierarchy are small (7,406 + 1,352 + 245 + 48 nodes below a
+// 932,137-node mesh), so the separate launches for their smoothing steps, residuals and
+// transfers are bound by launch latency, not by memory.  Here one work-group owns one
+// right-hand-side column and runs the whole V-cycle below level l_t - 1 out of LDS (gfx950: up
+// to 160 KB per work-group): the column's rhs of level l_t is read once from HBM, the solution
+// written once.  The arithmetic is the multi-launch V-cycle's: Chebyshev steps on the same
+// interval (three-term form), P^T as the 27-point gather, the same P (geometric or
+// operator-dependent weights), and the dense inverse on the coarsest level.  Coefficients and
 // transfer tables are read from global memory (they are shared by all columns and stay in L2).
 #pragma once
 #include "sdc_kernels.h"
@@ -95,42 +96,37 @@ __device__ void restrict_to(const Xfer<float>& t, const float* r, float* bc) {
     const uint32_t K = fdiv(rr, t.cfy);
     const uint32_t J = rr - K * t.cn1y;
     const int fi = t.rix[I], fj = t.riy[J], fk = t.riz[K];
-    const float wx[3] = {t.rlx[I], 1.0f, t.rhx[I]};
-    const float wy[3] = {t.rly[J], 1.0f, t.rhy[J]};
-    const float wz[3] = {t.rlz[K], 1.0f, t.rhz[K]};
     const int xi[3] = {fi > 0 ? fi - 1 : 0, fi, fi + 1 < (int)t.fn1x ? fi + 1 : fi};
     const int yj[3] = {fj > 0 ? fj - 1 : 0, fj, fj + 1 < (int)t.fn1y ? fj + 1 : fj};
     const int zk[3] = {fk > 0 ? fk - 1 : 0, fk, fk + 1 < fn1z ? fk + 1 : fk};
+    float w[27];
+    restrict_weights(t, I, J, K, xi, yj, zk, w);
     float acc = 0.0f;
     for (int c = 0; c < 3; ++c)
       for (int bb = 0; bb < 3; ++bb) {
         const uint32_t row = (uint32_t)yj[bb] * t.fn1x + (uint32_t)zk[c] * t.fplane;
-        acc += wy[bb] * wz[c] *
-               (wx[0] * r[row + xi[0]] + r[row + xi[1]] + wx[2] * r[row + xi[2]]);
+        const float* wb = w + 3 * bb + 9 * c;
+        acc += wb[0] * r[row + xi[0]] + wb[1] * r[row + xi[1]] + wb[2] * r[row + xi[2]];
       }
     bc[n] = acc;
   }
   __syncthreads();
 }
 
-// xf += P xc (trilinear)
+// xf += P xc (prolong_weights)
 __device__ void prolong_add(const Xfer<float>& t, const float* xc, float* xf) {
   for (uint32_t n = threadIdx.x; n < t.fN; n += blockDim.x) {
     const uint32_t rr = fdiv(n, t.ffx);
     const uint32_t i = n - rr * t.fn1x;
     const uint32_t k = fdiv(rr, t.ffy);
     const uint32_t j = rr - k * t.fn1y;
-    const float ux = t.pwx[i], uy = t.pwy[j], uz = t.pwz[k];
-    const uint32_t c000 = (uint32_t)t.pix[i] + (uint32_t)t.piy[j] * t.cn1x +
-                          (uint32_t)t.piz[k] * t.cplane;
+    const uint32_t I0 = (uint32_t)t.pix[i], J0 = (uint32_t)t.piy[j];
+    float w[8];
+    prolong_weights(t, n, i, j, k, I0, J0, w);
+    const uint32_t c000 = I0 + J0 * t.cn1x + (uint32_t)t.piz[k] * t.cplane;
     const uint32_t c010 = c000 + t.cn1x, c001 = c000 + t.cplane, c011 = c001 + t.cn1x;
-    const float a0 = xc[c000] + ux * (xc[c000 + 1] - xc[c000]);
-    const float a1 = xc[c010] + ux * (xc[c010 + 1] - xc[c010]);
-    const float a2 = xc[c001] + ux * (xc[c001 + 1] - xc[c001]);
-    const float a3 = xc[c011] + ux * (xc[c011 + 1] - xc[c011]);
-    const float b0 = a0 + uy * (a1 - a0);
-    const float b1 = a2 + uy * (a3 - a2);
-    xf[n] += b0 + uz * (b1 - b0);
+    xf[n] += w[0] * xc[c000] + w[1] * xc[c000 + 1] + w[2] * xc[c010] + w[3] * xc[c010 + 1] +
+             w[4] * xc[c001] + w[5] * xc[c001 + 1] + w[6] * xc[c011] + w[7] * xc[c011 + 1];
   }
   __syncthreads();
 }

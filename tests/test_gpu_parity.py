@@ -45,15 +45,18 @@ def test_apply_matches_assembled_operator(gpu, tiny, bc):
     assert rel(y32, A @ x) < 2e-6
 
 
+@pytest.mark.parametrize("interp", [0, 1])
 @pytest.mark.parametrize("wlevel", [0, 1])
-def test_vcycle_is_symmetric(gpu, tiny, wlevel):
+def test_vcycle_is_symmetric(gpu, tiny, wlevel, interp):
+    """The fp64 V-cycle (W-cycle on level 1) is a symmetric positive definite operator, with
+    geometric and with operator-dependent interpolation (restriction = P^T)."""
     mesh, survey, m = tiny
     sim = _sim(mesh, survey)
     sim.model = m
     sim._ensure_sigma()
     eng = sim.engine
     assert eng.levels >= 3
-    eng.set_params(wcycle_level=wlevel)
+    eng.set_params(wcycle_level=wlevel, interp=interp)
     rng = np.random.default_rng(2)
     a = rng.standard_normal((mesh.n_nodes, 2))
     b = rng.standard_normal((mesh.n_nodes, 2))
@@ -109,8 +112,9 @@ def test_mixed_solve_with_each_vcycle_storage(gpu, tiny, storage):
         assert rel(f[:, "phiSolution"][:, s], u_ref[:, s]) < 1e-7
 
 
+@pytest.mark.parametrize("interp", [0, 1])
 @pytest.mark.parametrize("bc", ["Robin", "Neumann"])
-def test_fused_coarse_tail_matches_multi_launch_cycle(gpu, tiny, bc):
+def test_fused_coarse_tail_matches_multi_launch_cycle(gpu, tiny, bc, interp):
     """The coarse levels run as one LDS-resident launch give the V-cycle of the separate
     launches (same smoothing, transfers and coarse inverse; fp32 in LDS instead of bf16 vectors),
     and the fused cycle stays symmetric to that accuracy."""
@@ -125,10 +129,10 @@ def test_fused_coarse_tail_matches_multi_launch_cycle(gpu, tiny, bc):
     b = rng.standard_normal((mesh.n_nodes, 4))
     out = {}
     for flag in (0, 1):
-        eng.set_params(fused_tail=flag, wcycle_level=0)
+        eng.set_params(fused_tail=flag, wcycle_level=0, interp=interp)
         out[flag] = eng.from_block(eng.vcycle(eng.to_block(a, torch.float32)).double())
     mb = eng.from_block(eng.vcycle(eng.to_block(b, torch.float32)).double())
-    eng.set_params(fused_tail=1, wcycle_level=-1)
+    eng.set_params(fused_tail=1, wcycle_level=-1, interp=1)
     assert 1e-7 < rel(out[1], out[0]) < 3e-2  # the fused path ran, and agrees
     lhs = np.sum(out[1] * b, axis=0)
     rhs = np.sum(a * mb, axis=0)
@@ -411,3 +415,43 @@ def test_ragged_batches_and_zero_columns(gpu, tiny, precision):
             Ainv * np.ones(mesh.n_nodes + 1)
     finally:
         Ainv.clean()
+
+
+def test_operator_dependent_interpolation_is_geometric_in_a_homogeneous_medium(gpu, tiny):
+    """With constant sigma the two edges of a node along a grid line have conductances in the
+    ratio of the inverse cell widths on every level, so the operator-dependent weights equal the
+    geometric ones (the padded mesh has unequal widths, which pins the along-line indexing): the
+    V-cycles agree to rounding, fp64 multi-launch and fp32 fused tail alike."""
+    mesh, survey, _ = tiny
+    sim = _sim(mesh, survey)
+    sim.model = np.full(mesh.n_cells, np.log(0.01))
+    sim._ensure_sigma()
+    eng = sim.engine
+    a = np.random.default_rng(8).standard_normal((mesh.n_nodes, 3))
+    out64, out32 = {}, {}
+    for interp in (0, 1):
+        eng.set_params(interp=interp)
+        out64[interp] = eng.from_block(eng.vcycle(eng.to_block(a)))
+        out32[interp] = eng.from_block(eng.vcycle(eng.to_block(a, torch.float32)).double())
+    assert rel(out64[1], out64[0]) < 1e-12
+    assert rel(out32[1], out32[0]) < 1e-5
+
+
+def test_operator_dependent_interpolation_on_a_high_contrast_model(gpu):
+    """Layers of 0.005 / 0.02 / 0.002 S/m with conductive ellipsoids (0.2-0.5 S/m): the
+    operator-dependent interpolation needs no more PCG iterations than the geometric one, and
+    both reach the direct solver's potentials."""
+    from simpeg_amd import workloads as W
+
+    wl = W.layered_ellipsoid_model(0, core=(24, 24, 16), npad=4, nlines=1, nsrc_per_line=2,
+                                   nrx=4)
+    orc = oracle_for(wl.mesh, wl.survey)
+    u_ref = orc.fields(wl.model)
+    its = {}
+    for interp in (0, 1):
+        sim = _sim(wl.mesh, wl.survey, rtol=1e-8, gpu_params={"interp": interp})
+        u = sim.fields(wl.model)[:, "phiSolution"]
+        its[interp] = sim.last_solve.iters
+        for s in range(wl.survey.nSrc):
+            assert rel(u[:, s], u_ref[:, s]) < 1e-7
+    assert its[1] <= its[0]
