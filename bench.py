@@ -134,14 +134,28 @@ FP64_BASELINE_CANDIDATES = [
 ]
 
 # library profiler class -> kernel (finest level, fp32 solver with bf16 multigrid vectors)
-KERNEL_NAMES = {
-    "update": "k_update_r4<false>", "smooth": "k_cheb<float,float,bf16,float,8,1,true>",
-    "apply": "k_apply_pq<float,8,false>", "pair": "k_smooth_pair<float,float,bf16,8>",
-    "residual": "k_residual<float,float,bf16,8>", "smooth0": "k_cheb<float,float,bf16,bf16,8,0,false>",
-    "prolong": "k_prolong_add<float,bf16,8>", "restrict": "k_restrict_axis<float,bf16,*,8>",
-    "update_p": "k_update_p4<float,false>", "refine": "k_ir_residual<float,8,false>",
-    "fold": "k_ir_fold_p4", "jtvec": "k_jtvec<0>",
-}
+def kernel_names(params, ncol):
+    """Template instance of each profiled kernel class for the engine's parameters (the library
+    picks 4 columns per work-item for stencils, 8 for transfers, unless set; vcycle_bf16 = 1
+    keeps the preconditioned residual z in bf16)."""
+    r = params.rhs_per_thread if params.rhs_per_thread in (1, 2, 4, 8) else \
+        (4 if ncol >= 4 else 2 if ncol >= 2 else 1)
+    rx = params.xfer_rhs_per_thread if params.xfer_rhs_per_thread in (1, 2, 4, 8) else \
+        (8 if ncol >= 8 else r)
+    rr = min(r, params.refine_rhs_per_thread or 8)
+    z = "bf16" if params.vcycle_bf16 == 1 else "float"
+    b = "bf16" if params.vcycle_bf16 == 3 else "float"
+    v = "float" if params.vcycle_bf16 == 0 else "bf16"
+    return {
+        "update": f"k_update_r4<{'true' if params.vcycle_bf16 == 3 else 'false'}>",
+        "smooth": f"k_cheb<float,{b},{v},{z},{r},1,true>",
+        "apply": f"k_apply_pq<{z},{r},false>", "pair": f"k_smooth_pair<float,{b},{v},{r}>",
+        "residual": f"k_residual<float,{b},{v},{r}>",
+        "smooth0": f"k_cheb<float,{b},{v},{v},{r},0,false>",
+        "prolong": f"k_prolong_add<float,{v},{rx}>", "restrict": f"k_restrict_axis<float,{v},*,{rx}>",
+        "update_p": "k_update_p4<float,false>", "refine": f"k_ir_residual<float,{rr},false>",
+        "fold": "k_ir_fold_p4", "jtvec": "k_jtvec<0>",
+    }
 
 
 def measured_traffic(tag, workload):
@@ -430,7 +444,7 @@ def main():
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if nlaunch else 0.0
     traffic = measured_traffic(args.profile_kernel, wl.name)
     out["roofline"] = {
-        "bound": "hbm", "kernel": KERNEL_NAMES[args.profile_kernel], "launches": nlaunch,
+        "bound": "hbm", "kernel": kernel_names(eng.params, nS)[args.profile_kernel], "launches": nlaunch,
         "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": avg_bytes,
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -77,11 +77,12 @@ typedef struct {
   int32_t nu_post;             /* Chebyshev post-smoothing steps (>= 1) */
   double cheb_lmax;            /* upper bound of the spectrum of D^-1 A (Gershgorin: 2) */
   double cheb_ratio;           /* lower bound = cheb_lmax / cheb_ratio */
-  int32_t rhs_per_thread;      /* 0 = automatic, else 1, 2, 4 or 8 columns per work-item */
+  int32_t rhs_per_thread;      /* stencil kernels: 0 = automatic (4), else 1, 2, 4 or 8 columns
+                                  per work-item */
   int32_t check_every;         /* PCG iterations between host convergence polls */
   double reliable_delta;       /* mixed mode: replace the fp32 residual by the fp64 true residual
                                   once it dropped by this factor (0 < delta < 1) */
-  int32_t xfer_rhs_per_thread; /* grid transfers: 0 = as rhs_per_thread, else 1, 2, 4 or 8 */
+  int32_t xfer_rhs_per_thread; /* grid transfers: 0 = automatic (8), else 1, 2, 4 or 8 */
   int32_t xcd_remap;           /* 1: give each XCD a contiguous range of work-groups, 0: off */
   int32_t vcycle_bf16;         /* 1: the fp32 solver keeps its multigrid vectors, including the
                                   preconditioned residual z, in bfloat16 (arithmetic stays fp32);

@@ -18,16 +18,20 @@ import torch
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "simpeg_amd", "autotune.json")
 
+# Every candidate is a symmetric cycle (nu_pre == nu_post), as PCG needs a symmetric
+# preconditioner.  Fields not named keep the engine's current values.
+_BASE = dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, xfer_rhs_per_thread=0,
+             wcycle_level=-1, interp=1, vcycle_bf16=1, check_every=1)
 CANDIDATES = [
-    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=-1),
-    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=0),
-    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=2),
-    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=-1, interp=0),
-    dict(nu_pre=2, nu_post=2, cheb_ratio=4.0, rhs_per_thread=0, wcycle_level=-1),
-    dict(nu_pre=2, nu_post=3, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=-1),
-    dict(nu_pre=3, nu_post=3, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=0),
-    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, wcycle_level=-1, check_every=2),
-    dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=4, wcycle_level=-1),
+    dict(_BASE),
+    dict(_BASE, wcycle_level=0),
+    dict(_BASE, wcycle_level=2),
+    dict(_BASE, interp=0),
+    dict(_BASE, cheb_ratio=4.0),
+    dict(_BASE, nu_pre=3, nu_post=3),
+    dict(_BASE, check_every=2),
+    dict(_BASE, rhs_per_thread=8, xfer_rhs_per_thread=8),
+    dict(_BASE, vcycle_bf16=2),
 ]
 
 
