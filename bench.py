@@ -142,7 +142,7 @@ def kernel_names(params, ncol):
         (4 if ncol >= 4 else 2 if ncol >= 2 else 1)
     rx = params.xfer_rhs_per_thread if params.xfer_rhs_per_thread in (1, 2, 4, 8) else \
         (8 if ncol >= 8 else r)
-    rr = min(r, params.refine_rhs_per_thread or 8)
+    rr = params.refine_rhs_per_thread or (8 if ncol >= 8 else r)
     z = "bf16" if params.vcycle_bf16 == 1 else "float"
     b = "bf16" if params.vcycle_bf16 == 3 else "float"
     v = "float" if params.vcycle_bf16 == 0 else "bf16"

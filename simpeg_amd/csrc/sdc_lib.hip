@@ -1086,9 +1086,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
         hipLaunchKernelGGL(k_zero<double>, dim3(nblk(ncol)), blk, 0, st, W.alpha, (size_t)ncol);
         xfresh = true;
         prof_begin(h, SDC_PROF_REFINE, st);
-        // fp64 stencil: refine_rhs_per_thread columns per work-item (default 8)
-        const int R64 =
-            std::min(R, h->prm.refine_rhs_per_thread > 0 ? h->prm.refine_rhs_per_thread : 8);
+        // fp64 stencil: refine_rhs_per_thread columns per work-item (default 8; its coefficients
+        // are fp64, so more columns per work-item save more traffic than in the fp32 kernels)
+        const int R64 = h->prm.refine_rhs_per_thread > 0 ? h->prm.refine_rhs_per_thread
+                        : ncol >= 8                      ? 8
+                                                         : R;
         DISPATCH_R(R64, hipLaunchKernelGGL((k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64),
                                            blk, 0, st, s2, q, (const double*)u, W.r, l0.ld, ncol,
                                            pf, W.partial, (unsigned*)W.qmask, r16));
