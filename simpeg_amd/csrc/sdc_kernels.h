@@ -664,6 +664,10 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
 // u += x + alpha p (fold the fp32 partial solution, including the pending x update of the last
 // step, into the fp64 solution).  x is not cleared: the next k_apply_pq treats it as zero.
 // xfresh: x counts as zero (not read).
+// UFRESH: u holds nothing yet (first fold of a solve): u = x + alpha p without reading u, and the
+// column padding [nN, ld) is zeroed, so the solve needs no separate clearing pass (the grid then
+// covers ld entries per column instead of nN).
+template <bool UFRESH>
 __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_p4(double* __restrict__ u,
                                                           const float* __restrict__ x,
                                                           const float* __restrict__ p,
@@ -677,15 +681,20 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_p4(double* __restrict__ u
     const vec4_t<float> pv = *(const vec4_t<float>*)(p + o + n0);
     const vec4_t<float> xv =
         (xfresh ? vec4_t<float>(0.0f) : *(const vec4_t<float>*)(x + o + n0)) + al * pv;
-    vec4_t<double> uv = *(const vec4_t<double>*)(u + o + n0);
+    vec4_t<double> uv = UFRESH ? vec4_t<double>(0.0) : *(const vec4_t<double>*)(u + o + n0);
     uv.x += (double)xv.x;
     uv.y += (double)xv.y;
     uv.z += (double)xv.z;
     uv.w += (double)xv.w;
     *(vec4_t<double>*)(u + o + n0) = uv;
   } else {
-    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n)
-      u[o + n] += (double)((xfresh ? 0.0f : x[o + n]) + al * p[o + n]);
+    const uint32_t end = UFRESH ? (uint32_t)ld : nN;
+    for (uint32_t n = n0; n < end && n < n0 + 4; ++n) {
+      if (n < nN)
+        u[o + n] = (UFRESH ? 0.0 : u[o + n]) + (double)((xfresh ? 0.0f : x[o + n]) + al * p[o + n]);
+      else
+        u[o + n] = 0.0;
+    }
   }
 }
 

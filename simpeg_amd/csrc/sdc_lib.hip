@@ -976,7 +976,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const Sten<float> s1 = make_sten<float>(h, 0, 1);
   const double tol2 = o->tol * o->tol;
   const double delta2 = h->prm.reliable_delta * h->prm.reliable_delta;
-  hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
+  // u is not cleared: the first fold writes it (and its column padding) without reading it
   hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
   // vcycle_bf16 = 3: the multigrid reads a bfloat16 copy of r, written with r
   __bf16* const r16 = (h->prm.vcycle_bf16 == 3 && h->L.size() > 1) ? W.r16 : nullptr;
@@ -1001,7 +1001,10 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   S.iters = 0;
   S.max_rel_res = 0.0;
   S.converged = nact == 0;
-  if (nact == 0) return SDC_OK;
+  if (nact == 0) {
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
+    return SDC_OK;
+  }
   // the finalisers test the convergence flag counters[3]; a previous solve may have left it set
   hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)4);
   // Iteration: z = M r (z kept in the V-cycle storage type), beta, p = z + beta p and
@@ -1014,6 +1017,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const int maxit = o->max_iter;
   const double bz = zb ? 2.0 : 4.0;
   bool xfresh = true;  // x holds nothing that is not yet in u (its memory is not read)
+  bool ufresh = true;  // u has not been written yet (the first fold sets it)
   int it;
   for (it = 1; it <= maxit; ++it) {
     const bool first = it == 1;
@@ -1079,10 +1083,18 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       }
       if (cand || drop || it == maxit) {
         prof_begin(h, SDC_PROF_FOLD, st);
-        hipLaunchKernelGGL(k_ir_fold_p4, grid4(l0.nN, ncol), blk, 0, st, u, (const float*)W.x,
-                           (const float*)W.p, (const double*)W.alpha, l0.ld, l0.nN,
-                           xfresh ? 1 : 0);
-        prof_end(h, SDC_PROF_FOLD, st, (double)l0.nN * ncol * 24);
+        if (ufresh) {
+          hipLaunchKernelGGL(k_ir_fold_p4<true>, grid4((uint32_t)l0.ld, ncol), blk, 0, st, u,
+                             (const float*)W.x, (const float*)W.p, (const double*)W.alpha, l0.ld,
+                             l0.nN, xfresh ? 1 : 0);
+        } else {
+          hipLaunchKernelGGL(k_ir_fold_p4<false>, grid4(l0.nN, ncol), blk, 0, st, u,
+                             (const float*)W.x, (const float*)W.p, (const double*)W.alpha, l0.ld,
+                             l0.nN, xfresh ? 1 : 0);
+        }
+        const double fold_bytes = (double)l0.nN * ncol * (ufresh ? 16 : 24);
+        ufresh = false;
+        prof_end(h, SDC_PROF_FOLD, st, fold_bytes);
         hipLaunchKernelGGL(k_zero<double>, dim3(nblk(ncol)), blk, 0, st, W.alpha, (size_t)ncol);
         xfresh = true;
         prof_begin(h, SDC_PROF_REFINE, st);
