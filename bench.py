@@ -128,12 +128,13 @@ class Pipeline:
 
 # settings tried for the plain fp64 per-source baseline (the fastest is reported)
 FP64_BASELINE_CANDIDATES = [
-    {"wcycle_level": 0, "check_every": 1},
-    {"wcycle_level": 0, "check_every": 2},
-    {"wcycle_level": -1, "check_every": 1},
+    {"wcycle_level": 0, "check_every": 1, "graph": 1},
+    {"wcycle_level": 0, "check_every": 2, "graph": 1},
+    {"wcycle_level": 0, "check_every": 4, "graph": 1},
+    {"wcycle_level": -1, "check_every": 1, "graph": 1},
+    {"wcycle_level": 0, "check_every": 1, "graph": 0},
 ]
 
-# library profiler class -> kernel (finest level, fp32 solver with bf16 multigrid vectors)
 def kernel_names(params, ncol):
     """Template instance of each profiled kernel class for the engine's parameters (the library
     picks 4 columns per work-item for stencils, 8 for transfers, unless set; vcycle_bf16 = 1
@@ -469,7 +470,8 @@ def main():
 
     if not args.no_fp64:
         # The baseline runs one source at a time, where kernel launches dominate: it gets the
-        # multigrid cycle and poll cadence that suit it best (the fastest of a few).
+        # multigrid cycle, poll cadence and V-cycle graph setting that suit it best (the fastest
+        # of a few).
         base = {f: getattr(eng.params, f) for f, _ in eng.params._fields_}
         best = None
         for cand in FP64_BASELINE_CANDIDATES:

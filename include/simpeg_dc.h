@@ -97,10 +97,10 @@ typedef struct {
                                   LDS as one launch (one work-group per column); 0: off */
   int32_t refine_rhs_per_thread; /* fp64 true-residual kernel: 0 = 8, else 1, 2, 4 or 8 columns
                                     per work-item */
-  int32_t graph;               /* 1: the mixed-precision solver captures its finest-level V-cycle
-                                  once per (workspace, column count) as a HIP graph and replays
-                                  it every iteration (not while sdc_profile brackets one of its
-                                  kernel classes); 0: launches */
+  int32_t graph;               /* 1: the solvers record their finest-level V-cycle once per
+                                  (workspace, column count) as a HIP graph and replay it every
+                                  iteration (not while sdc_profile brackets one of its kernel
+                                  classes); 0: launches */
   int32_t interp;              /* multigrid interpolation: 1 = operator-dependent (a node between
                                   two coarse nodes weights them by the conductances of its two
                                   edges on its grid line), 0 = geometric (linear in position) */

@@ -848,6 +848,41 @@ int vcycle0(sdc_handle_s* h, const T* b, T* x, int ncol, bool dot_out, const int
   return vcycle<T, T, T, T>(h, 0, b, x, ncol, dot_out, done, st);
 }
 
+// Runs fn(stream) -- V-cycle launches only, no host synchronisation -- on st, through an executable
+// graph recorded on the first call with the same (b, x, ncol, variant) when sdc_params.graph is
+// set and none of the V-cycle's kernel classes is being profiled.
+template <typename F>
+int graph_run(sdc_handle_s* h, const void* b, void* x, int ncol, int variant, hipStream_t st,
+              F&& fn) {
+  // events bracketing the V-cycle's own kernel classes cannot sit inside the graph
+  constexpr uint32_t vcycle_tags = (1u << SDC_PROF_SMOOTH) | (1u << SDC_PROF_SMOOTH0) |
+                                   (1u << SDC_PROF_PAIR) | (1u << SDC_PROF_RESIDUAL) |
+                                   (1u << SDC_PROF_RESTRICT) | (1u << SDC_PROF_PROLONG);
+  if (!h->prm.graph || (h->prof_mask & vcycle_tags)) return fn(st);
+  for (const auto& g : h->graphs)
+    if (g.b == b && g.x == x && g.ncol == ncol && g.variant == variant) {
+      HIPCK(hipGraphLaunch(g.exec, st));
+      return SDC_OK;
+    }
+  sdc_handle_s::Recorder rec;
+  HIPCK(hipGraphCreate(&rec.g, 0));
+  h->rec = &rec;
+  const int rc = fn(st);
+  h->rec = nullptr;
+  if (rc != SDC_OK || rec.err != hipSuccess) {
+    (void)hipGraphDestroy(rec.g);
+    if (rc != SDC_OK) return rc;
+    HIPCK(rec.err);
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, rec.g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(rec.g);
+  HIPCK(ei);
+  h->graphs.push_back({b, x, ncol, variant, exec});
+  HIPCK(hipGraphLaunch(exec, st));
+  return SDC_OK;
+}
+
 // Preconditioned CG on ncol columns: rhs in W.r (overwritten by the residual), solution in x.
 // Each column stops when |r| <= tol |b|.  col_mask (device, may be null) excludes columns.
 template <typename T>
@@ -869,7 +904,9 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
                      W.active, 1);
   if (col_mask)
     hipLaunchKernelGGL(k_and_active, dim3(nblk(ncol)), blk, 0, st, W.active, col_mask, ncol);
-  CK(vcycle0<T>(h, W.r, W.z, ncol, true, done, st));
+  CK(graph_run(h, W.r, W.z, ncol, 8 + (int)sizeof(T), st, [&](hipStream_t gs) {
+    return vcycle0<T>(h, W.r, W.z, ncol, true, done, gs);
+  }));
   hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                      W.beta, (const int*)W.active, 1, (const int*)done);
   hipLaunchKernelGGL((k_update_p4<T, true>), grid4(l0.nN, ncol), blk, 0, st, (const T*)W.z, W.p,
@@ -903,7 +940,9 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
         break;
       }
     }
-    CK(vcycle0<T>(h, W.r, W.z, ncol, true, done, st));
+    CK(graph_run(h, W.r, W.z, ncol, 8 + (int)sizeof(T), st, [&](hipStream_t gs) {
+      return vcycle0<T>(h, W.r, W.z, ncol, true, done, gs);
+    }));
     hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
                        W.beta, (const int*)W.active, 0, (const int*)done);
     prof_begin(h, SDC_PROF_UPDATE_P, st);
@@ -913,41 +952,6 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   }
   HIPCK(hipGetLastError());
   if (iters) *iters = std::min(it, maxit);
-  return SDC_OK;
-}
-
-// Runs fn(stream) -- V-cycle launches only, no host synchronisation -- on st, through an executable
-// graph recorded on the first call with the same (b, x, ncol, variant) when sdc_params.graph is
-// set and none of the V-cycle's kernel classes is being profiled.
-template <typename F>
-int graph_run(sdc_handle_s* h, const void* b, void* x, int ncol, int variant, hipStream_t st,
-              F&& fn) {
-  // events bracketing the V-cycle's own kernel classes cannot sit inside the graph
-  constexpr uint32_t vcycle_tags = (1u << SDC_PROF_SMOOTH) | (1u << SDC_PROF_SMOOTH0) |
-                                   (1u << SDC_PROF_PAIR) | (1u << SDC_PROF_RESIDUAL) |
-                                   (1u << SDC_PROF_RESTRICT) | (1u << SDC_PROF_PROLONG);
-  if (!h->prm.graph || (h->prof_mask & vcycle_tags)) return fn(st);
-  for (const auto& g : h->graphs)
-    if (g.b == b && g.x == x && g.ncol == ncol && g.variant == variant) {
-      HIPCK(hipGraphLaunch(g.exec, st));
-      return SDC_OK;
-    }
-  sdc_handle_s::Recorder rec;
-  HIPCK(hipGraphCreate(&rec.g, 0));
-  h->rec = &rec;
-  const int rc = fn(st);
-  h->rec = nullptr;
-  if (rc != SDC_OK || rec.err != hipSuccess) {
-    (void)hipGraphDestroy(rec.g);
-    if (rc != SDC_OK) return rc;
-    HIPCK(rec.err);
-  }
-  hipGraphExec_t exec = nullptr;
-  const hipError_t ei = hipGraphInstantiate(&exec, rec.g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(rec.g);
-  HIPCK(ei);
-  h->graphs.push_back({b, x, ncol, variant, exec});
-  HIPCK(hipGraphLaunch(exec, st));
   return SDC_OK;
 }
 
