@@ -44,10 +44,10 @@ class Fields3DNodal:
     def _cols(self, key):
         srcs = self.survey.source_list
         if isinstance(key, slice):
-            return list(range(len(srcs)))[key], False
+            return list(range(len(srcs)))[key]
         if isinstance(key, (list, tuple)):
-            return [self._src_index(s) for s in key], False
-        return [self._src_index(key)], True
+            return [self._src_index(s) for s in key]
+        return [self._src_index(key)]
 
     def _src_index(self, src):
         for i, s in enumerate(self.survey.source_list):
@@ -59,7 +59,7 @@ class Fields3DNodal:
         if not isinstance(key, tuple) or len(key) != 2:
             raise KeyError("index with f[source, field_name]")
         src, name = key
-        cols, single = self._cols(src)
+        cols = self._cols(src)
         if name not in self.knownFields and name not in self.aliasFields:
             raise KeyError(f"Field {name!r} is not known")
         phi = self.phi_host[:, cols]
@@ -78,15 +78,15 @@ class Fields3DNodal:
             else:  # charge_density
                 ch = -EPSILON_0 * (G.T @ (sp.diags(sim._edge_mass(None)) @ e))
                 out = (self.mesh.average_node_to_cell @ ch) / self.mesh.cell_volumes[:, None]
-        if single and out.shape[1] == 1:
-            return out[:, 0].copy()
+        # one column per requested source, also for a single source: the reference indexes
+        # with a list of source indices (simpeg/fields.py:322-341), giving (n, 1)
         return out
 
     def __setitem__(self, key, value):
         src, name = key
         if name != "phiSolution":
             raise KeyError("only phiSolution can be assigned")
-        cols, _ = self._cols(src)
+        cols = self._cols(src)
         host = self.phi_host.copy()
         value = np.asarray(value, dtype=float)
         host[:, cols] = value.reshape(host.shape[0], -1)

@@ -243,9 +243,14 @@ class Simulation3DNodal:
         if self._sigma_on_gpu is None or self.sigmaMap is None and self.rhoMap is None:
             sig = self.sigma
             if sig.size != self._nC:
+                # the reference raises this for Robin (resistivity/simulation.py:546-551); its
+                # Neumann path accepts tensors, which the GPU stencil does not implement
+                if self._bc_type != "Neumann":
+                    raise NotImplementedError(
+                        "Anisotropic conductivity is not supported for Robin boundary "
+                        "conditions, please use 'Neumann'.")
                 raise NotImplementedError(
-                    "Anisotropic conductivity is not supported for Robin boundary "
-                    "conditions, please use 'Neumann'." if sig.size != self._nC else "")
+                    "anisotropic conductivity is not supported by the GPU nodal simulation")
             self.engine.set_sigma(sig)
             self._sigma_on_gpu = True
 

@@ -1,0 +1,150 @@
+"""The reference's own test properties for Simulation3DNodal, run on the GPU simulation:
+analytic half-space pole-dipole data (tests/em/static/test_DC_analytic.py:10-81), Taylor and
+adjoint identities of Jvec/Jtvec (tests/em/static/test_DC_jvecjtvecadj.py:254-274), default top
+surface equal to explicit surface_faces (tests/em/static/test_DC_Boundaries.py:20-46), the
+sigma/rho parametrisations (props.py:135-171) and the derived fields e, j, charge
+(resistivity/fields.py:186-257)."""
+
+import numpy as np
+import pytest
+
+from tests._helpers import oracle_for, rel, tiny_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    return tiny_problem()
+
+
+@pytest.mark.parametrize("bc", ["Robin", "Neumann"])
+def test_halfspace_pole_dipole_matches_closed_form(gpu, bc):
+    """sigma = 1e-2 S/m half-space, 25 m cells, pole-dipole differences A - B against
+    2 I / (4 pi sigma) (1/r_AM - 1/r_AN): relative RMS below 5 %."""
+    from simpeg_amd import maps
+    from simpeg_amd.mesh import TensorMesh
+    from simpeg_amd.resistivity import Simulation3DNodal
+    from simpeg_amd.resistivity import receivers as Rx
+    from simpeg_amd.resistivity import sources as Src
+    from simpeg_amd.resistivity.survey import Survey
+
+    cs, npad = 25.0, 7
+    hx = [(cs, npad, -1.3), (cs, 21), (cs, npad, 1.3)]
+    hz = [(cs, npad, -1.3), (cs, 20)]
+    mesh = TensorMesh([hx, hx, hz], origin="CCN")
+    x = mesh.cell_centers_x[(mesh.cell_centers_x > -100) & (mesh.cell_centers_x < 100)]
+    y = mesh.cell_centers_y[(mesh.cell_centers_y > -100) & (mesh.cell_centers_y < 100)]
+    gx, gy = np.meshgrid(x, y, indexing="ij")
+    loc = np.c_[gx.ravel(order="F"), gy.ravel(order="F"), np.zeros(gx.size)]
+    M, N = loc - [25.0, 0, 0], loc + [25.0, 0, 0]
+    A, B = np.r_[-200.0, 0, 0], np.r_[200.0, 0, 0]
+    rx = Rx.Dipole(M, N)
+    survey = Survey([Src.Dipole([rx], A, B)])
+    sim = Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.IdentityMap(mesh), bc_type=bc)
+    d = sim.dpred(np.full(mesh.n_cells, 1e-2))
+
+    def pd(tx):
+        return 2.0 / (4 * np.pi * 1e-2) * (1 / np.linalg.norm(M - tx, axis=1) -
+                                             1 / np.linalg.norm(N - tx, axis=1))
+
+    ana = pd(A) - pd(B)
+    err = np.sqrt(np.mean(((d - ana) / ana) ** 2))
+    assert err < 0.05, err
+
+
+@pytest.mark.parametrize("bc", ["Robin", "Neumann"])
+def test_jvec_taylor_and_adjoint(gpu, tiny, bc):
+    """dpred(m + h dm) - dpred(m) - h J dm decays as O(h^2), and |w^T J v - v^T J^T w| is at
+    round-off (dipole receivers, so the Neumann identity row does not enter the data)."""
+    from simpeg_amd import maps
+    from simpeg_amd.resistivity import Simulation3DNodal
+
+    mesh, survey, m = tiny
+    sim = Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.ExpMap(mesh), bc_type=bc,
+                            rtol=1e-12)
+    rng = np.random.default_rng(42)
+    dm = rng.standard_normal(mesh.n_cells)
+    d0 = sim.dpred(m)
+    jdm = sim.Jvec(m, dm)
+    e2 = []
+    for h in (1e-1, 1e-2, 1e-3):
+        e2.append(np.linalg.norm(sim.dpred(m + h * dm) - d0 - h * jdm))
+    order = (np.log10(e2[0] / e2[1]), np.log10(e2[1] / e2[2]))
+    assert min(order) > 1.8, order
+    w = rng.standard_normal(survey.nD)
+    v = rng.standard_normal(mesh.n_cells)
+    wjv = w @ sim.Jvec(m, v)
+    vjtw = v @ sim.Jtvec(m, w)
+    assert abs(wjv - vjtw) < 1e-8 * max(abs(wjv), abs(vjtw))
+
+
+def test_default_top_surface_equals_explicit_surface_faces(gpu, tiny):
+    """With no surface_faces the top (z = max) boundary faces carry no Robin term; passing that
+    mask explicitly gives the same data.  (An empty mask changes nothing either: the Robin
+    weight (r.n)/r vanishes on the top faces by itself, r being horizontal there.)  Marking
+    every boundary face but the bottom ones as surface leaves the Robin term on the bottom only
+    and changes the data."""
+    from simpeg_amd import maps
+    from simpeg_amd.resistivity import Simulation3DNodal
+    from oracle import OracleMesh
+
+    mesh, survey, m = tiny
+    om = OracleMesh(mesh.h, origin=mesh.origin)
+    bf = om.boundary_faces
+    top = bf[:, 2] == om.nodes_z[-1]
+    bottom = bf[:, 2] == om.nodes_z[0]
+    d = {}
+    for key, sf in (("default", None), ("top", top), ("none", np.zeros_like(top)),
+                    ("bottom only", ~bottom)):
+        sim = Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.ExpMap(mesh), surface_faces=sf)
+        d[key] = sim.dpred(m)
+    assert rel(d["top"], d["default"]) < 1e-9
+    assert rel(d["none"], d["default"]) < 1e-9
+    assert rel(d["bottom only"], d["default"]) > 1e-4
+
+
+def test_sigma_and_rho_parametrisations_agree(gpu, tiny):
+    """sigmaMap = ExpMap on m, rhoMap = IdentityMap on rho = exp(-m) and a fixed sigma give the
+    same potentials; the rho form's Jtvec is the sigma form's times d sigma / d rho."""
+    from simpeg_amd import maps
+    from simpeg_amd.resistivity import Simulation3DNodal
+
+    mesh, survey, m = tiny
+    s_exp = Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.ExpMap(mesh))
+    s_rho = Simulation3DNodal(mesh, survey=survey, rhoMap=maps.IdentityMap(mesh))
+    s_fix = Simulation3DNodal(mesh, survey=survey, sigma=np.exp(m))
+    rho = np.exp(-m)
+    d = s_exp.dpred(m)
+    assert rel(s_rho.dpred(rho), d) < 1e-9
+    assert rel(s_fix.dpred(), d) < 1e-9
+    w = np.random.default_rng(5).standard_normal(survey.nD)
+    g_m = s_exp.Jtvec(m, w)          # d/dm with sigma = exp(m)
+    g_rho = s_rho.Jtvec(rho, w)      # d/drho with sigma = 1/rho
+    # chain rule: d/dm = d/drho * drho/dm = d/drho * (-rho)
+    assert rel(-rho * g_rho, g_m) < 1e-6
+
+
+def test_derived_fields_match_oracle(gpu, tiny):
+    """e = -G phi on edges, j = M_e(sigma) e / M_e(1), charge = -eps0 G^T M_e(1) e and the
+    charge density, from the GPU potentials, against the same quantities of the oracle's."""
+    from simpeg_amd import maps
+    from simpeg_amd.resistivity import Simulation3DNodal
+    from simpeg_amd.resistivity.fields import EPSILON_0
+
+    mesh, survey, m = tiny
+    sim = Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.ExpMap(mesh))
+    orc = oracle_for(mesh, survey)
+    u_ref = orc.fields(m)
+    f = sim.fields(m)
+    G = orc.mesh.nodal_gradient
+    me1 = orc.mesh.get_edge_inner_product(np.ones(mesh.n_cells)).diagonal()
+    mes = orc.mesh.get_edge_inner_product(np.exp(m)).diagonal()
+    e_ref = -(G @ u_ref)
+    assert rel(f[:, "e"], e_ref) < 1e-7
+    assert rel(f[:, "j"], (mes / me1)[:, None] * e_ref) < 1e-7
+    ch_ref = -EPSILON_0 * (G.T @ (me1[:, None] * e_ref))
+    assert rel(f[:, "charge"], ch_ref) < 1e-6
+    src = survey.source_list[1]
+    assert f[src, "phi"].shape == (mesh.n_nodes, 1)
+    assert rel(f[src, "phi"][:, 0], u_ref[:, 1]) < 1e-7
