@@ -144,11 +144,11 @@ def kernel_names(params, ncol):
     rx = params.xfer_rhs_per_thread if params.xfer_rhs_per_thread in (1, 2, 4, 8) else \
         (8 if ncol >= 8 else r)
     rr = params.refine_rhs_per_thread or (8 if ncol >= 8 else r)
-    z = "bf16" if params.vcycle_bf16 == 1 else "float"
-    b = "bf16" if params.vcycle_bf16 == 3 else "float"
+    z = "bf16" if params.vcycle_bf16 in (1, 4) else "float"
+    b = "bf16" if params.vcycle_bf16 in (3, 4) else "float"
     v = "float" if params.vcycle_bf16 == 0 else "bf16"
     return {
-        "update": f"k_update_r4<{'true' if params.vcycle_bf16 == 3 else 'false'}>",
+        "update": f"k_update_r4<{'true' if params.vcycle_bf16 in (3, 4) else 'false'}>",
         "smooth": f"k_cheb<float,{b},{v},{z},{r},1,true>",
         "apply": f"k_apply_pq<{z},{r},false>", "pair": f"k_smooth_pair<float,{b},{v},{r}>",
         "residual": f"k_residual<float,{b},{v},{r}>",

@@ -87,7 +87,8 @@ typedef struct {
   int32_t vcycle_bf16;         /* 1: the fp32 solver keeps its multigrid vectors, including the
                                   preconditioned residual z, in bfloat16 (arithmetic stays fp32);
                                   2: the same with z in fp32; 3: as 2, and the finest level
-                                  reads a bfloat16 copy of the residual; 0: fp32 vectors */
+                                  reads a bfloat16 copy of the residual; 4: as 1, and the finest
+                                  level reads a bfloat16 copy of the residual; 0: fp32 vectors */
   int32_t xfer_separable;      /* restriction: 1 = one pass per coarsened axis, 0 = one 27-point
                                   gather pass */
   int32_t wcycle_level;        /* 0: V-cycle; l >= 1: the coarse correction of level l runs twice

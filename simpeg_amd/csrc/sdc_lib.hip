@@ -141,7 +141,7 @@ struct Work {
   int* ocount = nullptr;
   // level 0 nodes where some right-hand-side column is non-zero (fp64 residual skips q elsewhere)
   unsigned* qmask = nullptr;
-  // bfloat16 copy of r, the finest-level multigrid input (vcycle_bf16 = 3)
+  // bfloat16 copy of r, the finest-level multigrid input (vcycle_bf16 = 3 or 4)
   __bf16* r16 = nullptr;
   void release() {
     dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(r16);
@@ -462,7 +462,8 @@ template <typename T>
 int ensure_work(sdc_handle_s* h, int ncol) {
   Work<T>& w = work_of<T>(h);
   const int vb = vcycle_bytes<T>(h);
-  const bool want16 = std::is_same<T, float>::value && h->prm.vcycle_bf16 == 3;
+  const bool want16 =
+      std::is_same<T, float>::value && (h->prm.vcycle_bf16 == 3 || h->prm.vcycle_bf16 == 4);
   if (w.ncap >= ncol && w.vbytes == vb && (w.r16 != nullptr) == want16) return SDC_OK;
   ncol = std::max(ncol, w.ncap);
   drop_graphs(h);
@@ -503,7 +504,7 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   CK(dmalloc(&w.omask, cap));
   CK(dmalloc(&w.ocount, 1));
   CK(dmalloc(&w.qmask, ld0));
-  if (std::is_same<T, float>::value && h->prm.vcycle_bf16 == 3) CK(dmalloc(&w.r16, n0));
+  if (want16) CK(dmalloc(&w.r16, n0));
   w.ncap = cap;
   return SDC_OK;
 }
@@ -982,8 +983,9 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const double delta2 = h->prm.reliable_delta * h->prm.reliable_delta;
   // u is not cleared: the first fold writes it (and its column padding) without reading it
   hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
-  // vcycle_bf16 = 3: the multigrid reads a bfloat16 copy of r, written with r
-  __bf16* const r16 = (h->prm.vcycle_bf16 == 3 && h->L.size() > 1) ? W.r16 : nullptr;
+  // vcycle_bf16 = 3, 4: the multigrid reads a bfloat16 copy of r, written with r
+  __bf16* const r16 =
+      ((h->prm.vcycle_bf16 == 3 || h->prm.vcycle_bf16 == 4) && h->L.size() > 1) ? W.r16 : nullptr;
   DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
                                    (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask, r16));
   // aux = |q|^2 and active = |q| > 0; orr = |q|^2
@@ -1014,7 +1016,8 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   // Iteration: z = M r (z kept in the V-cycle storage type), beta, p = z + beta p and
   // q = A z + beta q in one pass (plus the lagged x += alpha p), alpha, r -= alpha q.  A reliable
   // update folds x + alpha p into u, so the lagged update is cleared with alpha.
-  const bool zb = h->prm.vcycle_bf16 == 1 && h->L.size() > 1;
+  // vcycle_bf16 = 1, 4: the preconditioned residual z is bfloat16
+  const bool zb = (h->prm.vcycle_bf16 == 1 || h->prm.vcycle_bf16 == 4) && h->L.size() > 1;
   __bf16* const z16 = (__bf16*)W.xb0;
   const int* done = W.counters + 3;
   const int check = std::max(1, (int)h->prm.check_every);
@@ -1025,7 +1028,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   int it;
   for (it = 1; it <= maxit; ++it) {
     const bool first = it == 1;
-    if (zb) {
+    if (zb && r16) {
+      CK(graph_run(h, r16, z16, ncol, 4, st, [&](hipStream_t s) {
+        return vcycle<float, __bf16, __bf16, __bf16>(h, 0, r16, z16, ncol, true, nullptr, s);
+      }));
+    } else if (zb) {
       CK(graph_run(h, W.r, z16, ncol, 1, st, [&](hipStream_t s) {
         return vcycle<float, __bf16, float, __bf16>(h, 0, W.r, z16, ncol, true, nullptr, s);
       }));
@@ -1311,8 +1318,8 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
       p->xfer_rhs_per_thread != 2 && p->xfer_rhs_per_thread != 4 && p->xfer_rhs_per_thread != 8)
     return fail(SDC_ERR_ARG, "xfer_rhs_per_thread must be 0, 1, 2, 4 or 8");
   if (p->xcd_remap != 0 && p->xcd_remap != 1) return fail(SDC_ERR_ARG, "xcd_remap must be 0 or 1");
-  if (p->vcycle_bf16 < 0 || p->vcycle_bf16 > 3)
-    return fail(SDC_ERR_ARG, "vcycle_bf16 must be 0, 1, 2 or 3");
+  if (p->vcycle_bf16 < 0 || p->vcycle_bf16 > 4)
+    return fail(SDC_ERR_ARG, "vcycle_bf16 must be 0, 1, 2, 3 or 4");
   if (p->xfer_separable != 0 && p->xfer_separable != 1)
     return fail(SDC_ERR_ARG, "xfer_separable must be 0 or 1");
   if (p->wcycle_level < -1) return fail(SDC_ERR_ARG, "wcycle_level must be >= -1");

@@ -96,12 +96,12 @@ def test_fp32_vcycle_bf16_storage(gpu, tiny, bc):
     assert np.all(np.sum(out[1] * a, axis=0) > 0)
 
 
-@pytest.mark.parametrize("storage", [0, 1, 2, 3])
+@pytest.mark.parametrize("storage", [0, 1, 2, 3, 4])
 def test_mixed_solve_with_each_vcycle_storage(gpu, tiny, storage):
     """Mixed-precision solves with fp32 multigrid vectors (0), bf16 vectors and a bf16
-    preconditioned residual (1), bf16 vectors with an fp32 one (2), and (3) bf16 vectors fed
-    from a bf16 copy of the residual all reach the fp64 tolerance and the direct solver's
-    potentials."""
+    preconditioned residual (1), bf16 vectors with an fp32 one (2), bf16 vectors fed from a
+    bf16 copy of the residual (3), and both (4) all reach the fp64 tolerance and the direct
+    solver's potentials."""
     mesh, survey, m = tiny
     sim = _sim(mesh, survey, gpu_params={"vcycle_bf16": storage})
     orc = oracle_for(mesh, survey)
