@@ -455,3 +455,33 @@ def test_operator_dependent_interpolation_on_a_high_contrast_model(gpu):
         for s in range(wl.survey.nSrc):
             assert rel(u[:, s], u_ref[:, s]) < 1e-7
     assert its[1] <= its[0]
+
+
+@pytest.mark.parametrize("shape", [(12, 1, 8), (3, 17, 2), (21, 9, 13)])
+@pytest.mark.parametrize("bc", ["Robin", "Neumann"])
+def test_thin_and_odd_meshes(gpu, shape, bc):
+    """A one-cell-thick mesh, an axis with 2 cells and odd cell counts (unequal coarsening
+    gaps, axes that stop coarsening early): potentials against splu for random right-hand
+    sides through the solver wrapper, both interpolations."""
+    from simpeg_amd.mesh import TensorMesh
+    from simpeg_amd.solvers import NodalGPUSolver
+    from oracle import OracleDC, OracleMesh
+
+    rng = np.random.default_rng(sum(shape))
+    h = [rng.uniform(5.0, 15.0, n) for n in shape]
+    mesh = TensorMesh(h)
+    om = OracleMesh(mesh.h, origin=mesh.origin)
+    orc = OracleDC(om, [], bc_type=bc)
+    orc.set_model(np.log(0.01) + rng.standard_normal(mesh.n_cells))
+    A = orc.getA()
+    b = rng.standard_normal((mesh.n_nodes, 3))
+    ref = orc.solve(b)
+    for interp in (0, 1):
+        Ainv = NodalGPUSolver(A, mesh=mesh, gpu_params={"interp": interp})
+        try:
+            x = Ainv * b
+            assert Ainv.last_solve.converged
+            for c in range(3):
+                assert rel(x[:, c], ref[:, c]) < 1e-7, (interp, c)
+        finally:
+            Ainv.clean()
