@@ -13,6 +13,7 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 
+from .maps import sdiag
 from .resistivity.simulation import Simulation3DNodal as _DCNodal
 
 
@@ -55,11 +56,11 @@ class Simulation3DNodal(_DCNodal):
 
     @property
     def sigmaDeriv(self):
-        return sp.csr_matrix(-sp.diags(self.sigma) @ self.etaDeriv)
+        return sp.csr_matrix(sdiag(-self.sigma) @ self.etaDeriv)
 
     @property
     def rhoDeriv(self):
-        return sp.csr_matrix(sp.diags(self.rho) @ self.etaDeriv)
+        return sp.csr_matrix(sdiag(self.rho) @ self.etaDeriv)
 
     def _ensure_sigma(self):
         # sigma does not depend on the model: upload it once

@@ -13,6 +13,15 @@ import numpy as np
 import scipy.sparse as sp
 
 
+def sdiag(d):
+    """Diagonal CSR matrix of the vector d, built directly (scipy.sparse.diags goes through the
+    DIA and COO formats, ~80 ms at a million entries)."""
+    d = np.asarray(d, dtype=float).ravel()
+    n = d.size
+    idx = np.arange(n + 1, dtype=np.int64 if n >= 2**31 - 1 else np.int32)
+    return sp.csr_matrix((d, idx[:n], idx), shape=(n, n))
+
+
 class IdentityMap:
     def __init__(self, mesh=None, nP=None, **kwargs):
         if nP is None and mesh is not None:
@@ -35,7 +44,7 @@ class IdentityMap:
         return np.asarray(m, dtype=float).ravel()
 
     def deriv(self, m, v=None):
-        d = sp.identity(np.size(m), format="csr")
+        d = sdiag(np.ones(np.size(m)))
         return d if v is None else d @ v
 
     def __mul__(self, val):
@@ -59,7 +68,7 @@ class ExpMap(IdentityMap):
         return np.log(np.asarray(D, dtype=float).ravel())
 
     def deriv(self, m, v=None):
-        d = sp.diags(np.exp(np.asarray(m, dtype=float).ravel()), format="csr")
+        d = sdiag(np.exp(np.asarray(m, dtype=float).ravel()))
         return d if v is None else d @ v
 
     @property
@@ -78,7 +87,7 @@ class ReciprocalMap(IdentityMap):
 
     def deriv(self, m, v=None):
         m = np.asarray(m, dtype=float).ravel()
-        d = sp.diags(-1.0 / m ** 2, format="csr")
+        d = sdiag(-1.0 / m ** 2)
         return d if v is None else d @ v
 
     @property

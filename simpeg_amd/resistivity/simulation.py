@@ -225,7 +225,7 @@ class Simulation3DNodal:
             return sp.csr_matrix(self.sigmaMap.deriv(self._model))
         if self.rhoMap is not None:
             s = self.sigma
-            return sp.csr_matrix(sp.diags(-s * s) @ self.rhoMap.deriv(self._model))
+            return sp.csr_matrix(_maps.sdiag(-s * s) @ self.rhoMap.deriv(self._model))
         raise AttributeError("sigmaDeriv needs sigmaMap or rhoMap")
 
     # ------------------------------------------------------------------ GPU state
