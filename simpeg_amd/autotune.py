@@ -32,6 +32,10 @@ CANDIDATES = [
     dict(_BASE, check_every=2),
     dict(_BASE, rhs_per_thread=8, xfer_rhs_per_thread=8),
     dict(_BASE, vcycle_bf16=2),
+    # few columns: the fused coarse tail runs one work-group per column, so it can idle most
+    # of the device; the multi-launch coarse levels spread the same work over nodes
+    dict(_BASE, fused_tail=0),
+    dict(_BASE, fused_tail=0, wcycle_level=0),
 ]
 
 
