@@ -267,7 +267,7 @@ def cpu_baseline(budget_s=12.0):
 
 def bench_batch(args):
     """48 independent problems (uneven sizes / source counts) behind MetaSimulation: fields +
-    Jtvec for all of them, one stream after the other vs concurrently on 4 streams.  Multigrid
+    Jtvec for all of them, one stream after the other vs concurrently on 8 streams.  Multigrid
     settings come from the autotuner (tuned during the one-stream warm-up, cached per mesh shape
     and column bucket)."""
     import torch
@@ -281,7 +281,7 @@ def bench_batch(args):
     m = np.concatenate([wl.model for wl in batch])
     nsrc = sum(wl.survey.nSrc for wl in batch)
     res = {}
-    for ns in (1, 4):
+    for ns in (1, 8):
         sims = [Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
                                   rtol=args.rtol, autotune=True,
                                   gpu_params=json.loads(args.gpu_params)) for wl in batch]
@@ -301,13 +301,13 @@ def bench_batch(args):
         res[ns] = (t1 - t0, t2 - t1)
         log(f"[bench] batch streams={ns}: fields {1e3*(t1-t0):.1f} ms, Jtvec {1e3*(t2-t1):.1f} ms")
     out = {"metric": "uneven batch: source-solves/s (fields of 48 problems)", "unit": "source-solves/s",
-           "value": round(nsrc / res[4][0], 2), "n_gpus": 1, "higher_is_better": True,
+           "value": round(nsrc / res[8][0], 2), "n_gpus": 1, "higher_is_better": True,
            "config": {"workload": "uneven_batch_48", "problems": len(batch), "sources": nsrc,
                       "node_columns": int(sum(wl.mesh.n_nodes * wl.survey.nSrc for wl in batch))},
            "fields_ms": {str(k): round(1e3 * v[0], 1) for k, v in res.items()},
            "jtvec_ms": {str(k): round(1e3 * v[1], 1) for k, v in res.items()},
            "autotuned": True,
-           "concurrency_speedup": round(sum(res[1]) / sum(res[4]), 2)}
+           "concurrency_speedup": round(sum(res[1]) / sum(res[8]), 2)}
     print(json.dumps(out), flush=True)
 
 
