@@ -865,6 +865,8 @@ int graph_run(sdc_handle_s* h, const void* b, void* x, int ncol, int variant, hi
       HIPCK(hipGraphLaunch(g.exec, st));
       return SDC_OK;
     }
+  // bounded cache: a caller cycling through many column counts starts over past 64 graphs
+  if (h->graphs.size() >= 64) drop_graphs(h);
   sdc_handle_s::Recorder rec;
   HIPCK(hipGraphCreate(&rec.g, 0));
   h->rec = &rec;
