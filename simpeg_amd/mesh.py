@@ -320,6 +320,44 @@ def edge_mass(widths, sigma=None):
     return np.concatenate([ex.ravel(order="F"), ey.ravel(order="F"), ez.ravel(order="F")])
 
 
+def edge_cell_weights(widths):
+    """A_v (n_edges x n_cells, CSR): V_c / 4 for every cell c bordering edge e, so that
+    edge_mass(widths, sigma) = A_v @ sigma (the isotropic edge inner product's averaging)."""
+    hx, hy, hz = widths
+    nx, ny, nz = len(hx), len(hy), len(hz)
+    vol = (hx[:, None, None] * hy[None, :, None] * hz[None, None, :]).ravel(order="F") / 4.0
+
+    def cell(i, j, k):
+        return i + nx * (j + ny * k)
+
+    rows, cols, vals = [], [], []
+    off = 0
+    # x edges (i, j, k), i < nx: cells (i, j - a, k - b); y edges: (i - a, j, k - b); z edges:
+    # (i - a, j - b, k)
+    for axis, shape in ((0, (nx, ny + 1, nz + 1)), (1, (nx + 1, ny, nz + 1)),
+                        (2, (nx + 1, ny + 1, nz))):
+        I, J, K = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+        e = (I + shape[0] * (J + shape[1] * K)).ravel(order="F") + off
+        I, J, K = (X.ravel(order="F") for X in (I, J, K))
+        for a in (0, 1):
+            for b in (0, 1):
+                ci, cj, ck = I.copy(), J.copy(), K.copy()
+                if axis == 0:
+                    cj, ck = J - a, K - b
+                elif axis == 1:
+                    ci, ck = I - a, K - b
+                else:
+                    ci, cj = I - a, J - b
+                ok = (ci >= 0) & (ci < nx) & (cj >= 0) & (cj < ny) & (ck >= 0) & (ck < nz)
+                c = cell(ci[ok], cj[ok], ck[ok])
+                rows.append(e[ok])
+                cols.append(c)
+                vals.append(vol[c])
+        off += int(np.prod(shape))
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(off, nx * ny * nz))
+
+
 def robin_diagonal(widths, beta, sigma):
     """b_n = sum over boundary faces f containing node n of beta_f sigma(cell of f)."""
     hx, hy, hz = widths

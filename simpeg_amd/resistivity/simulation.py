@@ -120,6 +120,7 @@ class Simulation3DNodal:
         self._sigma_on_gpu = None
         self._Jmatrix = None
         self._gtgdiag = None
+        self._Av = None
         self._proj = None
         self._q = None
         self._q_block = None
@@ -523,6 +524,26 @@ class Simulation3DNodal:
         lb = eng.to_block(np.asarray(v, dtype=float))
         g = eng.contract(ub, lb, lb.shape[0]).cpu().numpy()
         return self.sigmaDeriv.T @ g
+
+    def getRHS(self):
+        """Right-hand sides q (nN, nSrc) (``getRHS``, simulation.py:646-653)."""
+        return self.getSourceTerm()
+
+    def MeSigmaDeriv(self, u, v=None, adjoint=False):
+        """Derivative of M_e(sigma) u with respect to the model (base/pde_simulation.py:296-321,
+        applied as in _inner_mat_mul_op, 15-83): the matrix diag(u) A_v sigmaDeriv, or its product
+        with v (forward: u * (A_v sigmaDeriv v); adjoint: sigmaDeriv^T A_v^T (u * v))."""
+        from ..mesh import edge_cell_weights
+
+        if self._Av is None:
+            self._Av = edge_cell_weights(self._widths)
+        u = np.asarray(u, dtype=float)
+        if v is None:
+            return _maps.sdiag(u) @ self._Av @ self.sigmaDeriv
+        v = np.asarray(v, dtype=float)
+        if adjoint:
+            return self.sigmaDeriv.T @ (self._Av.T @ (u * v))
+        return u * (self._Av @ (self.sigmaDeriv @ v))
 
     def getRHSDeriv(self, source, v, adjoint=False):
         return 0.0

@@ -251,3 +251,30 @@ def test_meta_adjoint_sum_matches_sparse_transpose():
     parts = [rng.standard_normal(mp.shape[0]) for mp in mps]
     ref = sum(np.asarray(mp.deriv(model).T @ p).ravel() for mp, p in zip(mps, parts))
     assert np.allclose(sum_adjoint(mps, model, parts), ref, rtol=1e-14, atol=1e-14)
+
+
+def test_edge_cell_weights_and_mesigma_deriv_match_oracle():
+    """A_v equals the oracle's edge-cell averaging (so A_v sigma = edge_mass), MeSigmaDeriv
+    applies diag(u) A_v sigmaDeriv forward and transposed (base/pde_simulation.py:296-321), and
+    getRHS is the source term."""
+    from simpeg_amd.mesh import edge_cell_weights
+    from simpeg_amd.resistivity import Simulation3DNodal
+
+    mesh, survey, m = tiny_problem()
+    om = OracleMesh(mesh.h, origin=mesh.origin)
+    Av = edge_cell_weights(mesh.h)
+    assert abs(Av - om.edge_cell_weights).max() < 1e-12 * abs(Av).max()
+    s = np.exp(m)
+    assert rel(Av @ s, edge_mass(mesh.h, s)) < 1e-14
+    sim = Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.ExpMap(mesh))
+    sim.model = m
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal(Av.shape[0])
+    v = rng.standard_normal(mesh.n_cells)
+    w = rng.standard_normal(Av.shape[0])
+    ref = u * (om.edge_cell_weights @ (s * v))
+    assert rel(sim.MeSigmaDeriv(u, v), ref) < 1e-13
+    assert rel(sim.MeSigmaDeriv(u) @ v, ref) < 1e-13
+    assert np.isclose(w @ sim.MeSigmaDeriv(u, v), v @ sim.MeSigmaDeriv(u, w, adjoint=True),
+                      rtol=1e-12)
+    assert np.array_equal(sim.getRHS(), sim.getSourceTerm())
