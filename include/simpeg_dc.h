@@ -105,6 +105,9 @@ typedef struct {
   int32_t interp;              /* multigrid interpolation: 1 = operator-dependent (a node between
                                   two coarse nodes weights them by the conductances of its two
                                   edges on its grid line), 0 = geometric (linear in position) */
+  int32_t wcycle_smooth;       /* smoothing steps between the two coarse corrections of the W
+                                  step (0 = nu_post; default 1, measured 1.4 % faster than 2 at
+                                  the same iteration counts) */
 } sdc_params;
 
 typedef struct {

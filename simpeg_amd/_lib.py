@@ -63,6 +63,7 @@ class Params(C.Structure):
         ("refine_rhs_per_thread", C.c_int32),
         ("graph", C.c_int32),
         ("interp", C.c_int32),
+        ("wcycle_smooth", C.c_int32),
     ]
 
 

@@ -21,7 +21,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "simpeg_amd", "a
 # Every candidate is a symmetric cycle (nu_pre == nu_post), as PCG needs a symmetric
 # preconditioner.  Fields not named keep the engine's current values.
 _BASE = dict(nu_pre=2, nu_post=2, cheb_ratio=8.0, rhs_per_thread=0, xfer_rhs_per_thread=0,
-             wcycle_level=-1, interp=1, vcycle_bf16=1, check_every=1)
+             wcycle_level=-1, wcycle_smooth=1, interp=1, vcycle_bf16=1, check_every=1)
 CANDIDATES = [
     dict(_BASE),
     dict(_BASE, wcycle_level=0),

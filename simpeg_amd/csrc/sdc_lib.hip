@@ -682,7 +682,10 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   // W-cycle step on one level: the coarse correction runs twice, with nu_post smoothing steps
   // between the two (the error propagator stays symmetric: S* C S C S)
   const int ngam = (h->wlevel > 0 && l == h->wlevel) ? 2 : 1;
-  const int swaps = (nu1 >= 2 ? nu1 - 2 : 0) + nu2 * ngam;
+  // smoothing steps between the two coarse corrections of a W step (any number keeps the cycle
+  // symmetric: every Chebyshev polynomial in D^-1 A is self-adjoint in the A inner product)
+  const int num = h->prm.wcycle_smooth > 0 ? h->prm.wcycle_smooth : nu2;
+  const int swaps = (nu1 >= 2 ? nu1 - 2 : 0) + nu2 + (ngam - 1) * num;
   S* const xa = (S*)W.xa[l];
   S* cur;
   S* oth;
@@ -728,8 +731,9 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   S* const cx = (S*)W.xs[l + 1];
   const int RX = pick_rx(h, ncol);
   auto post_smooth = [&](bool fin) {
-    for (int k = 0; k < nu2; ++k) {
-      const bool last = k == nu2 - 1 && fin;
+    const int nsteps = fin ? nu2 : num;
+    for (int k = 0; k < nsteps; ++k) {
+      const bool last = k == nsteps - 1 && fin;
       double* part = (last && dot_out) ? W.partial : nodot;
       const T c1 = (T)(k == 0 ? 0.0 : cc.c1[k]), c2 = (T)cc.c2[k];
       const int ptag =
@@ -1177,6 +1181,7 @@ int sdc_default_params(sdc_params* p) {
   p->refine_rhs_per_thread = 0;
   p->graph = 1;
   p->interp = 1;
+  p->wcycle_smooth = 1;
   p->check_every = 1;
   p->reliable_delta = 1e-4;
   return SDC_OK;
@@ -1334,6 +1339,8 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
     return fail(SDC_ERR_ARG, "reliable_delta must be in (0, 1)");
   if (p->graph != 0 && p->graph != 1) return fail(SDC_ERR_ARG, "graph must be 0 or 1");
   if (p->interp != 0 && p->interp != 1) return fail(SDC_ERR_ARG, "interp must be 0 or 1");
+  if (p->wcycle_smooth < 0 || p->wcycle_smooth > 15)
+    return fail(SDC_ERR_ARG, "wcycle_smooth must be in [0, 15]");
   DeviceGuard dg(h->device);
   drop_graphs(h);
   h->prm = *p;
