@@ -108,6 +108,8 @@ typedef struct {
   int32_t wcycle_smooth;       /* smoothing steps between the two coarse corrections of the W
                                   step (0 = nu_post; default 1, measured 1.4 % faster than 2 at
                                   the same iteration counts) */
+  int32_t nu_coarse;           /* Chebyshev steps before and after the coarse correction on the
+                                  levels below the finest (0 = nu_pre / nu_post) */
 } sdc_params;
 
 typedef struct {

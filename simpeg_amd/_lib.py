@@ -64,6 +64,7 @@ class Params(C.Structure):
         ("graph", C.c_int32),
         ("interp", C.c_int32),
         ("wcycle_smooth", C.c_int32),
+        ("nu_coarse", C.c_int32),
     ]
 
 

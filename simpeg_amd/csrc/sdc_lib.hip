@@ -629,8 +629,8 @@ int tail_vcycle(sdc_handle_s* h, const S* b, S* x, int ncol, const int* done, hi
   d.offY = off;
   d.ainv = h->ainv32;
   d.nn = (int)h->L[nl - 1].nN;
-  d.nu_pre = h->prm.nu_pre;
-  d.nu_post = h->prm.nu_post;
+  d.nu_pre = h->prm.nu_coarse > 0 ? h->prm.nu_coarse : h->prm.nu_pre;
+  d.nu_post = h->prm.nu_coarse > 0 ? h->prm.nu_coarse : h->prm.nu_post;
   const Cheb cc = cheb_coefs(h->prm);
   for (int k = 0; k < 16; ++k) {
     d.c1[k] = (float)cc.c1[k];
@@ -674,7 +674,10 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   }
   const Sten<T> s = make_sten<T>(h, l, 1);
   const Cheb cc = cheb_coefs(h->prm);
-  const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
+  // coarse levels may use fewer symmetric smoothing steps (nu_coarse, 0 = as the finest)
+  const int nuc = h->prm.nu_coarse;
+  const int nu1 = (l > 0 && nuc > 0) ? nuc : h->prm.nu_pre;
+  const int nu2 = (l > 0 && nuc > 0) ? nuc : h->prm.nu_post;
   // Smoothing steps ping-pong between two S buffers.  When x has the storage type (O == S) it
   // is one of them and the first output goes where the last one then lands in x; otherwise
   // (finest level of the fp32 solver with bf16 storage) the last step writes x directly.
@@ -1182,6 +1185,7 @@ int sdc_default_params(sdc_params* p) {
   p->graph = 1;
   p->interp = 1;
   p->wcycle_smooth = 1;
+  p->nu_coarse = 0;
   p->check_every = 1;
   p->reliable_delta = 1e-4;
   return SDC_OK;
@@ -1341,6 +1345,7 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
   if (p->interp != 0 && p->interp != 1) return fail(SDC_ERR_ARG, "interp must be 0 or 1");
   if (p->wcycle_smooth < 0 || p->wcycle_smooth > 15)
     return fail(SDC_ERR_ARG, "wcycle_smooth must be in [0, 15]");
+  if (p->nu_coarse < 0 || p->nu_coarse > 15) return fail(SDC_ERR_ARG, "nu_coarse must be in [0, 15]");
   DeviceGuard dg(h->device);
   drop_graphs(h);
   h->prm = *p;

@@ -46,18 +46,18 @@ def test_apply_matches_assembled_operator(gpu, tiny, bc):
 
 
 @pytest.mark.parametrize("interp", [0, 1])
-@pytest.mark.parametrize("wlevel,wsmooth", [(0, 0), (1, 0), (1, 1)])
-def test_vcycle_is_symmetric(gpu, tiny, wlevel, wsmooth, interp):
-    """The fp64 V-cycle (W-cycle on level 1, with the default or one smoothing step between its
-    coarse corrections) is a symmetric positive definite operator, with geometric and with
-    operator-dependent interpolation (restriction = P^T)."""
+@pytest.mark.parametrize("wlevel,wsmooth,nuc", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 1)])
+def test_vcycle_is_symmetric(gpu, tiny, wlevel, wsmooth, nuc, interp):
+    """The fp64 V-cycle (W-cycle on level 1, with two or one smoothing steps between its coarse
+    corrections, and one step on the coarse levels) is a symmetric positive definite operator,
+    with geometric and with operator-dependent interpolation (restriction = P^T)."""
     mesh, survey, m = tiny
     sim = _sim(mesh, survey)
     sim.model = m
     sim._ensure_sigma()
     eng = sim.engine
     assert eng.levels >= 3
-    eng.set_params(wcycle_level=wlevel, interp=interp, wcycle_smooth=wsmooth)
+    eng.set_params(wcycle_level=wlevel, interp=interp, wcycle_smooth=wsmooth, nu_coarse=nuc)
     rng = np.random.default_rng(2)
     a = rng.standard_normal((mesh.n_nodes, 2))
     b = rng.standard_normal((mesh.n_nodes, 2))
