@@ -60,7 +60,7 @@ def build_problem(name, seed=0):
 class Pipeline:
     """The hot path of Simulation3DNodal.fields/dpred/Jtvec at the engine level."""
 
-    def __init__(self, wl, device, gpu_params=None):
+    def __init__(self, wl, device, gpu_params=None, coarse_max=0):
         import torch
 
         from simpeg_amd import maps
@@ -69,7 +69,8 @@ class Pipeline:
         self.torch = torch
         self.wl = wl
         self.sim = Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh),
-                                     device=device, gpu_params=gpu_params)
+                                     device=device, gpu_params=gpu_params,
+                                     coarse_max=coarse_max)
         self.sim.model = wl.model
         self.eng = self.sim.engine
         self.nS = wl.survey.nSrc

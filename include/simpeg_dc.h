@@ -67,7 +67,7 @@ typedef struct {
   const double* beta_ymax;
   const double* beta_zmin;     /* nx*ny, index i + nx*j */
   const double* beta_zmax;
-  int32_t coarse_max;          /* multigrid: coarsen until at most this many nodes (0 = 128) */
+  int32_t coarse_max;          /* multigrid: coarsen until at most this many nodes (0 = 64) */
   int32_t device;              /* HIP device ordinal */
 } sdc_mesh_desc;
 

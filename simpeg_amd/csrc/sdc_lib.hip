@@ -1244,7 +1244,9 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
   }
   fill_level_dims(l0);
   h->L.push_back(l0);
-  const uint32_t cmax = d->coarse_max > 0 ? (uint32_t)d->coarse_max : 128u;
+  // coarsest level: at most 64 nodes by default (its dense inverse is rebuilt by one work-group
+  // at every sdc_set_sigma, O(n^3): 128 nodes took 240 us, 64 nodes ~40 us)
+  const uint32_t cmax = d->coarse_max > 0 ? (uint32_t)d->coarse_max : 64u;
   double hmin = 1e300;
   for (int a = 0; a < 3; ++a)
     for (double v : *w[a]) hmin = std::min(hmin, v);
