@@ -26,6 +26,7 @@ CANDIDATES = [
     dict(_BASE),
     dict(_BASE, wcycle_level=0),
     dict(_BASE, wcycle_level=2),
+    dict(_BASE, wcycle_smooth=2),
     dict(_BASE, interp=0),
     dict(_BASE, cheb_ratio=4.0),
     dict(_BASE, nu_pre=3, nu_post=3),
