@@ -121,6 +121,7 @@ class Simulation3DNodal:
         self._Jmatrix = None
         self._gtgdiag = None
         self._Av = None
+        self._sd_cache = None
         self._proj = None
         self._q = None
         self._q_block = None
@@ -222,12 +223,19 @@ class Simulation3DNodal:
 
     @property
     def sigmaDeriv(self):
+        # kept for the current model array and maps (every model assignment stores a new array)
+        key = (self._model, self.sigmaMap, self.rhoMap)
+        if self._sd_cache is not None and all(a is b for a, b in zip(self._sd_cache[0], key)):
+            return self._sd_cache[1]
         if self.sigmaMap is not None:
-            return sp.csr_matrix(self.sigmaMap.deriv(self._model))
-        if self.rhoMap is not None:
+            sd = sp.csr_matrix(self.sigmaMap.deriv(self._model))
+        elif self.rhoMap is not None:
             s = self.sigma
-            return sp.csr_matrix(_maps.sdiag(-s * s) @ self.rhoMap.deriv(self._model))
-        raise AttributeError("sigmaDeriv needs sigmaMap or rhoMap")
+            sd = sp.csr_matrix(_maps.sdiag(-s * s) @ self.rhoMap.deriv(self._model))
+        else:
+            raise AttributeError("sigmaDeriv needs sigmaMap or rhoMap")
+        self._sd_cache = (key, sd)
+        return sd
 
     # ------------------------------------------------------------------ GPU state
     @property
