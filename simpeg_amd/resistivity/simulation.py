@@ -237,6 +237,19 @@ class Simulation3DNodal:
         self._sd_cache = (key, sd)
         return sd
 
+    def _sigma_deriv_T(self, g):
+        """sigmaDeriv^T g; a diagonal derivative (Exp, Identity, Reciprocal maps) as a product of
+        vectors instead of a transposed sparse product."""
+        sd = self.sigmaDeriv
+        if len(self._sd_cache) == 2:  # is the cached derivative diagonal? (checked once)
+            n = sd.shape[0]
+            diag = sd.shape[1] == n and sd.nnz == n and np.array_equal(
+                sd.indptr, np.arange(n + 1)) and np.array_equal(sd.indices, np.arange(n))
+            self._sd_cache = self._sd_cache + (diag,)
+        if self._sd_cache[2]:
+            return sd.data * np.asarray(g, dtype=float).ravel()
+        return sd.T @ g
+
     # ------------------------------------------------------------------ GPU state
     @property
     def engine(self):
@@ -414,7 +427,7 @@ class Simulation3DNodal:
         rhs = eng.scatter(pr["scatter"], v, nS)
         lam = self._solve(rhs, nS)
         g = eng.contract(f.u_block, lam, nS).cpu().numpy()
-        return -(self.sigmaDeriv.T @ g)
+        return -self._sigma_deriv_T(g)
 
     def getJ(self, m, f=None):
         """Dense sensitivity (nD, nP) (``getJ``, simulation.py:115-121), one adjoint column per datum."""
