@@ -122,6 +122,7 @@ class Simulation3DNodal:
         self._gtgdiag = None
         self._Av = None
         self._sd_cache = None
+        self._sd_diag = None
         self._proj = None
         self._q = None
         self._q_block = None
@@ -241,12 +242,12 @@ class Simulation3DNodal:
         """sigmaDeriv^T g; a diagonal derivative (Exp, Identity, Reciprocal maps) as a product of
         vectors instead of a transposed sparse product."""
         sd = self.sigmaDeriv
-        if len(self._sd_cache) == 2:  # is the cached derivative diagonal? (checked once)
+        if self._sd_diag is None or self._sd_diag[0] is not sd:  # checked once per matrix
             n = sd.shape[0]
             diag = sd.shape[1] == n and sd.nnz == n and np.array_equal(
                 sd.indptr, np.arange(n + 1)) and np.array_equal(sd.indices, np.arange(n))
-            self._sd_cache = self._sd_cache + (diag,)
-        if self._sd_cache[2]:
+            self._sd_diag = (sd, diag)
+        if self._sd_diag[1]:
             return sd.data * np.asarray(g, dtype=float).ravel()
         return sd.T @ g
 

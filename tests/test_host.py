@@ -278,3 +278,21 @@ def test_edge_cell_weights_and_mesigma_deriv_match_oracle():
     assert np.isclose(w @ sim.MeSigmaDeriv(u, v), v @ sim.MeSigmaDeriv(u, w, adjoint=True),
                       rtol=1e-12)
     assert np.array_equal(sim.getRHS(), sim.getSourceTerm())
+
+
+def test_map_derivative_transpose_paths():
+    """sigmaDeriv^T g through the diagonal fast path (exp and rho maps, cached) and through the
+    IP simulation's own sigmaDeriv equals the sparse transpose product."""
+    from simpeg_amd.induced_polarization import Simulation3DNodal as IPNodal
+    from simpeg_amd.resistivity import Simulation3DNodal
+
+    mesh, survey, m = tiny_problem()
+    g = np.random.default_rng(2).random(mesh.n_cells)
+    sims = [Simulation3DNodal(mesh, survey=survey, sigmaMap=maps.ExpMap(mesh)),
+            Simulation3DNodal(mesh, survey=survey, rhoMap=maps.IdentityMap(mesh)),
+            IPNodal(mesh, survey=survey, sigma=np.exp(m), etaMap=maps.IdentityMap(mesh))]
+    models = [m, np.exp(-m), np.full(mesh.n_cells, 0.01)]
+    for sim, mm in zip(sims, models):
+        sim.model = mm
+        for _ in range(2):  # second call: cached derivative and diagonal flag
+            assert np.allclose(sim._sigma_deriv_T(g), sim.sigmaDeriv.T @ g, rtol=1e-14)
