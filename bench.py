@@ -13,7 +13,7 @@ in HBM before the timed region starts.
 
 Reported:
   value            forward source-solves/s of the batched mixed-precision path
-                   (fp32 multigrid PCG inside fp64 iterative refinement to --rtol)
+                   (fp32 multigrid PCG with fp64 reliable updates to --rtol)
   jtvec_ms         mean Jtvec time of the same path
   fp64_per_source  the plain fp64 multigrid PCG solving one source at a time (same kernels,
                    one column), source-solves/s, Jtvec ms and the speed-up of the batched
@@ -22,8 +22,14 @@ Reported:
                    the GPU; the CPU direct solver cannot factor this mesh)
   roofline         the dominant kernel's algorithmic bytes per launch / its mean duration,
                    measured with HIP events on its stream during the timed steps
+  kernels          per kernel class: launches, time, algorithmic GB/s (one more profiled step)
+  api_level        the same step through the drop-in Simulation3DNodal methods with host numpy
+                   in and out
   cpu_baseline     the FP64 CPU oracle (scipy splu, the reference's solver without MKL/MUMPS)
                    on a bounded sample (see "sample")
+
+Other workloads: --workload small (the 24x24x12-core block model), batch (48 uneven problems
+behind MetaSimulation), gn (Gauss-Newton on the 160x160x96-core layered model).
 """
 
 from __future__ import annotations
