@@ -125,6 +125,8 @@ typedef struct {
   int32_t iters;               /* PCG iterations summed over refinement steps */
   double max_rel_res;          /* max over columns of |q - A u|/|q| after the solve */
   int32_t converged;           /* 1 if every column reached tol */
+  int32_t fp64_iters;          /* mixed mode: iterations of the fp64 PCG continuation that takes
+                                  over when the fp32 solve stops making progress (0: not used) */
 } sdc_solve_stats;
 
 /* Library identification and the last error message of the calling thread. */

@@ -78,7 +78,7 @@ class SolveOpts(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [
         ("outer", C.c_int32), ("iters", C.c_int32),
-        ("max_rel_res", C.c_double), ("converged", C.c_int32),
+        ("max_rel_res", C.c_double), ("converged", C.c_int32), ("fp64_iters", C.c_int32),
     ]
 
 

@@ -98,7 +98,7 @@ def tune(engine, q, ncol=None, mode="mixed", tol=1e-10, candidates=None, repeats
             dt = (time.perf_counter() - t0) / repeats
         except RuntimeError:
             continue
-        timings.append(dict(params=cand, seconds=dt, iters=res.iters))
+        timings.append(dict(params=cand, seconds=dt, iters=res.iters, fp64_iters=res.fp64_iters))
         if dt < best_t:
             best, best_cand, best_t = p, cand, dt
     if best is None:

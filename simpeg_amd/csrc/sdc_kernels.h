@@ -927,9 +927,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_true(const double* __restrict
 }
 
 // u += e
-template <int R>
+template <typename TE, int R>
 __global__ __launch_bounds__(SDC_BLOCK) void k_ir_update(double* __restrict__ u,
-                                                         const float* __restrict__ e, size_t ld,
+                                                         const TE* __restrict__ e, size_t ld,
                                                          uint32_t nN, int ncol) {
   SDC_NODE_PROLOGUE(nN)
   if (!live) return;

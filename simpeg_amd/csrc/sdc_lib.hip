@@ -1034,6 +1034,13 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const double bz = zb ? 2.0 : 4.0;
   bool xfresh = true;  // x holds nothing that is not yet in u (its memory is not read)
   bool ufresh = true;  // u has not been written yet (the first fold sets it)
+  // Stall guard: a reliable update is forced after kStallWindow iterations without one, and the
+  // solve gives up (converged = 0, the caller continues in fp64) when an update fails to halve
+  // the largest true relative residual of the previous one.  This happens when the fp32
+  // recursion no longer tracks the true residual (conductivity contrasts of 1e4 and more).
+  constexpr int kStallWindow = 25;
+  double m_prev = 1.0;
+  int last_update = 0;
   int it;
   for (it = 1; it <= maxit; ++it) {
     const bool first = it == 1;
@@ -1101,7 +1108,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
         if (h_rr[c] > tol2 * h_qn[c]) cand = false;
         if (h_rr[c] <= delta2 * h_rmax[c]) drop = true;
       }
-      if (cand || drop || it == maxit) {
+      if (cand || drop || it == maxit || it - last_update >= kStallWindow) {
         prof_begin(h, SDC_PROF_FOLD, st);
         if (ufresh) {
           hipLaunchKernelGGL(k_ir_fold_p4<true>, grid4((uint32_t)l0.ld, ncol), blk, 0, st, u,
@@ -1143,7 +1150,9 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
           if (h_qn[c] > 0.0) m = std::max(m, std::sqrt(h_rr[c] / h_qn[c]));
         }
         S.max_rel_res = m;
-        if (nact == 0 || it == maxit) break;
+        if (nact == 0 || it == maxit || m > 0.5 * m_prev) break;
+        m_prev = m;
+        last_update = it;
       }
     }
   }
@@ -1517,6 +1526,7 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
   S.iters = 0;
   S.max_rel_res = 0.0;
   S.converged = 0;
+  S.fp64_iters = 0;
   std::vector<double> h_rr(ncol), h_qn(ncol);
   auto rel_max = [&](const double* d_rr, const double* d_qn) -> int {
     HIPCK(hipMemcpyAsync(h_rr.data(), d_rr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1530,6 +1540,30 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
   };
   if (o->mode == SDC_MODE_MIXED) {
     CK(pcg_reliable(h, q, u, ncol, o, S, st));
+    if (!S.converged) {
+      // fp64 continuation from the fp32 result: e solves A e = q - A u in fp64 PCG (to the
+      // residual reduction that brings the worst column to tol), then u += e
+      CK(ensure_work<double>(h, ncol));
+      Work<double>& W = h->w64;
+      const Work<float>& W32 = h->w32;
+      DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
+                                       (const double*)u, W.r, l0.ld, ncol, pf, W.partial,
+                                       W32.qmask, (__bf16*)nullptr));
+      const double tin = std::min(0.5, 0.5 * o->tol / std::max(S.max_rel_res, 1e-300));
+      int it = 0;
+      CK(pcg<double>(h, W.x, ncol, tin, o->max_iter, nullptr, &it, st));
+      S.fp64_iters = it;
+      DISPATCH_R(R, hipLaunchKernelGGL((k_ir_update<double, R>), g, blk, 0, st, u,
+                                       (const double*)W.x, l0.ld, l0.nN, ncol));
+      DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
+                                       (const double*)u, W.z, l0.ld, ncol, pf, W.partial,
+                                       W32.qmask, (__bf16*)nullptr));
+      hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
+                         W.active, 0);
+      CK(rel_max(W.orr, W32.aux));
+      S.outer += 1;
+      S.converged = S.max_rel_res <= o->tol ? 1 : 0;
+    }
   } else if (o->mode == SDC_MODE_FP64) {
     CK(ensure_work<double>(h, ncol));
     Work<double>& W = h->w64;
@@ -1591,7 +1625,7 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
       int it = 0;
       CK(pcg<float>(h, W.x, ncol, o->inner_tol, o->max_iter, W.omask, &it, st));
       S.iters += it;
-      DISPATCH_R(R, hipLaunchKernelGGL((k_ir_update<R>), g, blk, 0, st, u, (const float*)W.x,
+      DISPATCH_R(R, hipLaunchKernelGGL((k_ir_update<float, R>), g, blk, 0, st, u, (const float*)W.x,
                                        l0.ld, l0.nN, ncol));
     }
   }

@@ -23,6 +23,7 @@ class SolveResult:
     iters: int
     max_rel_res: float
     converged: bool
+    fp64_iters: int = 0
 
 
 @dataclass
@@ -188,7 +189,7 @@ class NodalEngine:
         st = _lib.SolveStats()
         rc = self.lib.sdc_solve(self.h, _lib.ptr(q), _lib.ptr(u), ncol, C.byref(o), C.byref(st),
                                 self._stream())
-        res = SolveResult(st.outer, st.iters, st.max_rel_res, bool(st.converged))
+        res = SolveResult(st.outer, st.iters, st.max_rel_res, bool(st.converged), st.fp64_iters)
         if rc != _lib.SDC_ERR_NOCONV or raise_on_fail:
             _lib.check(rc, "sdc_solve")
         return u, res

@@ -109,6 +109,8 @@ def test_mixed_solve_with_each_vcycle_storage(gpu, tiny, storage):
     u_ref = orc.fields(m)
     f = sim.fields(m)
     assert sim.last_solve.converged and sim.last_solve.max_rel_res <= 1e-10
+    # an ordinary model never needs the fp64 continuation
+    assert sim.last_solve.fp64_iters == 0, sim.last_solve
     for s in range(survey.nSrc):
         assert rel(f[:, "phiSolution"][:, s], u_ref[:, s]) < 1e-7
 
@@ -486,3 +488,25 @@ def test_thin_and_odd_meshes(gpu, shape, bc):
                 assert rel(x[:, c], ref[:, c]) < 1e-7, (interp, c)
         finally:
             Ainv.clean()
+
+
+@pytest.mark.parametrize("contrast", [1e3, 1e4, 1e5])
+def test_extreme_conductivity_contrast(gpu, tiny, contrast):
+    """A conductive block of contrast 1e3 .. 1e5 against the background (and a resistive one of
+    the inverse contrast): the mixed solve still reaches the fp64 tolerance and the direct
+    solver's potentials.  From 1e4 the fp32 recursion stops tracking the true residual; the
+    stall guard hands the solve to the fp64 continuation."""
+    mesh, survey, m = tiny
+    nx, ny, nz = mesh.shape_cells
+    s = np.full((nx, ny, nz), 1e-2)
+    s[4:8, 5:9, 3:6] = 1e-2 * contrast
+    s[9:12, 3:6, 4:8] = 1e-2 / contrast
+    mm = np.log(s.ravel(order="F"))
+    orc = oracle_for(mesh, survey)
+    u_ref = orc.fields(mm)
+    sim = _sim(mesh, survey)
+    u = sim.fields(mm)[:, "phiSolution"]
+    res = sim.last_solve
+    assert res.converged and res.max_rel_res <= 1e-10, res
+    for c in range(survey.nSrc):
+        assert rel(u[:, c], u_ref[:, c]) < 1e-7, (contrast, c, res)
