@@ -177,6 +177,8 @@ struct sdc_handle_s {
   double *tcx = nullptr, *tcy = nullptr, *tcz = nullptr, *tbd = nullptr, *tdg = nullptr,
          *tidg = nullptr;
   bool sigma_set = false;
+  // a mixed solve stalled on the current operator and parameters: later mixed solves run fp64
+  bool mixed_stalled = false;
   sdc_params prm;
   int wlevel = 0;  // level whose coarse correction runs twice (0: none)
   int tail_level = 0;    // first level of the single-launch coarse tail (0: none)
@@ -1360,6 +1362,7 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
   DeviceGuard dg(h->device);
   drop_graphs(h);
   h->prm = *p;
+  h->mixed_stalled = false;
   if (h->prm.check_every < 1) h->prm.check_every = 1;
   h->wlevel = pick_wlevel(h);
   pick_tail(h);
@@ -1444,6 +1447,7 @@ int build_levels(sdc_handle_s* h, hipStream_t st) {
                      h->ainv32, (int)nc);
   HIPCK(hipGetLastError());
   h->sigma_set = true;
+  h->mixed_stalled = false;
   return SDC_OK;
 }
 
@@ -1538,11 +1542,41 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     S.max_rel_res = m;
     return SDC_OK;
   };
-  if (o->mode == SDC_MODE_MIXED) {
+  // plain fp64 PCG from u = 0
+  auto fp64_solve = [&](int* iters) -> int {
+    CK(ensure_work<double>(h, ncol));
+    Work<double>& W = h->w64;
+    hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
+    DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, true>), g, blk, 0, st, s2, q,
+                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask,
+                                     (__bf16*)nullptr));
+    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
+                       W.active, 0);
+    CK(pcg<double>(h, u, ncol, o->tol, o->max_iter, nullptr, iters, st));
+    S.outer = 1;
+    DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
+                                     (const double*)u, W.z, l0.ld, ncol, pf, W.partial,
+                                     W.qmask, (__bf16*)nullptr));
+    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
+                       W.active, 0);
+    CK(rel_max(W.orr, W.aux));
+    // converged when every column met the PCG test (recursive residual)
+    HIPCK(hipMemcpyAsync(h->h_flag, W.counters + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    S.converged = (h->h_flag[0] == 0) ? 1 : 0;
+    return SDC_OK;
+  };
+  if (o->mode == SDC_MODE_MIXED && h->mixed_stalled) {
+    // an earlier mixed solve on this operator stalled: fp64 from the start
+    int it = 0;
+    CK(fp64_solve(&it));
+    S.fp64_iters = it;
+  } else if (o->mode == SDC_MODE_MIXED) {
     CK(pcg_reliable(h, q, u, ncol, o, S, st));
     if (!S.converged) {
       // fp64 continuation from the fp32 result: e solves A e = q - A u in fp64 PCG (to the
       // residual reduction that brings the worst column to tol), then u += e
+      h->mixed_stalled = true;
       CK(ensure_work<double>(h, ncol));
       Work<double>& W = h->w64;
       const Work<float>& W32 = h->w32;
@@ -1565,28 +1599,9 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
       S.converged = S.max_rel_res <= o->tol ? 1 : 0;
     }
   } else if (o->mode == SDC_MODE_FP64) {
-    CK(ensure_work<double>(h, ncol));
-    Work<double>& W = h->w64;
-    hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, true>), g, blk, 0, st, s2, q,
-                                     (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask,
-                                     (__bf16*)nullptr));
-    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
-                       W.active, 0);
     int it = 0;
-    CK(pcg<double>(h, u, ncol, o->tol, o->max_iter, nullptr, &it, st));
+    CK(fp64_solve(&it));
     S.iters = it;
-    S.outer = 1;
-    DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
-                                     (const double*)u, W.z, l0.ld, ncol, pf, W.partial,
-                                     W.qmask, (__bf16*)nullptr));
-    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
-                       W.active, 0);
-    CK(rel_max(W.orr, W.aux));
-    // converged when every column met the PCG test (recursive residual)
-    HIPCK(hipMemcpyAsync(h->h_flag, W.counters + 2, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCK(hipStreamSynchronize(st));
-    S.converged = (h->h_flag[0] == 0) ? 1 : 0;
   } else {
     CK(ensure_work<float>(h, ncol));
     Work<float>& W = h->w32;
