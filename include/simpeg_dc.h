@@ -44,7 +44,9 @@ extern "C" {
 #define SDC_F64 1
 
 #define SDC_MODE_MIXED 0     /* fp32 GMG-PCG with fp64 reliable updates (true residual and
-                                solution accumulation in fp64, Krylov space kept) */
+                                solution accumulation in fp64, Krylov space kept); when the fp32
+                                phase stalls, fp64 GMG-PCG continues from its result, and later
+                                mixed solves on the same operator run fp64 from the start */
 #define SDC_MODE_FP64 1      /* plain fp64 GMG-PCG */
 #define SDC_MODE_MIXED_IR 2  /* fp32 GMG-PCG restarted inside classic fp64 iterative refinement */
 
