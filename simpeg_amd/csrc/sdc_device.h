@@ -87,24 +87,19 @@ __device__ __forceinline__ NodeC<T> node_coef(const Sten<T>& s, uint32_t n) {
   c.izp = hzp ? n + s.plane : n;
   const T ax = s.cx[n], ay = s.cy[n], az = s.cz[n];
   const T bx = s.cx[c.ixm], by = s.cy[c.iym], bz = s.cz[c.izm];
-  c.wxp = hxp ? ax : T(0);
-  c.wyp = hyp ? ay : T(0);
-  c.wzp = hzp ? az : T(0);
-  c.wxm = hxm ? bx : T(0);
-  c.wym = hym ? by : T(0);
-  c.wzm = hzm ? bz : T(0);
+  // Missing and pinned edges are removed by multiplying with 0/1 masks (the clamped loads read
+  // finite coefficients), not by a branch or a select: with a branch, or a select the compiler
+  // turned into a conditional load, the coefficient loads were waited for before the operand
+  // loads were issued, one extra memory round trip per wavefront.
+  const bool z0 = s.pin0 != 0 && n == 0;
+  const bool p1 = s.pin0 == 1;
+  c.wxp = ax * (T)(hxp && !z0);
+  c.wyp = ay * (T)(hyp && !z0);
+  c.wzp = az * (T)(hzp && !z0);
+  c.wxm = bx * (T)(hxm && !(p1 && n == 1));
+  c.wym = by * (T)(hym && !(p1 && n == s.n1x));
+  c.wzm = bz * (T)(hzm && !(p1 && n == s.plane));
   c.d = s.dg[n];
-  if (s.pin0) {
-    const bool z0 = n == 0;
-    c.wxp = z0 ? T(0) : c.wxp;
-    c.wyp = z0 ? T(0) : c.wyp;
-    c.wzp = z0 ? T(0) : c.wzp;
-    if (s.pin0 == 1) {
-      c.wxm = (n == 1) ? T(0) : c.wxm;
-      c.wym = (n == s.n1x) ? T(0) : c.wym;
-      c.wzm = (n == s.plane) ? T(0) : c.wzm;
-    }
-  }
   return c;
 }
 

@@ -857,7 +857,7 @@ struct PinFix {
 // with U_ZERO the u loads are skipped (first refinement step).  The rows next to node 0 are
 // shifted into the symmetric (identity row and column 0) form the inner PCG solves.
 // Partial sums of |r|^2 (before the shift) per column.
-// qmask (per node, may be null): with U_ZERO it is set to 1 on every node where some column of
+// qmask (per node, never null): with U_ZERO it is set to 1 on every node where some column of
 // q is non-zero (the caller clears it first); otherwise q is read only on flagged nodes.  A
 // right-hand side of point sources touches a few nodes, so the fp64 residual then streams u.
 template <typename TO, int R, bool U_ZERO>
@@ -881,12 +881,15 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
       else if (n == pf.n1x) wfix = pf.cy[0];
       else if (n == pf.plane) wfix = pf.cz[0];
     }
-    const bool readq = U_ZERO || qmask == nullptr || qmask[n] != 0u;
-    // all loads of the R columns first (clamped columns, results discarded), then arithmetic
+    // Load order: the mask word, then the 7R stencil loads of u, then the q loads that depend
+    // on the mask.  Issued the other way round, the branch on the mask held the stencil loads
+    // back by one memory round trip per wavefront.
+    const unsigned qm = U_ZERO ? 1u : qmask[n];
     double qv[R], uc[R], au[R];
+    if (!U_ZERO) stencil_cols<double, R>(c, u, n, ld, c0, ncol, uc, au);
+    const bool readq = qm != 0u;
 #pragma unroll
     for (int r = 0; r < R; ++r) qv[r] = readq ? ld_once(col_ptr(q, ld, c0, r, ncol) + n) : 0.0;
-    if (!U_ZERO) stencil_cols<double, R>(c, u, n, ld, c0, ncol, uc, au);
     bool anyq = false;
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -904,7 +907,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
         if (r16) r16[o + n] = (__bf16)rv;
       }
     // every writer stores the same value; the mask is cleared before this pass
-    if (U_ZERO && qmask != nullptr && anyq) qmask[n] = 1u;
+    if (U_ZERO && anyq) qmask[n] = 1u;
   }
   block_partials<R>(acc, partial, c0, ncol, nbx, bx);
 }
