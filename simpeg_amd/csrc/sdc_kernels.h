@@ -383,7 +383,10 @@ struct AxisXfer {
   int remap;
 };
 
-template <typename T, typename X, int AX, int R>
+// OPD: operator-dependent weights wop (else the geometric wlo / whi alone).  The weight loads
+// are unconditional (clamped indices) and the missing neighbours are masked by multiplying, so
+// they go out with the value loads in the second round trip after the fi lookup.
+template <typename T, typename X, int AX, int R, bool OPD>
 __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const int* __restrict__ fi,
                                                              const T* __restrict__ wlo,
                                                              const T* __restrict__ whi,
@@ -412,9 +415,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const i
   // operator-dependent weights (wop, over this pass's input grid): the left neighbour gives
   // its weight of the right coarse node, the right neighbour 1 minus its own
   T wl = wlo[C], wh = whi[C];
-  if (wop) {
-    wl = wl != T(0) ? wop[il] : T(0);
-    wh = wh != T(0) ? T(1) - wop[ih] : T(0);
+  if (OPD) {
+    wl = wop[il] * (T)(wl != T(0));
+    wh = (T(1) - wop[ih]) * (T)(wh != T(0));
   }
   T v[R][3];
 #pragma unroll
@@ -430,10 +433,14 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const i
       st_once(out + (size_t)(c0 + r) * ldo + n, (X)(wl * v[r][0] + v[r][1] + wh * v[r][2]));
 }
 
-// xf += P xc.  Every fine node reads the 2x2x2 coarse nodes around it (prolong_weights); a
-// coincident axis has weight 0 on its second node, whose index still lies inside the coarse
-// grid.
-template <typename T, typename X, int R>
+// xf += P xc.  Every fine node reads the 2x2x2 coarse nodes around it (weights as in
+// prolong_weights); a coincident axis has weight 0 on its second node, whose index still lies
+// inside the coarse grid.  OPD: operator-dependent weights (t.wx, t.wy, t.wz), else the
+// geometric tables.
+// Load order: the parent index tables, the node's own x weight and the R fine values first
+// (none depends on another load), then the weights and coarse values addressed through the
+// parents -- two memory round trips per wavefront.
+template <typename T, typename X, int R, bool OPD>
 __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* __restrict__ xc,
                                                            X* __restrict__ xf, size_t ldf,
                                                            size_t ldc, int ncol,
@@ -445,12 +452,28 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* _
   const uint32_t i = n - rr * t.fn1x;
   const uint32_t k = fdiv(rr, t.ffy);
   const uint32_t j = rr - k * t.fn1y;
-  const uint32_t I0 = (uint32_t)t.pix[i], J0 = (uint32_t)t.piy[j];
-  T w[8];
-  prolong_weights(t, n, i, j, k, I0, J0, w);
-  const uint32_t c000 = I0 + J0 * t.cn1x + (uint32_t)t.piz[k] * t.cplane;
+  const uint32_t I0 = (uint32_t)t.pix[i], J0 = (uint32_t)t.piy[j], K0 = (uint32_t)t.piz[k];
+  const T fx = OPD ? t.wx[n] : t.pwx[i];
+  T fv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) fv[r] = (T)col_ptr((const X*)xf, ldf, c0, r, ncol)[n];
+  T fy[2], fz[2][2];
+  if (OPD) {
+    const uint32_t py = I0 + t.cn1x * (j + t.fn1y * k);
+    fy[0] = t.wy[py];
+    fy[1] = t.wy[py + 1];
+    const uint32_t pz = I0 + t.cn1x * J0 + t.cplane * k;
+    fz[0][0] = t.wz[pz];
+    fz[1][0] = t.wz[pz + 1];
+    fz[0][1] = t.wz[pz + t.cn1x];
+    fz[1][1] = t.wz[pz + t.cn1x + 1];
+  } else {
+    fy[0] = fy[1] = t.pwy[j];
+    fz[0][0] = fz[1][0] = fz[0][1] = fz[1][1] = t.pwz[k];
+  }
+  const uint32_t c000 = I0 + J0 * t.cn1x + K0 * t.cplane;
   const uint32_t c010 = c000 + t.cn1x, c001 = c000 + t.cplane, c011 = c001 + t.cn1x;
-  T cv[R][8], fv[R];
+  T cv[R][8];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const X* c = col_ptr(xc, ldc, c0, r, ncol);
@@ -462,8 +485,16 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* _
     cv[r][5] = (T)c[c001 + 1];
     cv[r][6] = (T)c[c011];
     cv[r][7] = (T)c[c011 + 1];
-    fv[r] = (T)col_ptr((const X*)xf, ldf, c0, r, ncol)[n];
   }
+  T w[8];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        w[a + 2 * b + 4 * c] = (a ? fx : T(1) - fx) * (b ? fy[a] : T(1) - fy[a]) *
+                               (c ? fz[a][b] : T(1) - fz[a][b]);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (c0 + r < ncol) {

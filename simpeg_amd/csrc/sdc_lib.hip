@@ -809,20 +809,18 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
         const T* wop = a == 0 ? t.wx : a == 1 ? t.wy : t.wz;
         const dim3 ga = grid2(ax.nout, ncol, RX);
         prof_begin(h, rtag, st);
-        switch (a) {
-          case 0:
-            DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 0, R>), ga, blk, 0, st, ax, fi,
-                                              wl, wh, wop, src, dst, lv.ld, ldo, ncol, done));
-            break;
-          case 1:
-            DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 1, R>), ga, blk, 0, st, ax, fi,
-                                              wl, wh, wop, src, dst, lv.ld, ldo, ncol, done));
-            break;
-          default:
-            DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, 2, R>), ga, blk, 0, st, ax, fi,
-                                              wl, wh, wop, src, dst, lv.ld, ldo, ncol, done));
-            break;
+#define SDC_RESTRICT_AXIS(AXN, OPD)                                                          \
+  DISPATCH_R(RX, launch(h, (k_restrict_axis<T, S, AXN, R, OPD>), ga, blk, 0, st, ax, fi, wl, \
+                        wh, wop, src, dst, lv.ld, ldo, ncol, done))
+        switch (a * 2 + (wop != nullptr)) {
+          case 0: SDC_RESTRICT_AXIS(0, false); break;
+          case 1: SDC_RESTRICT_AXIS(0, true); break;
+          case 2: SDC_RESTRICT_AXIS(1, false); break;
+          case 3: SDC_RESTRICT_AXIS(1, true); break;
+          case 4: SDC_RESTRICT_AXIS(2, false); break;
+          default: SDC_RESTRICT_AXIS(2, true); break;
         }
+#undef SDC_RESTRICT_AXIS
         prof_end(h, rtag, st, ((double)ax.a0 * ax.a1 * ax.a2 + (double)ax.nout) * cols * sS);
         src = dst;
       }
@@ -835,8 +833,13 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
     }
     CK((vcycle<T, S, S, S>(h, l + 1, cb, cx, ncol, false, done, st)));
     prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
-    DISPATCH_R(RX, launch(h, (k_prolong_add<T, S, R>), grid2(lv.nN, ncol, RX), blk, 0, st,
-                                      t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
+    if (t.wx) {
+      DISPATCH_R(RX, launch(h, (k_prolong_add<T, S, R, true>), grid2(lv.nN, ncol, RX), blk, 0,
+                            st, t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
+    } else {
+      DISPATCH_R(RX, launch(h, (k_prolong_add<T, S, R, false>), grid2(lv.nN, ncol, RX), blk, 0,
+                            st, t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
+    }
     prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sS);
     if (gam + 1 < ngam) post_smooth(false);
   }
