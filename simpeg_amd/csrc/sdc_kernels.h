@@ -1283,18 +1283,68 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ a, d
 // (the derivative of u^T A(sigma) lambda with respect to the cell value sigma_c).
 // (per-column output for getJ, weighted squares for getJtJdiag, or the sum over columns)
 
+// columns per work-item of the adjoint contraction
+#define SDC_JT_COLS 8
+
+// per-cell factors of the contraction: edge weights hy hz / 4hx ... and the Robin face weights
+struct JtCell {
+  double fxw, fyw, fzw, bx0, bx1, by0, by1, bz0, bz1;
+};
+
+// g_c of one column from the 8 corner values of u and lambda (corner m = a + 2b + 4c)
+__device__ __forceinline__ double jt_cell_value(const JtCell& w, bool robin, const double (&uv)[8],
+                                                const double (&lv)[8]) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // x edges: pairs (0,1),(2,3),(4,5),(6,7)
+    sx += (uv[2 * q + 1] - uv[2 * q]) * (lv[2 * q + 1] - lv[2 * q]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // y edges: (0,2),(1,3),(4,6),(5,7)
+    const int m0 = (q & 1) + 4 * (q >> 1);
+    sy += (uv[m0 + 2] - uv[m0]) * (lv[m0 + 2] - lv[m0]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // z edges: (m, m+4)
+    sz += (uv[q + 4] - uv[q]) * (lv[q + 4] - lv[q]);
+  }
+  double gval = w.fxw * sx + w.fyw * sy + w.fzw * sz;
+  if (robin) {
+    double ul[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) ul[m] = uv[m] * lv[m];
+    gval += w.bx0 * (ul[0] + ul[2] + ul[4] + ul[6]) + w.bx1 * (ul[1] + ul[3] + ul[5] + ul[7]);
+    gval += w.by0 * (ul[0] + ul[1] + ul[4] + ul[5]) + w.by1 * (ul[2] + ul[3] + ul[6] + ul[7]);
+    gval += w.bz0 * (ul[0] + ul[1] + ul[2] + ul[3]) + w.bz1 * (ul[4] + ul[5] + ul[6] + ul[7]);
+  }
+  return gval;
+}
+
 // MODE 0: out[c] = sum_j g_j;  MODE 1: out[j*ldo + c] = g_j;
 // MODE 2: out[c] += sum_j wsq[j] g_j^2 (diagonal of J^T W^2 J before the map derivative).
+// Grid (cell blocks, column groups of SDC_JT_COLS columns): a work-item sums its group's
+// columns.  The column groups keep the work-items that share a node column in flight
+// together, so the 8 cells around a node read it from L2 (one long loop over every column let
+// the cell planes drift apart and fetched the shared node planes from HBM ~2.7 times).  With several groups,
+// MODE 0 / 2 write one partial per group to part[g*nC + c], summed in group order by
+// k_jtvec_reduce (deterministic); with one group they write out directly.
 template <int MODE>
 __global__ __launch_bounds__(SDC_BLOCK) void k_jtvec(Geo g, const double* __restrict__ u,
                                                      const double* __restrict__ lam, size_t ld,
                                                      int ncol, const int* __restrict__ ucol,
                                                      double* __restrict__ out, size_t ldo,
-                                                     const double* __restrict__ wsq) {
+                                                     const double* __restrict__ wsq,
+                                                     double* __restrict__ part) {
   const uint32_t nC = g.nx * g.ny * g.nz;
   const uint32_t nbx = gridDim.x;
   const uint32_t c = xcd_remap(blockIdx.x, nbx) * SDC_BLOCK + threadIdx.x;
   if (c >= nC) return;
+  constexpr int G = SDC_JT_COLS;
+  const int cbeg = (int)blockIdx.y * G;
+  const int nc = min(ncol - cbeg, G);
   const uint32_t r = fdiv(c, g.cfx);
   const uint32_t i = c - r * g.nx;
   const uint32_t k = fdiv(r, g.cfy);
@@ -1303,64 +1353,80 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_jtvec(Geo g, const double* __rest
   const double fxw = hy * hz / (4.0 * hx), fyw = hx * hz / (4.0 * hy), fzw = hx * hy / (4.0 * hz);
   double bx0 = 0.0, bx1 = 0.0, by0 = 0.0, by1 = 0.0, bz0 = 0.0, bz1 = 0.0;
   if (g.robin) {
-    if (i == 0) bx0 = g.bxm[j + g.ny * k];
-    if (i == g.nx - 1) bx1 = g.bxp[j + g.ny * k];
-    if (j == 0) by0 = g.bym[i + g.nx * k];
-    if (j == g.ny - 1) by1 = g.byp[i + g.nx * k];
-    if (k == 0) bz0 = g.bzm[i + g.nx * j];
-    if (k == g.nz - 1) bz1 = g.bzp[i + g.nx * j];
+    // unconditional loads (the face index is in range for every cell) masked by multiplying,
+    // so that they go out with the first column's loads
+    bx0 = g.bxm[j + g.ny * k] * (double)(i == 0);
+    bx1 = g.bxp[j + g.ny * k] * (double)(i == g.nx - 1);
+    by0 = g.bym[i + g.nx * k] * (double)(j == 0);
+    by1 = g.byp[i + g.nx * k] * (double)(j == g.ny - 1);
+    bz0 = g.bzm[i + g.nx * j] * (double)(k == 0);
+    bz1 = g.bzp[i + g.nx * j] * (double)(k == g.nz - 1);
   }
   const size_t n000 = (size_t)i + (size_t)j * g.n1x + (size_t)k * g.plane;
   size_t off[8];
   for (int cc = 0; cc < 2; ++cc)
     for (int b = 0; b < 2; ++b)
       for (int a = 0; a < 2; ++a) off[a + 2 * b + 4 * cc] = n000 + a + b * g.n1x + cc * g.plane;
+  // the group's column indices and weights first, so that no load in the loop waits on
+  // another; then two columns per step (32 loads in flight per work-item)
+  int scol[G];
+  double wq[G];
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    const int col = cbeg + (q < nc ? q : nc - 1);
+    scol[q] = ucol ? ucol[col] : col;
+    wq[q] = MODE == 2 ? wsq[col] : 0.0;
+  }
+  const JtCell jc{fxw, fyw, fzw, bx0, bx1, by0, by1, bz0, bz1};
   double total = 0.0;
-  for (int col = 0; col < ncol; ++col) {
-    const int sc = ucol ? ucol[col] : col;
-    const double* uc = u + (size_t)sc * ld;
-    const double* lc = lam + (size_t)col * ld;
-    double uv[8], lv[8];
+#pragma unroll
+  for (int q = 0; q < G; q += 2) {
+    if (q >= nc) break;
+    const int qb = q + 1 < nc ? q + 1 : q;
+    const double* ua = u + (size_t)scol[q] * ld;
+    const double* la = lam + (size_t)(cbeg + q) * ld;
+    const double* ub = u + (size_t)scol[qb] * ld;
+    const double* lb = lam + (size_t)(cbeg + qb) * ld;
+    double uva[8], lva[8], uvb[8], lvb[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      uv[m] = uc[off[m]];
-      lv[m] = lc[off[m]];
+      uva[m] = ua[off[m]];
+      lva[m] = la[off[m]];
+      uvb[m] = ub[off[m]];
+      lvb[m] = lb[off[m]];
     }
-    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const double gv[2] = {jt_cell_value(jc, g.robin, uva, lva), jt_cell_value(jc, g.robin, uvb, lvb)};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      // x edges: pairs (0,1),(2,3),(4,5),(6,7)
-      sx += (uv[2 * q + 1] - uv[2 * q]) * (lv[2 * q + 1] - lv[2 * q]);
+    for (int e = 0; e < 2; ++e) {
+      if (q + e >= nc) break;
+      if (MODE == 1)
+        out[(size_t)(cbeg + q + e) * ldo + c] = gv[e];
+      else if (MODE == 2)
+        total += wq[q + e] * gv[e] * gv[e];
+      else
+        total += gv[e];
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      // y edges: (0,2),(1,3),(4,6),(5,7)
-      const int m0 = (q & 1) + 4 * (q >> 1);
-      sy += (uv[m0 + 2] - uv[m0]) * (lv[m0 + 2] - lv[m0]);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      // z edges: (m, m+4)
-      sz += (uv[q + 4] - uv[q]) * (lv[q + 4] - lv[q]);
-    }
-    double gval = fxw * sx + fyw * sy + fzw * sz;
-    if (g.robin) {
-      double ul[8];
-#pragma unroll
-      for (int m = 0; m < 8; ++m) ul[m] = uv[m] * lv[m];
-      gval += bx0 * (ul[0] + ul[2] + ul[4] + ul[6]) + bx1 * (ul[1] + ul[3] + ul[5] + ul[7]);
-      gval += by0 * (ul[0] + ul[1] + ul[4] + ul[5]) + by1 * (ul[2] + ul[3] + ul[6] + ul[7]);
-      gval += bz0 * (ul[0] + ul[1] + ul[2] + ul[3]) + bz1 * (ul[4] + ul[5] + ul[6] + ul[7]);
-    }
-    if (MODE == 1)
-      out[(size_t)col * ldo + c] = gval;
-    else if (MODE == 2)
-      total += wsq[col] * gval * gval;
-    else
-      total += gval;
   }
-  if (MODE == 0) out[c] = total;
-  if (MODE == 2) out[c] += total;
+  if (MODE != 1) {
+    if (part)
+      part[(size_t)blockIdx.y * nC + c] = total;
+    else if (MODE == 0)
+      out[c] = total;
+    else
+      out[c] += total;
+  }
+}
+
+// out[c] (ACC: out[c] +) = sum over the ngroups partials of cell c, in group order
+template <bool ACC>
+__global__ __launch_bounds__(SDC_BLOCK) void k_jtvec_reduce(const double* __restrict__ part,
+                                                            int ngroups, uint32_t nC,
+                                                            double* __restrict__ out) {
+  const uint32_t c = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (c >= nC) return;
+  double t = 0.0;
+  for (int g = 0; g < ngroups; ++g) t += ld_once(part + (size_t)g * nC + c);
+  out[c] = ACC ? out[c] + t : t;
 }
 
 // ------------------------------------------------------------------------------------------

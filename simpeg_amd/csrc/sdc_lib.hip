@@ -176,6 +176,9 @@ struct sdc_handle_s {
   // scratch coefficient arrays for A(dsigma) (Jvec right-hand side)
   double *tcx = nullptr, *tcy = nullptr, *tcz = nullptr, *tbd = nullptr, *tdg = nullptr,
          *tidg = nullptr;
+  // per-column-group partial sums of the adjoint contraction (jpart_cap doubles)
+  double* jpart = nullptr;
+  size_t jpart_cap = 0;
   bool sigma_set = false;
   // a mixed solve stalled on the current operator and parameters: later mixed solves run fp64
   bool mixed_stalled = false;
@@ -1171,6 +1174,21 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
 
 namespace {
 int build_levels(sdc_handle_s* h, hipStream_t st);
+
+// adjoint contraction: columns per work-item group, and the partial-sum buffer for ng groups
+constexpr int kJtvecGroupCols = SDC_JT_COLS;
+int jtvec_groups(int ncol) { return (ncol + kJtvecGroupCols - 1) / kJtvecGroupCols; }
+int jtvec_partials(sdc_handle_s* h, int ng, size_t nC, double** part) {
+  const size_t need = (size_t)ng * nC;
+  if (h->jpart_cap < need) {
+    dfree(h->jpart);
+    h->jpart_cap = 0;
+    CK(dmalloc(&h->jpart, need));
+    h->jpart_cap = need;
+  }
+  *part = h->jpart;
+  return SDC_OK;
+}
 }  // namespace
 
 // ============================================================================================
@@ -1317,6 +1335,7 @@ int sdc_destroy(sdc_handle h) {
   for (auto& v : h->L) v.release();
   dfree(h->ainv64); dfree(h->ainv32); dfree(h->gj_a); dfree(h->gj_f);
   dfree(h->tcx); dfree(h->tcy); dfree(h->tcz); dfree(h->tbd); dfree(h->tdg); dfree(h->tidg);
+  dfree(h->jpart);
   h->w32.release();
   h->w64.release();
   if (h->h_flag) (void)hipHostFree(h->h_flag);
@@ -1696,15 +1715,23 @@ int sdc_jtvec_contract(sdc_handle h, const double* u, const double* lam, int nco
   hipStream_t st = (hipStream_t)stream;
   const Geo geo = make_geo(h);
   const Level& l0 = h->L[0];
+  const int ng = jtvec_groups(ncol);
+  double* part = nullptr;
+  if (!per_column && ng > 1) CK(jtvec_partials(h, ng, (size_t)nC, &part));
+  const dim3 grid(nblk(nC), (uint32_t)ng);
   prof_begin(h, SDC_PROF_JTVEC, st);
   if (per_column)
-    hipLaunchKernelGGL(k_jtvec<1>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
-                       ncol, (const int*)ucol, out, (size_t)ldo, (const double*)nullptr);
+    hipLaunchKernelGGL(k_jtvec<1>, grid, dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld, ncol,
+                       (const int*)ucol, out, (size_t)ldo, (const double*)nullptr,
+                       (double*)nullptr);
   else
-    hipLaunchKernelGGL(k_jtvec<0>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld,
-                       ncol, (const int*)ucol, out, (size_t)ldo, (const double*)nullptr);
+    hipLaunchKernelGGL(k_jtvec<0>, grid, dim3(SDC_BLOCK), 0, st, geo, u, lam, l0.ld, ncol,
+                       (const int*)ucol, out, (size_t)ldo, (const double*)nullptr, part);
   prof_end(h, SDC_PROF_JTVEC, st,
            (double)l0.nN * ncol * 16 + (double)nC * 8 * (per_column ? ncol : 1));
+  if (part)
+    hipLaunchKernelGGL(k_jtvec_reduce<false>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st,
+                       (const double*)part, ng, (uint32_t)nC, out);
   HIPCK(hipGetLastError());
   return SDC_OK;
 }
@@ -1716,8 +1743,14 @@ int sdc_jtj_diag_accumulate(sdc_handle h, const double* u, const double* lam, in
   DeviceGuard dg(h->device);
   hipStream_t st = (hipStream_t)stream;
   const Geo geo = make_geo(h);
-  hipLaunchKernelGGL(k_jtvec<2>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st, geo, u, lam,
-                     h->L[0].ld, ncol, (const int*)ucol, out, (size_t)nC, wsq);
+  const int ng = jtvec_groups(ncol);
+  double* part = nullptr;
+  if (ng > 1) CK(jtvec_partials(h, ng, (size_t)nC, &part));
+  hipLaunchKernelGGL(k_jtvec<2>, dim3(nblk(nC), (uint32_t)ng), dim3(SDC_BLOCK), 0, st, geo, u,
+                     lam, h->L[0].ld, ncol, (const int*)ucol, out, (size_t)nC, wsq, part);
+  if (part)
+    hipLaunchKernelGGL(k_jtvec_reduce<true>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st,
+                       (const double*)part, ng, (uint32_t)nC, out);
   HIPCK(hipGetLastError());
   return SDC_OK;
 }
