@@ -522,3 +522,39 @@ def test_extreme_conductivity_contrast(gpu, tiny, contrast):
     if res.fp64_iters:
         assert adj.iters == 0 and adj.fp64_iters > 0, adj
     assert rel(jt, orc.Jtvec(mm, w, u=u_ref)) < 1e-6
+
+
+@pytest.mark.parametrize("bc", ["Robin", "Neumann"])
+def test_adjoint_contraction_column_groups(gpu, tiny, bc):
+    """sdc_jtvec_contract and sdc_jtj_diag_accumulate against the oracle's getADeriv adjoint
+    (map derivative = identity) for column counts below, at and across the kernel's 8-column
+    groups (1, 8, 9, 17), each lambda column paired with a drawn u column: the sum over columns,
+    one row per column, and the W^2-weighted squares added onto existing values."""
+    import scipy.sparse as sp
+
+    mesh, survey, m = tiny
+    sim = _sim(mesh, survey, bc)
+    orc = oracle_for(mesh, survey, bc)
+    orc.set_model(m)
+    orc.sigma_deriv = sp.identity(mesh.n_cells, format="csr")
+    sim.model = m
+    sim._ensure_sigma()
+    eng = sim.engine
+    rng = np.random.default_rng(7)
+    nu = 5
+    U = rng.standard_normal((mesh.n_nodes, nu))
+    ub = eng.to_block(U)
+    for ncol in (1, 8, 9, 17):
+        L = rng.standard_normal((mesh.n_nodes, ncol))
+        ucol = rng.integers(0, nu, ncol)
+        ref = np.stack([orc.getADeriv(U[:, ucol[j]], L[:, j], adjoint=True) for j in range(ncol)])
+        lb = eng.to_block(L)
+        got = eng.contract(ub, lb, ncol, ucol=ucol).cpu().numpy()
+        assert rel(got, ref.sum(0)) < 1e-12, ncol
+        per = eng.contract(ub, lb, ncol, ucol=ucol, per_column=True).cpu().numpy()
+        assert rel(per, ref) < 1e-12, ncol
+        w = rng.random(ncol)
+        base = rng.random(mesh.n_cells)
+        out = torch.as_tensor(base).to(eng.device)
+        eng.jtj_diag_accumulate(ub, lb, ncol, ucol, w, out)
+        assert rel(out.cpu().numpy(), base + (w[:, None] * ref ** 2).sum(0)) < 1e-12, ncol
