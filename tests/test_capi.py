@@ -75,3 +75,41 @@ def test_null_handle_calls():
     assert lib.sdc_set_sigma(None, None, None) == _lib.SDC_ERR_ARG
     assert lib.sdc_set_coefficients(None, None, None, None, None, None) == _lib.SDC_ERR_ARG
     assert lib.sdc_solve(None, None, None, 1, None, None, None) == _lib.SDC_ERR_ARG
+
+
+def _header_struct_fields(name):
+    """(field, C type) of `typedef struct { ... } name;` in declaration order."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\}\s*" + name + r"\s*;", text).group(1)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"((?:const\s+)?[A-Za-z_0-9]+\s*\**)\s*(.*)$", decl)
+        ctype = m.group(1).replace(" ", "")
+        for var in m.group(2).split(","):
+            var = var.strip()
+            stars = len(var) - len(var.lstrip("*"))
+            fields.append((var.lstrip("*").strip(), ctype + "*" * stars))
+    return fields
+
+
+_CTYPE = {"int32_t": C.c_int32, "double": C.c_double, "uint32_t": C.c_uint32}
+
+
+@pytest.mark.parametrize("cname,pyname", [("sdc_mesh_desc", "MeshDesc"), ("sdc_params", "Params"),
+                                          ("sdc_solve_opts", "SolveOpts"),
+                                          ("sdc_solve_stats", "SolveStats")])
+def test_ctypes_structs_mirror_the_header(cname, pyname):
+    """Every struct the C ABI passes has the same fields, in the same order and of the same
+    scalar types, in its ctypes mirror (a field added on one side only shifts the others)."""
+    hdr = _header_struct_fields(cname)
+    py = getattr(_lib, pyname)._fields_
+    assert [f for f, _ in hdr] == [f for f, _ in py]
+    for (f, ct), (_, pt) in zip(hdr, py):
+        if ct in _CTYPE:
+            assert pt is _CTYPE[ct], (f, ct, pt)
+        else:
+            assert ct.endswith("*"), (f, ct)
+            assert issubclass(pt, (C._Pointer, C.c_void_p)) or pt is C.c_void_p, (f, ct, pt)
