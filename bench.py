@@ -272,6 +272,20 @@ def cpu_baseline(budget_s=12.0):
     }
 
 
+def _range(name):
+    """Open (name) or close (None) a named range in the profiler's marker trace (roctx through
+    torch.cuda.nvtx), so that a kernel trace can be split into the timed phases."""
+    import torch
+
+    try:
+        if name is None:
+            torch.cuda.nvtx.range_pop()
+        else:
+            torch.cuda.nvtx.range_push(name)
+    except Exception:
+        pass
+
+
 def bench_batch(args):
     """48 independent problems (uneven sizes / source counts) behind MetaSimulation: fields +
     Jtvec for all of them, one stream after the other vs concurrently on 8 streams.  Multigrid
@@ -300,10 +314,14 @@ def bench_batch(args):
         meta.Jtvec(m, v, f=f)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        _range(f"batch fields streams={ns}")
         f = meta.fields(m + 1e-3)  # a new model: every problem rebuilds its operator
+        _range(None)
         t1 = time.perf_counter()
+        _range(f"batch jtvec streams={ns}")
         meta.Jtvec(m + 1e-3, v, f=f)
         torch.cuda.synchronize()
+        _range(None)
         t2 = time.perf_counter()
         res[ns] = (t1 - t0, t2 - t1)
         log(f"[bench] batch streams={ns}: fields {1e3*(t1-t0):.1f} ms, Jtvec {1e3*(t2-t1):.1f} ms")
