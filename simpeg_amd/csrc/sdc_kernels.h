@@ -1284,7 +1284,9 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ a, d
 // (per-column output for getJ, weighted squares for getJtJdiag, or the sum over columns)
 
 // columns per work-item of the adjoint contraction
+#ifndef SDC_JT_COLS
 #define SDC_JT_COLS 8
+#endif
 
 // per-cell factors of the contraction: edge weights hy hz / 4hx ... and the Robin face weights
 struct JtCell {
