@@ -28,8 +28,9 @@ Reported:
   cpu_baseline     the FP64 CPU oracle (scipy splu, the reference's solver without MKL/MUMPS)
                    on a bounded sample (see "sample")
 
-Other workloads: --workload small (the 24x24x12-core block model), batch (48 uneven problems
-behind MetaSimulation), gn (Gauss-Newton on the 160x160x96-core layered model).
+Other workloads: --workload small (the 24x24x12-core block model), layered (the same step on the
+160x160x96-core layered model, 3.29M nodes and 256 sources), batch (48 uneven problems behind
+MetaSimulation), gn (Gauss-Newton on the layered model).
 """
 
 from __future__ import annotations
@@ -60,6 +61,8 @@ def build_problem(name, seed=0):
         return W.large_grf_model(seed)
     if name == "small":
         return W.small_block_model(seed)
+    if name == "layered":
+        return W.layered_ellipsoid_model(seed)
     raise ValueError(name)
 
 
@@ -385,7 +388,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="large", choices=["large", "small", "batch", "gn"])
+    ap.add_argument("--workload", default="large", choices=["large", "small", "layered", "batch", "gn"])
     ap.add_argument("--gn-beta", type=float, default=1e-2)
     ap.add_argument("--gn-store-fields", action="store_true",
                     help="keep the source fields between Hessian products (default: recompute "
@@ -472,7 +475,8 @@ def main():
     ms_step = 1e3 * (t_fwd + t_adj) / K
     jt_ms = 1e3 * t_adj / K
     out = {
-        "metric": "source-solves/s + Jtvec ms at 96x96x64/128 src, mixed vs FP64 GPU @1e-6",
+        "metric": ("source-solves/s + Jtvec ms at 96x96x64/128 src, mixed vs FP64 GPU @1e-6"
+                   if args.workload == "large" else f"source-solves/s + Jtvec ms on {wl.name}"),
         "value": round(value, 2),
         "unit": "source-solves/s",
         "n_gpus": 1,
