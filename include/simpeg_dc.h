@@ -112,6 +112,11 @@ typedef struct {
                                   the same iteration counts) */
   int32_t nu_coarse;           /* Chebyshev steps before and after the coarse correction on the
                                   levels below the finest (0 = nu_pre / nu_post) */
+  int32_t deriv_robin;         /* Robin handles: 1 (default) = the derivative entry points
+                                  (sdc_jvec_rhs, sdc_jtvec_contract, sdc_jtj_diag_accumulate)
+                                  include the boundary term d(AvgBC sigma)/dsigma; 0 = they leave
+                                  it out, as the reference's getADeriv does when the model is a
+                                  resistivity (rhoMap; resistivity/simulation.py:566-570) */
 } sdc_params;
 
 typedef struct {

@@ -356,12 +356,18 @@ class OracleDC:
     (``simpeg/survey.py:592-627``).
     """
 
-    def __init__(self, mesh: OracleMesh, sources, mapping=None, bc_type="Robin", surface_faces=None):
+    def __init__(self, mesh: OracleMesh, sources, mapping=None, bc_type="Robin", surface_faces=None,
+                 param="sigma"):
         if bc_type not in ("Robin", "Mixed", "Neumann"):
             raise ValueError("bc_type must be 'Robin' or 'Neumann' for the nodal formulation")
+        if param not in ("sigma", "rho"):
+            raise ValueError("param must be 'sigma' (sigmaMap) or 'rho' (rhoMap)")
         self.mesh = mesh
         self.sources = sources
         self.mapping = mapping if mapping is not None else OracleExpMap()
+        # "rho": the mapping gives the resistivity (rhoMap); sigma = 1/rho through the reciprocal
+        # relation of props.py:47-56, 139-157
+        self.param = param
         self.bc_type = "Neumann" if bc_type == "Neumann" else "Robin"
         self.G = mesh.nodal_gradient
         self.Av = mesh.edge_cell_weights
@@ -388,8 +394,12 @@ class OracleDC:
         m = np.asarray(m, dtype=float)
         if self.model is None or not np.allclose(self.model, m):
             self.model = m.copy()
-            self.sigma = self.mapping.transform(m)
-            self.sigma_deriv = self.mapping.deriv(m)
+            if self.param == "rho":
+                self.sigma = 1.0 / self.mapping.transform(m)
+                self.sigma_deriv = (sp.diags(-self.sigma * self.sigma) @ self.mapping.deriv(m)).tocsr()
+            else:
+                self.sigma = self.mapping.transform(m)
+                self.sigma_deriv = self.mapping.deriv(m)
             self._lu = None
 
     def getA(self, sigma=None):
@@ -406,7 +416,10 @@ class OracleDC:
         return A
 
     def getADeriv(self, u, v, adjoint=False):
-        """``Simulation3DNodal.getADeriv`` (557-571) with ``MeSigmaDeriv`` (pde_simulation.py:296-321)."""
+        """``Simulation3DNodal.getADeriv`` (557-571) with ``MeSigmaDeriv`` (pde_simulation.py:296-321).
+
+        The Robin term is added only for a conductivity model (``sigmaMap is not None``, :566): a
+        resistivity model (``param="rho"``, rhoMap) leaves it out, as the reference does."""
         Gu = self.G @ u
         M = self.Av @ self.sigma_deriv
         if not adjoint:
@@ -417,7 +430,7 @@ class OracleDC:
                 out = M.T @ (Gu[:, None] * Gv)
             else:
                 out = M.T @ (Gu * Gv)
-        if self.bc_type != "Neumann":
+        if self.bc_type != "Neumann" and self.param == "sigma":
             MBC = self.AvgBC @ self.sigma_deriv
             if not adjoint:
                 out = out + u * (MBC @ v)

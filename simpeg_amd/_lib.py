@@ -65,6 +65,7 @@ class Params(C.Structure):
         ("interp", C.c_int32),
         ("wcycle_smooth", C.c_int32),
         ("nu_coarse", C.c_int32),
+        ("deriv_robin", C.c_int32),
     ]
 
 

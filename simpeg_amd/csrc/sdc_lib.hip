@@ -354,11 +354,13 @@ Xfer<T> make_xfer(const sdc_handle_s* h, int l) {
   return t;
 }
 
-Geo make_geo(const sdc_handle_s* h) {
+// deriv: geometry for a derivative kernel (dA/dsigma products), whose Robin term follows
+// sdc_params.deriv_robin
+Geo make_geo(const sdc_handle_s* h, bool deriv = false) {
   Geo g;
   g.hx = h->hx; g.hy = h->hy; g.hz = h->hz;
   g.bxm = h->bxm; g.bxp = h->bxp; g.bym = h->bym; g.byp = h->byp; g.bzm = h->bzm; g.bzp = h->bzp;
-  g.robin = h->bc == SDC_BC_ROBIN;
+  g.robin = h->bc == SDC_BC_ROBIN && (!deriv || h->prm.deriv_robin != 0);
   g.nx = h->nx; g.ny = h->ny; g.nz = h->nz;
   g.n1x = h->nx + 1; g.n1y = h->ny + 1; g.nN = h->L[0].nN; g.plane = g.n1x * g.n1y;
   g.fx = make_fastdiv(g.n1x); g.fy = make_fastdiv(g.n1y);
@@ -1218,6 +1220,7 @@ int sdc_default_params(sdc_params* p) {
   p->interp = 1;
   p->wcycle_smooth = 1;
   p->nu_coarse = 0;
+  p->deriv_robin = 1;
   p->check_every = 1;
   p->reliable_delta = 1e-4;
   return SDC_OK;
@@ -1381,6 +1384,7 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
   if (p->wcycle_smooth < 0 || p->wcycle_smooth > 15)
     return fail(SDC_ERR_ARG, "wcycle_smooth must be in [0, 15]");
   if (p->nu_coarse < 0 || p->nu_coarse > 15) return fail(SDC_ERR_ARG, "nu_coarse must be in [0, 15]");
+  if (p->deriv_robin != 0 && p->deriv_robin != 1) return fail(SDC_ERR_ARG, "deriv_robin must be 0 or 1");
   DeviceGuard dg(h->device);
   drop_graphs(h);
   h->prm = *p;
@@ -1683,7 +1687,7 @@ int sdc_jvec_rhs(sdc_handle h, const double* dsig, const double* u, double* out,
   DeviceGuard dg(h->device);
   hipStream_t st = (hipStream_t)stream;
   const Level& l0 = h->L[0];
-  const Geo geo = make_geo(h);
+  const Geo geo = make_geo(h, true);
   hipLaunchKernelGGL(k_coef_from_sigma, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, geo, dsig,
                      h->tcx, h->tcy, h->tcz, h->tbd, -1.0);
   hipLaunchKernelGGL(k_coef_diag, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, l0.n1x, l0.n1y,
@@ -1713,7 +1717,7 @@ int sdc_jtvec_contract(sdc_handle h, const double* u, const double* lam, int nco
   if (per_column && ldo < nC) return fail(SDC_ERR_ARG, "ldo must be >= number of cells");
   DeviceGuard dg(h->device);
   hipStream_t st = (hipStream_t)stream;
-  const Geo geo = make_geo(h);
+  const Geo geo = make_geo(h, true);
   const Level& l0 = h->L[0];
   const int ng = jtvec_groups(ncol);
   double* part = nullptr;
@@ -1742,7 +1746,7 @@ int sdc_jtj_diag_accumulate(sdc_handle h, const double* u, const double* lam, in
   const int64_t nC = (int64_t)h->nx * h->ny * h->nz;
   DeviceGuard dg(h->device);
   hipStream_t st = (hipStream_t)stream;
-  const Geo geo = make_geo(h);
+  const Geo geo = make_geo(h, true);
   const int ng = jtvec_groups(ncol);
   double* part = nullptr;
   if (ng > 1) CK(jtvec_partials(h, ng, (size_t)nC, &part));

@@ -229,13 +229,30 @@ class NodalEngine:
                                          self._stream()), "sdc_jvec_rhs")
         return out
 
+    def _check_pairing(self, u, lam, ncol, ucol):
+        """Column j of lam pairs with u column ucol[j] (j when ucol is None): every paired column
+        must exist, since the kernels index u and lam without bounds."""
+        if ncol < 1 or ncol > lam.shape[0]:
+            raise ValueError(f"ncol={ncol} outside the {lam.shape[0]} columns of lam")
+        if ucol is None:
+            if ncol > u.shape[0]:
+                raise ValueError(f"lam has {ncol} columns but u only {u.shape[0]}")
+            return None
+        uc = np.asarray(ucol.cpu() if isinstance(ucol, torch.Tensor) else ucol).ravel()
+        if uc.size < ncol:
+            raise ValueError(f"ucol has {uc.size} entries, {ncol} needed")
+        if uc.size and (uc[:ncol].min() < 0 or uc[:ncol].max() >= u.shape[0]):
+            raise ValueError(f"ucol entries must lie in [0, {u.shape[0]})")
+        return torch.as_tensor(uc[:ncol].astype(np.int32)).to(self.device).contiguous()
+
     def contract(self, u, lam, ncol=None, ucol=None, per_column=False):
         ncol = lam.shape[0] if ncol is None else ncol
+        ucol = self._check_pairing(u, lam, ncol, ucol)
         if per_column:
             out = torch.empty((ncol, self.nC), dtype=torch.float64, device=self.device)
         else:
             out = torch.empty((self.nC,), dtype=torch.float64, device=self.device)
-        uc = None if ucol is None else torch.as_tensor(ucol, dtype=torch.int32).to(self.device).contiguous()
+        uc = ucol
         _lib.check(self.lib.sdc_jtvec_contract(self.h, _lib.ptr(u), _lib.ptr(lam), ncol,
                                                _lib.ptr(uc) if uc is not None else None,
                                                1 if per_column else 0, _lib.ptr(out), self.nC,
@@ -244,7 +261,9 @@ class NodalEngine:
 
     def jtj_diag_accumulate(self, u, lam, ncol, ucol, wsq, out):
         """out[c] += sum_j wsq[j] g_j[c]^2 over ncol adjoint columns (device arrays)."""
-        uc = torch.as_tensor(ucol, dtype=torch.int32).to(self.device).contiguous()
+        uc = self._check_pairing(u, lam, ncol, ucol)
+        if uc is None:
+            uc = torch.arange(ncol, dtype=torch.int32, device=self.device)
         w = torch.as_tensor(np.asarray(wsq, dtype=np.float64)).to(self.device).contiguous()
         _lib.check(self.lib.sdc_jtj_diag_accumulate(self.h, _lib.ptr(u), _lib.ptr(lam), ncol,
                                                     _lib.ptr(uc), _lib.ptr(w), _lib.ptr(out),

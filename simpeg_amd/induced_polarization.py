@@ -34,6 +34,12 @@ class Simulation3DNodal(_DCNodal):
     def _active_map(self):
         return self.etaMap
 
+    # the reference IP class reports an IdentityMap as its sigmaMap, so its getADeriv keeps the
+    # Robin term (induced_polarization/simulation.py:19-25; resistivity/simulation.py:566-570)
+    @property
+    def _deriv_includes_bc(self):
+        return True
+
     @property
     def sigma(self):
         if self._sigma_fixed is not None:

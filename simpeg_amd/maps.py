@@ -57,6 +57,14 @@ class IdentityMap:
     def inverse(self, D):
         return np.asarray(D, dtype=float).ravel()
 
+    # Elementwise maps also evaluate on device tensors (torch), so a model that lives on the GPU
+    # never goes through host memory: the value and the diagonal of the derivative.
+    def torch_transform(self, m):
+        return m
+
+    def torch_deriv_diag(self, m):
+        return m.new_ones(m.shape)
+
 
 class ExpMap(IdentityMap):
     """sigma = exp(m); derivative diag(exp(m))."""
@@ -70,6 +78,12 @@ class ExpMap(IdentityMap):
     def deriv(self, m, v=None):
         d = sdiag(np.exp(np.asarray(m, dtype=float).ravel()))
         return d if v is None else d @ v
+
+    def torch_transform(self, m):
+        return m.exp()
+
+    def torch_deriv_diag(self, m):
+        return m.exp()
 
     @property
     def is_linear(self):
@@ -89,6 +103,12 @@ class ReciprocalMap(IdentityMap):
         m = np.asarray(m, dtype=float).ravel()
         d = sdiag(-1.0 / m ** 2)
         return d if v is None else d @ v
+
+    def torch_transform(self, m):
+        return 1.0 / m
+
+    def torch_deriv_diag(self, m):
+        return -1.0 / (m * m)
 
     @property
     def is_linear(self):
@@ -110,6 +130,9 @@ class ComboMap(IdentityMap):
     @property
     def nP(self):
         return self.maps[-1].nP
+
+    torch_transform = None
+    torch_deriv_diag = None
 
     def _transform(self, m):
         for mp in reversed(self.maps):
@@ -144,6 +167,9 @@ class Projection(IdentityMap):
     @property
     def shape(self):
         return (self.index.size, self.nP)
+
+    torch_transform = None
+    torch_deriv_diag = None
 
     @property
     def is_linear(self):

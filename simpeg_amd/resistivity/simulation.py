@@ -55,6 +55,10 @@ class Simulation3DNodal:
     autotune : bool
         time the candidate multigrid/launch settings on the first ``fields`` call and keep the
         fastest (cached per mesh shape in ~/.cache/simpeg_amd/autotune.json).
+    exact_bc_deriv : bool
+        with ``rhoMap`` and a Robin boundary, include the boundary term d(AvgBC sigma)/dm in
+        Jvec / Jtvec / getJ.  The reference leaves it out for resistivity models
+        (resistivity/simulation.py:566-570), and so does this class by default.
     """
 
     _solutionType = "phiSolution"
@@ -66,7 +70,7 @@ class Simulation3DNodal:
                  surface_faces=None, sensitivity_path=None, counter=None, verbose=False,
                  model=None, *, precision="mixed", rtol=1e-10, inner_rtol=1e-5, max_outer=10,
                  max_iter=1000, device=None, gpu_params=None, coarse_max=0, jchunk=512,
-                 autotune=False):
+                 autotune=False, exact_bc_deriv=False):
         self.mesh = mesh
         self._widths, self._origin = mesh_widths(mesh)
         self._nC = int(np.prod([len(h) for h in self._widths]))
@@ -115,6 +119,7 @@ class Simulation3DNodal:
         self._coarse_max = int(coarse_max)
         self._engine = None
         self.autotune = bool(autotune)
+        self.exact_bc_deriv = bool(exact_bc_deriv)
         self._tuned = False
         self._model = None
         self._sigma_on_gpu = None
@@ -130,7 +135,7 @@ class Simulation3DNodal:
         if miniaturize:
             from .utils import _mini_pole_pole
 
-            self._dipoles, self._invs, self._mini = _mini_pole_pole(self.survey)
+            self._mini_T, self._mini = _mini_pole_pole(self.survey)
         if model is not None:
             self.model = model
 
@@ -166,14 +171,14 @@ class Simulation3DNodal:
             return d
         from .utils import mini_survey_data
 
-        return mini_survey_data(self._dipoles, self._invs, d)
+        return mini_survey_data(self._mini_T, d)
 
     def _mini_dataT(self, v):
         if self._mini is None:
             return v
         from .utils import mini_survey_dataT
 
-        return mini_survey_dataT(self._dipoles, self._invs, self._mini.nD, v)
+        return mini_survey_dataT(self._mini_T, v)
 
     @property
     def _active_map(self):
@@ -276,6 +281,19 @@ class Simulation3DNodal:
                     "anisotropic conductivity is not supported by the GPU nodal simulation")
             self.engine.set_sigma(sig)
             self._sigma_on_gpu = True
+
+    @property
+    def _deriv_includes_bc(self):
+        """Whether dA/dm carries the Robin term d(AvgBC sigma)/dm.  The reference's getADeriv adds
+        it only when sigmaMap is set (resistivity/simulation.py:566-570): with a rhoMap model it is
+        left out, unless ``exact_bc_deriv`` asks for the exact derivative."""
+        return self.sigmaMap is not None or self.exact_bc_deriv
+
+    def _sync_deriv_bc(self):
+        eng = self.engine
+        want = 1 if self._deriv_includes_bc else 0
+        if eng.params.deriv_robin != want:
+            eng.set_params(deriv_robin=want)
 
     def _solve(self, rhs, ncol):
         u, res = self.engine.solve(rhs, ncol, mode=self.precision, tol=self.rtol,
@@ -401,6 +419,7 @@ class Simulation3DNodal:
         if self.storeJ:
             return self.getJ(m, f=f) @ np.asarray(v, dtype=float)
         self._ensure_sigma()
+        self._sync_deriv_bc()
         dsig = self.sigmaDeriv @ np.asarray(v, dtype=float).ravel()
         eng = self.engine
         nS = self._sim_survey.nSrc
@@ -419,6 +438,7 @@ class Simulation3DNodal:
         if self.storeJ:
             return np.asarray(self.getJ(m, f=f).T @ v)
         self._ensure_sigma()
+        self._sync_deriv_bc()
         eng = self.engine
         nS = self._sim_survey.nSrc
         pr = self._projection()
@@ -437,6 +457,7 @@ class Simulation3DNodal:
             if f is None:
                 f = self.fields(m)
             self._ensure_sigma()
+            self._sync_deriv_bc()
             eng = self.engine
             pr = self._projection()
             nD = pr["nD"]
@@ -475,6 +496,7 @@ class Simulation3DNodal:
             if f is None:
                 f = self.fields(m)
             self._ensure_sigma()
+            self._sync_deriv_bc()
             eng = self.engine
             pr = self._projection()
             nD = pr["nD"]
@@ -510,6 +532,77 @@ class Simulation3DNodal:
         return SyntheticData(survey=self.survey, dobs=dobs, dclean=dclean,
                              relative_error=relative_error, noise_floor=noise_floor)
 
+    # ------------------------------------------------------------------ device-resident path
+    def _device_map(self):
+        """(sigma(m), diagonal of dsigma/dm) as functions of a model held in a device tensor, or
+        None when the map has no device evaluation (it is not elementwise) or the survey is
+        miniaturised (its recombination is a host sparse matrix)."""
+        if self._mini is not None:
+            return None
+        if self.sigmaMap is not None:
+            mp, recip = self.sigmaMap, False
+        elif self.rhoMap is not None:
+            mp, recip = self.rhoMap, True
+        else:
+            return None
+        tf, td = getattr(mp, "torch_transform", None), getattr(mp, "torch_deriv_diag", None)
+        if tf is None or td is None or mp.nP not in ("*", self._nC):
+            return None
+        if not recip:
+            return tf, td
+
+        def sig(m):
+            return 1.0 / tf(m)
+
+        def dsig(m):
+            s = 1.0 / tf(m)
+            return -s * s * td(m)
+
+        return sig, dsig
+
+    def _device_model(self, m_t):
+        fm = self._device_map()
+        if fm is None:
+            raise ValueError("the model map has no device evaluation (use the host API)")
+        m_t = m_t.detach().to(self.engine.device, torch.float64).contiguous()
+        if m_t.numel() != self._nC:
+            raise ValueError(f"Model is the wrong size: {m_t.numel()}, expected {self._nC}")
+        return fm, m_t
+
+    def dpred_device(self, m_t):
+        """Predicted data of a model held in a device tensor, as a device tensor, without host
+        copies of the model or the data.  Returns (d, state); state = (sigma, potentials) is what
+        ``jtvec_device`` reuses."""
+        fm, m_t = self._device_model(m_t)
+        eng = self.engine
+        sig = fm[0](m_t).contiguous()
+        eng.set_sigma(sig)
+        # the engine's operator no longer belongs to the host-side model
+        self._sigma_on_gpu = None
+        u = self._solve(self._source_block(), self._sim_survey.nSrc)
+        d = eng.gather(self._projection()["gather"], u)
+        return d, (sig, u)
+
+    def jtvec_device(self, m_t, v_t, state=None):
+        """J^T v for a device model and a device data vector, as a device tensor.  ``state`` from
+        ``dpred_device`` at the same model reuses its potentials; None solves them again."""
+        fm, m_t = self._device_model(m_t)
+        eng = self.engine
+        if state is None:
+            _, state = self.dpred_device(m_t)
+        sig, u = state
+        if eng._sigma is not sig:
+            eng.set_sigma(sig)
+            self._sigma_on_gpu = None
+        self._sync_deriv_bc()
+        nS = self._sim_survey.nSrc
+        v_t = v_t.detach().to(eng.device, torch.float64).contiguous()
+        if v_t.numel() != self.survey.nD:
+            raise ValueError(f"v must have {self.survey.nD} entries, got {v_t.numel()}")
+        lam = self._solve(eng.scatter(self._projection()["scatter"], v_t, nS), nS)
+        g = eng.contract(u, lam, nS)
+        return -(fm[1](m_t) * g)
+
     # ------------------------------------------------------------------ operators on the host
     def _edge_mass(self, sigma):
         return edge_mass(self._widths, sigma)
@@ -535,16 +628,36 @@ class Simulation3DNodal:
         return (A + sp.diags(robin_diagonal(self._widths, beta, sig))).tocsr()
 
     def getADeriv(self, u, v, adjoint=False):
-        """dA/dm products (resistivity/simulation.py:557-571), evaluated on the GPU."""
+        """dA/dm products (resistivity/simulation.py:557-571 with _inner_mat_mul_op,
+        base/pde_simulation.py:15-83), evaluated on the GPU.
+
+        Forward: u (nN,) or (nN, k), v (nP,) -> (nN,) or (nN, k).  Adjoint, with u and v squeezed
+        as the reference does: u (nN,) and v (nN,) -> (nP,); u (nN,) and v (nN, k) -> (nP, k), one
+        column per column of v; u (nN, k) and v (nN, k) -> (nP,), summed over the k pairs."""
         self._ensure_sigma()
+        self._sync_deriv_bc()
         eng = self.engine
-        ub = eng.to_block(np.asarray(u, dtype=float))
+        u = np.squeeze(np.asarray(u, dtype=float))
+        if u.ndim not in (1, 2) or u.shape[0] != self._nN:
+            raise ValueError(f"u must have {self._nN} rows, got shape {np.shape(u)}")
+        ub = eng.to_block(u)
         if not adjoint:
             dsig = self.sigmaDeriv @ np.asarray(v, dtype=float).ravel()
-            out = eng.jvec_rhs(dsig, ub, ub.shape[0])
-            return -eng.from_block(out)[:, 0] if ub.shape[0] == 1 else -eng.from_block(out)
-        lb = eng.to_block(np.asarray(v, dtype=float))
-        g = eng.contract(ub, lb, lb.shape[0]).cpu().numpy()
+            out = -eng.from_block(eng.jvec_rhs(dsig, ub, ub.shape[0]))
+            return out[:, 0] if u.ndim == 1 else out
+        v = np.squeeze(np.asarray(v, dtype=float))
+        if v.ndim not in (1, 2) or v.shape[0] != self._nN:
+            raise ValueError(f"v must have {self._nN} rows, got shape {np.shape(v)}")
+        lb = eng.to_block(v)
+        k = lb.shape[0]
+        if u.ndim == 1 and v.ndim == 2:
+            # every column of v is paired with the single u: per-column products
+            G = eng.contract(ub, lb, k, ucol=np.zeros(k, dtype=np.int32), per_column=True)
+            return np.asarray(self.sigmaDeriv.T @ G.cpu().numpy().T)
+        if u.ndim == 2 and (v.ndim != 2 or v.shape[1] != u.shape[1]):
+            raise ValueError(
+                f"u with {u.shape[1]} columns needs v with as many columns, got shape {v.shape}")
+        g = eng.contract(ub, lb, k).cpu().numpy()
         return self.sigmaDeriv.T @ g
 
     def getRHS(self):

@@ -1,68 +1,68 @@
-"""Survey miniaturisation (electromagnetics/static/resistivity/utils.py:75-128).
+"""Survey miniaturisation (behaviour of electromagnetics/static/resistivity/utils.py:75-128 and
+simulation.py:296-318).
 
-Every datum of a dipole/pole survey is a combination AM - AN - BM + BN of pole-pole
-potentials.  By reciprocity (A is symmetric) a pole-pole pair is the same whichever electrode
-is the source, so the survey reduces to the unique (sorted) electrode pairs: one pole source
-per unique first electrode with pole receivers at its partners.  ``dipoles`` flags which data
-have a B (dipole_tx) and an N (dipole_rx) electrode; ``invs`` maps every datum's AM, AN, BM, BN
-term to its row in the mini survey's data vector.
+A datum of a dipole or pole survey is V(A,M) - V(A,N) - V(B,M) + V(B,N), where V(X,Y) is the
+potential at electrode Y of a unit pole source at electrode X; a pole source has no B term and a
+pole receiver no N term.  A is symmetric, so V(X,Y) = V(Y,X): every term is named by an
+unordered electrode pair.  The survey is reduced to its distinct pairs, solved as pole sources
+with pole receivers, and recombined by a sparse (nD x n_pairs) matrix T of +-1 entries:
+d = T d_pairs, and the adjoint is T^T v.
 """
 
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse as sp
 
 from . import receivers, sources
 from .survey import Survey
 
 
 def _mini_pole_pole(survey, verbose=False):
-    A = survey.locations_a
-    B = survey.locations_b
-    M = survey.locations_m
-    N = survey.locations_n
-    elecs, inverse = np.unique(np.r_[A, B, M, N], axis=0, return_inverse=True)
-    inverse = np.asarray(inverse).ravel()
-    inv_A, inv_B, inv_M, inv_N = inverse.reshape(4, -1)
-    dipole_tx = inv_A != inv_B
-    dipole_rx = inv_M != inv_N
+    """(T, mini_survey) for ``survey``.
 
-    AM = np.sort(np.c_[inv_A, inv_M])
-    AN = np.sort(np.c_[inv_A[dipole_rx], inv_N[dipole_rx]])
-    BM = np.sort(np.c_[inv_B[dipole_tx], inv_M[dipole_tx]])
-    BN = np.sort(np.c_[inv_B[dipole_tx & dipole_rx], inv_N[dipole_tx & dipole_rx]])
-    pairs, pair_inv = np.unique(np.r_[AM, AN, BM, BN], axis=0, return_inverse=True)
-    pair_inv = np.asarray(pair_inv).ravel()
-    inv_AM, rest = pair_inv[: len(AM)], pair_inv[len(AM):]
-    inv_AN, rest = rest[: len(AN)], rest[len(AN):]
-    inv_BM, inv_BN = rest[: len(BM)], rest[len(BM):]
+    Electrodes are numbered by their sorted unique coordinates.  A pair {i, j} is keyed as
+    min * n + max; the mini survey has one pole source per distinct smaller index, with pole
+    receivers at its partners in increasing order, so row r of the mini data is pair key r in
+    sorted order.
+    """
+    loc = np.concatenate([survey.locations_a, survey.locations_b, survey.locations_m,
+                          survey.locations_n])
+    elecs, code = np.unique(loc, axis=0, return_inverse=True)
+    code = np.asarray(code).ravel()
+    nd = survey.nD
+    a, b, m, n = code[:nd], code[nd:2 * nd], code[2 * nd:3 * nd], code[3 * nd:]
+    has_b = a != b
+    has_n = m != n
+    ne = len(elecs)
+    datum = np.arange(nd)
+    # (datum, source electrode, receiver electrode, sign) of the four potential terms
+    parts = [(datum, a, m, np.ones(nd)),
+             (datum[has_n], a[has_n], n[has_n], -np.ones(has_n.sum())),
+             (datum[has_b], b[has_b], m[has_b], -np.ones(has_b.sum())),
+             (datum[has_b & has_n], b[has_b & has_n], n[has_b & has_n],
+              np.ones((has_b & has_n).sum()))]
+    rows = np.concatenate([p[0] for p in parts])
+    x = np.concatenate([p[1] for p in parts]).astype(np.int64)
+    y = np.concatenate([p[2] for p in parts]).astype(np.int64)
+    sign = np.concatenate([p[3] for p in parts])
+    keys = np.minimum(x, y) * ne + np.maximum(x, y)
+    ukeys, col = np.unique(keys, return_inverse=True)
+    T = sp.csr_matrix((sign, (rows, np.asarray(col).ravel())), shape=(nd, len(ukeys)))
     if verbose:
-        print(f"There are {pairs.shape[0]} unique pole-pole combinations.")
-    # pairs are sorted by (first, second): one pole source per distinct first electrode, its
-    # receivers in the order of the pairs, so mini-survey data rows equal pair indices
-    src_list = []
-    firsts, starts = np.unique(pairs[:, 0], return_index=True)
-    bounds = list(starts) + [len(pairs)]
-    for f, s0, s1 in zip(firsts, bounds[:-1], bounds[1:]):
-        rx = receivers.Pole(elecs[pairs[s0:s1, 1]])
-        src_list.append(sources.Pole([rx], elecs[f]))
-    return [dipole_rx, dipole_tx], [inv_AM, inv_AN, inv_BM, inv_BN], Survey(src_list)
+        print(f"There are {len(ukeys)} unique pole-pole combinations.")
+    lo, hi = np.divmod(ukeys, ne)
+    cut = np.flatnonzero(np.diff(lo)) + 1
+    src_list = [sources.Pole([receivers.Pole(elecs[hi[g]])], elecs[lo[g[0]]])
+                for g in np.split(np.arange(len(ukeys)), cut)]
+    return T, Survey(src_list)
 
 
-def mini_survey_data(dipoles, invs, d_mini):
-    """Full-survey data from mini-survey data (simulation.py:296-304)."""
-    out = np.array(d_mini[invs[0]], dtype=float)
-    out[dipoles[0]] -= d_mini[invs[1]]
-    out[dipoles[1]] -= d_mini[invs[2]]
-    out[dipoles[0] & dipoles[1]] += d_mini[invs[3]]
-    return out
+def mini_survey_data(T, d_mini):
+    """Full-survey data from the pole-pole data (simulation.py:296-304)."""
+    return np.asarray(T @ np.asarray(d_mini, dtype=float))
 
 
-def mini_survey_dataT(dipoles, invs, n_mini, v):
+def mini_survey_dataT(T, v):
     """Adjoint of ``mini_survey_data`` (simulation.py:306-318)."""
-    out = np.zeros(n_mini)
-    np.add.at(out, invs[0], v)
-    np.subtract.at(out, invs[1], v[dipoles[0]])
-    np.subtract.at(out, invs[2], v[dipoles[1]])
-    np.add.at(out, invs[3], v[dipoles[0] & dipoles[1]])
-    return out
+    return np.asarray(T.T @ np.asarray(v, dtype=float))

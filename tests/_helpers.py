@@ -23,11 +23,11 @@ def oracle_sources(survey):
     return out
 
 
-def oracle_for(mesh, survey, bc_type="Robin", mapping="exp", surface_faces=None):
+def oracle_for(mesh, survey, bc_type="Robin", mapping="exp", surface_faces=None, param="sigma"):
     om = OracleMesh(mesh.h, origin=mesh.origin)
     mp = OracleExpMap() if mapping == "exp" else OracleIdentityMap()
     return OracleDC(om, oracle_sources(survey), mapping=mp, bc_type=bc_type,
-                    surface_faces=surface_faces)
+                    surface_faces=surface_faces, param=param)
 
 
 def rel(a, b):

@@ -77,3 +77,23 @@ def test_l2_data_misfit_conventions(gpu, problem):
     assert np.isclose(dmis(m), float(np.sum((W * r) ** 2)), rtol=1e-12)
     g = dmis.deriv(m)
     assert rel(g, 2.0 * sim.Jtvec(m, W * W * r)) < 1e-10
+
+
+@pytest.mark.parametrize("recompute", [False, True])
+def test_device_model_stays_on_the_gpu(gpu, problem, recompute):
+    """A model in a GPU tensor gives data and a gradient in GPU tensors (the device path of the
+    autograd bridge), equal to the host path's, and leaves the host API consistent after it."""
+    from simpeg_amd.autograd import dpred, half_weighted_misfit
+
+    wl, sim, dobs, W = problem
+    m_host = torch.tensor(wl.model, dtype=torch.float64, requires_grad=True)
+    half_weighted_misfit(sim, m_host, dobs, W).backward()
+    m_dev = torch.tensor(wl.model, dtype=torch.float64, device=gpu, requires_grad=True)
+    d = dpred(sim, m_dev, recompute=recompute)
+    assert d.is_cuda
+    phi = half_weighted_misfit(sim, m_dev, torch.as_tensor(dobs, device=gpu),
+                               torch.as_tensor(W, device=gpu), recompute=recompute)
+    phi.backward()
+    assert m_dev.grad.is_cuda
+    assert rel(m_dev.grad.cpu().numpy(), m_host.grad.numpy()) < 1e-9
+    assert rel(d.detach().cpu().numpy(), sim.dpred(wl.model)) < 1e-9

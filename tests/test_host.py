@@ -167,19 +167,19 @@ def test_miniaturize_recombination_with_oracle():
 
     mesh, _, m = tiny_problem()
     survey = mixed_survey()
-    dipoles, invs, mini = _mini_pole_pole(survey)
+    T, mini = _mini_pole_pole(survey)
     assert mini.nSrc < survey.nSrc + 10 and all(isinstance(s, Src.Pole) for s in mini.source_list)
     full = oracle_for(mesh, survey)
     small = oracle_for(mesh, mini)
     d_full = full.dpred(m)
     d_mini = small.dpred(m)
-    assert np.allclose(mini_survey_data(dipoles, invs, d_mini), d_full, rtol=1e-10, atol=1e-14)
+    assert np.allclose(mini_survey_data(T, d_mini), d_full, rtol=1e-10, atol=1e-14)
     # adjoint of the recombination
     rng = np.random.default_rng(0)
     a = rng.standard_normal(mini.nD)
     b = rng.standard_normal(survey.nD)
-    lhs = b @ mini_survey_data(dipoles, invs, a)
-    rhs = a @ mini_survey_dataT(dipoles, invs, mini.nD, b)
+    lhs = b @ mini_survey_data(T, a)
+    rhs = a @ mini_survey_dataT(T, b)
     assert np.isclose(lhs, rhs, rtol=1e-12)
 
 

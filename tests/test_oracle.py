@@ -324,3 +324,124 @@ def test_neumann_gauge_row():
     u = sim.fields(sigma)
     q = sim.source_term()
     assert np.allclose(u[0], q[0])
+
+
+# ---------------------------------------------------------------- interpolation and widths
+# The source term (sources.py:84-86) and the receiver projections (receivers.py:414-415, 493)
+# both use discretize's trilinear node interpolation.  These checks fail for swapped left/right
+# weights, an axis mix-up or a reversed padding order.
+
+
+def _stretched_mesh():
+    return OracleMesh([[(7.0, 3, -1.4), (5.0, 4), (6.0, 2, 1.25)],
+                       [(4.0, 2, -1.5), (3.0, 5), (4.0, 3, 1.3)],
+                       [(8.0, 3, -1.2), (2.5, 4)]], origin="CCN")
+
+
+def test_unpack_widths_hand_computed():
+    """(h, n, f) tuples: h*|f|^k for k = 1..n, reversed for f < 0 (the padding grows away from
+    the core on both sides)."""
+    from oracle import unpack_widths
+
+    got = unpack_widths([(10.0, 3, -1.3), (5.0, 2), (10.0, 2, 1.5), 4.0])
+    want = [16.9, 13.0, 10.0 * 1.3, 5.0, 5.0, 15.0, 22.5, 4.0]
+    want[0] = 10.0 * 1.3 ** 3
+    want[1] = 10.0 * 1.3 ** 2
+    assert np.allclose(got, want, rtol=1e-14)
+    mesh = _stretched_mesh()
+    assert np.allclose(mesh.h[0][:3], [7.0 * 1.4 ** 3, 7.0 * 1.4 ** 2, 7.0 * 1.4])
+    assert np.allclose(mesh.h[1][-3:], [4.0 * 1.3, 4.0 * 1.69, 4.0 * 1.3 ** 3])
+    # origin "C" centres the axis, "N" puts its top at 0
+    assert np.isclose(mesh.nodes_x[0] + mesh.nodes_x[-1], 0.0, atol=1e-12)
+    assert np.isclose(mesh.nodes_z[-1], 0.0, atol=1e-12)
+
+
+def test_interpolation_reproduces_trilinear_functions():
+    """At random points (off nodes and off cell mid-points) on a stretched mesh, P reproduces
+    1, x, y, z and the trilinear products xy, yz, xz, xyz exactly, with 8 non-negative
+    weights per row."""
+    mesh = _stretched_mesh()
+    rng = np.random.default_rng(11)
+    lo = np.array([mesh.nodes_x[0], mesh.nodes_y[0], mesh.nodes_z[0]])
+    hi = np.array([mesh.nodes_x[-1], mesh.nodes_y[-1], mesh.nodes_z[-1]])
+    pts = lo + (hi - lo) * rng.random((40, 3))
+    P = mesh.get_interpolation_matrix(pts)
+    nodes = mesh.nodes
+    x, y, z = nodes.T
+    for f_nodes, f_pts in [(np.ones(len(x)), np.ones(len(pts))),
+                           (x, pts[:, 0]), (y, pts[:, 1]), (z, pts[:, 2]),
+                           (x * y, pts[:, 0] * pts[:, 1]), (y * z, pts[:, 1] * pts[:, 2]),
+                           (x * z, pts[:, 0] * pts[:, 2]),
+                           (x * y * z, pts[:, 0] * pts[:, 1] * pts[:, 2])]:
+        scale = np.abs(f_nodes).max()
+        assert np.allclose(P @ f_nodes, f_pts, rtol=0, atol=1e-12 * scale)
+    assert P.min() >= 0.0
+    assert np.all(np.diff(P.indptr) == 8)
+
+
+def test_interpolation_quarter_cell_weights():
+    """A point a quarter of a cell past node (i, j, k) along x, on the node's y and z lines,
+    weighs node i with 0.75 and node i+1 with 0.25 and nothing else; three quarters along y and
+    a tenth along z give the tensor product of the 1D weights on the right 8 nodes."""
+    mesh = _stretched_mesh()
+    xn, yn, zn = mesh.nodes_x, mesh.nodes_y, mesh.nodes_z
+    n1x, n1y = len(xn), len(yn)
+    i, j, k = 4, 6, 2
+    node = lambda a, b, c: a + n1x * (b + n1y * c)  # noqa: E731
+    p = np.array([[xn[i] + 0.25 * (xn[i + 1] - xn[i]), yn[j], zn[k]]])
+    row = mesh.get_interpolation_matrix(p).toarray()[0]
+    assert np.isclose(row[node(i, j, k)], 0.75) and np.isclose(row[node(i + 1, j, k)], 0.25)
+    assert np.isclose(row.sum(), 1.0) and np.count_nonzero(row) == 2
+    p = np.array([[xn[i] + 0.25 * (xn[i + 1] - xn[i]), yn[j] + 0.75 * (yn[j + 1] - yn[j]),
+                   zn[k] + 0.1 * (zn[k + 1] - zn[k])]])
+    row = mesh.get_interpolation_matrix(p).toarray()[0]
+    want = np.zeros_like(row)
+    for a, wa in ((0, 0.75), (1, 0.25)):
+        for b, wb in ((0, 0.25), (1, 0.75)):
+            for c, wc in ((0, 0.9), (1, 0.1)):
+                want[node(i + a, j + b, k + c)] = wa * wb * wc
+    assert np.allclose(row, want, rtol=0, atol=1e-14)
+    # a point on a node gets weight 1 there, including the last node of an axis
+    for q in [(i, j, k), (n1x - 1, n1y - 1, len(zn) - 1), (0, 0, 0)]:
+        p = np.array([[xn[q[0]], yn[q[1]], zn[q[2]]]])
+        row = mesh.get_interpolation_matrix(p).toarray()[0]
+        assert np.isclose(row[node(*q)], 1.0) and np.isclose(row.sum(), 1.0)
+
+
+# ---------------------------------------------------------------- resistivity models
+
+
+def test_rho_model_derivative_leaves_out_the_robin_term():
+    """With a resistivity model (rhoMap) the reference's getADeriv adds no boundary term
+    (resistivity/simulation.py:566-570), so its Jvec differs from the derivative of dpred by
+    exactly -P A^-1 (u * AvgBC sigma' v); with Neumann it is the derivative itself.  The adjoint
+    identity holds in both."""
+    mesh = _small_mesh()
+    srcs = _dipole_sources(mesh)
+    rng = np.random.default_rng(21)
+    rho0 = np.exp(-(np.log(0.01) + 0.4 * rng.standard_normal(mesh.n_cells)))
+    v = rho0 * rng.standard_normal(mesh.n_cells)
+    for bc in ("Robin", "Neumann"):
+        sim = OracleDC(mesh, srcs, mapping=OracleIdentityMap(), bc_type=bc, param="rho")
+        u = sim.fields(rho0)
+        Jv = sim.Jvec(rho0, v, u=u)
+        h = 1e-6
+        fd = (OracleDC(mesh, srcs, mapping=OracleIdentityMap(), bc_type=bc, param="rho")
+              .dpred(rho0 + h * v)
+              - OracleDC(mesh, srcs, mapping=OracleIdentityMap(), bc_type=bc, param="rho")
+              .dpred(rho0 - h * v)) / (2 * h)
+        sim.set_model(rho0)
+        if bc == "Robin":
+            dsig = sim.sigma_deriv @ v
+            corr = []
+            for s, plist in enumerate(sim._P):
+                du = sim.solve(-(u[:, s] * (sim.AvgBC @ dsig)))
+                corr += [P @ du for P in plist]
+            corr = np.concatenate(corr)
+            assert np.linalg.norm(corr) > 1e-3 * np.linalg.norm(fd)  # the term is not negligible
+            assert np.allclose(Jv + corr, fd, rtol=1e-5, atol=1e-7 * np.abs(fd).max())
+        else:
+            assert np.allclose(Jv, fd, rtol=1e-5, atol=1e-7 * np.abs(fd).max())
+        w = rng.standard_normal(sim.nD)
+        a, b = w @ Jv, v @ sim.Jtvec(rho0, w, u=u)
+        assert abs(a - b) < 1e-10 * max(abs(a), abs(b))
