@@ -143,6 +143,18 @@ def large_grf_model(seed=0, nlines=8, nsrc_per_line=16, nrx=16):
                     {"core": (96, 96, 64), "npad": 8, "seed": seed})
 
 
+def medium_grf_model(seed=5):
+    """32x32x20 core of 10 m cells + 6 pads x1.3 (44x44x26 cells, 54,675 nodes): the largest
+    mesh of the large_grf_model construction that the FP64 direct solver factors in seconds;
+    log sigma = log(0.01) + GRF(std 1, corr 60 m); 4 lines x 8 dipole-dipole sources x 8 Rx
+    (32 sources, 256 data)."""
+    mesh = padded_mesh((32, 32, 20), npad=6)
+    m = np.log(0.01) + gaussian_random_field(mesh, std=1.0, corr=60.0, seed=seed)
+    survey = grid_survey(4, 8, 8, spacing=20.0, y_span=200.0)
+    return Workload("grf_32x32x20_32src", mesh, survey, m,
+                    {"core": (32, 32, 20), "npad": 6, "seed": seed})
+
+
 def misfit_problem(seed=0):
     """Block model + survey of ``small_block_model`` with a data-misfit setup: observed data
     from a second log-conductivity (log 0.01 + GRF(std 0.5, corr 40 m, seed+1)) plus 3 % Gaussian
