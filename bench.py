@@ -26,7 +26,11 @@ Reported:
   api_level        the same step through the drop-in Simulation3DNodal methods with host numpy
                    in and out
   cpu_baseline     the FP64 CPU oracle (scipy splu, the reference's solver without MKL/MUMPS)
-                   on a bounded sample (see "sample")
+                   on a bounded sample (see "sample"), with the host's core count and BLAS
+                   thread settings
+  layered_160x160x96_256src, uneven_batch_48
+                   the same step on the 3.29M-node layered model, and the 48-problem batch
+                   behind MetaSimulation (fields and Jtvec), measured in the same run
 
 Other workloads: --workload small (the 24x24x12-core block model), layered (the same step on the
 160x160x96-core layered model, 3.29M nodes and 256 sources), batch (48 uneven problems behind
@@ -234,16 +238,14 @@ def rel_cols(u, ref, nN):
     return float(e.max())
 
 
-def cpu_baseline(budget_s=12.0):
-    """scipy splu oracle on the 24x24x12-core block model (22,050 nodes, 20 sources)."""
-    from simpeg_amd import workloads as W
+def _oracle_timing(wl, budget_s):
+    """Repeat fields (factor + solves + projection) and Jtvec of the scipy splu oracle on wl
+    until budget_s has passed (at least once); mean times."""
     from tests._helpers import oracle_for
 
-    wl = W.small_block_model(0)
     n = 0
     t_fwd = t_adj = t_fac = 0.0
-    rng = np.random.default_rng(0)
-    v = rng.standard_normal(wl.survey.nD)
+    v = np.random.default_rng(0).standard_normal(wl.survey.nD)
     start = time.perf_counter()
     while n == 0 or time.perf_counter() - start < budget_s:
         orc = oracle_for(wl.mesh, wl.survey)
@@ -261,17 +263,49 @@ def cpu_baseline(budget_s=12.0):
         t_adj += t3 - t2
         n += 1
     nS = wl.survey.nSrc
+    return {"workload": wl.name, "nodes": wl.mesh.n_nodes, "sources": nS, "repetitions": n,
+            "source_solves_per_s": round(nS * n / t_fwd, 3), "fields_s": round(t_fwd / n, 3),
+            "factor_s": round(t_fac / n, 3), "jtvec_ms": round(1e3 * t_adj / n, 1)}
+
+
+def _blas_threads():
+    """Thread settings of the BLAS / OpenMP libraries numpy and scipy loaded."""
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS",
+                                          "MKL_NUM_THREADS") if os.environ.get(k) is not None}
+    try:
+        from threadpoolctl import threadpool_info
+
+        libs = [{"library": i.get("internal_api"), "threads": i.get("num_threads")}
+                for i in threadpool_info()]
+    except Exception:
+        libs = []
+    return {"env": env, "pools": libs}
+
+
+def cpu_baseline(small_budget_s=6.0):
+    """The FP64 CPU oracle (scipy splu, what the reference runs without Pardiso or MUMPS) on
+    the 54,675-node mesh built like the benchmark's (44x44x26 cells, Gaussian random field, 32
+    dipole-dipole sources; one repetition, ~25 s) and on the 22,050-node block model (repeated
+    for small_budget_s).  value: source-solves/s of the first."""
+    from simpeg_amd import workloads as W
+
+    med = _oracle_timing(W.medium_grf_model(5), 0.0)
+    small = _oracle_timing(W.small_block_model(0), small_budget_s)
     return {
-        "value": round(nS * n / t_fwd, 3),
+        "value": med["source_solves_per_s"],
         "unit": "source-solves/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"scipy splu FP64 oracle on the 34x34x17-cell block model ({wl.mesh.n_nodes} "
-                   f"nodes, {nS} sources), {n} repetitions: fields (factor + solves) and Jtvec; "
-                   "splu cannot factor the 932,137-node mesh within the host's time/memory budget"),
-        "factor_s": round(t_fac / n, 3),
-        "jtvec_ms": round(1e3 * t_adj / n, 1),
-        "node_solves_per_s": round(wl.mesh.n_nodes * nS * n / t_fwd, 1),
+        "sample": ("scipy splu FP64 oracle, fields (factor + solves + projection) and Jtvec: "
+                   f"the {med['nodes']}-node 44x44x26-cell random-field mesh with {med['sources']} "
+                   f"sources (value), and the {small['nodes']}-node block model with "
+                   f"{small['sources']} sources; SuperLU factors and solves on one thread; "
+                   "splu cannot factor the 932,137-node mesh within the host's time and memory "
+                   "budget (factor time grows ~nN^2.4)"),
+        "host_cores": len(os.sched_getaffinity(0)),
+        "blas_threads": _blas_threads(),
+        "medium": med,
+        "small": small,
     }
 
 
@@ -289,11 +323,11 @@ def _range(name):
         pass
 
 
-def bench_batch(args):
+def batch_result(args):
     """48 independent problems (uneven sizes / source counts) behind MetaSimulation: fields +
     Jtvec for all of them, one stream after the other vs concurrently on 8 streams.  Multigrid
     settings come from the autotuner (tuned during the one-stream warm-up, cached per mesh shape
-    and column bucket)."""
+    and column bucket).  Returns the result dict."""
     import torch
 
     from simpeg_amd import maps, workloads
@@ -336,7 +370,37 @@ def bench_batch(args):
            "jtvec_ms": {str(k): round(1e3 * v[1], 1) for k, v in res.items()},
            "autotuned": True,
            "concurrency_speedup": round(sum(res[1]) / sum(res[8]), 2)}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_batch(args):
+    print(json.dumps(batch_result(args)), flush=True)
+
+
+def layered_result(args, dev, steps=3, warmup=1):
+    """The forward + Jtvec step of the main benchmark on the 160x160x96-core layered model
+    (3,289,545 nodes, 256 sources, 6,144 data) at --rtol, mean of `steps` steps."""
+    import torch
+
+    wl = build_problem("layered")
+    pl = Pipeline(wl, dev, json.loads(args.gpu_params))
+    for _ in range(warmup):
+        pl.step("mixed", args.rtol)
+    tf = ta = 0.0
+    for _ in range(steps):
+        r = pl.step("mixed", args.rtol)
+        tf += r["t_fwd"]
+        ta += r["t_adj"]
+    out = {"workload": wl.name, "nodes": wl.mesh.n_nodes, "sources": wl.survey.nSrc,
+           "data": wl.survey.nD, "rtol": args.rtol, "steps": steps,
+           "source_solves_per_s": round(wl.survey.nSrc * steps / tf, 2),
+           "forward_ms": round(1e3 * tf / steps, 2), "jtvec_ms": round(1e3 * ta / steps, 2),
+           "iterations": {"fwd_pcg": r["it_fwd"], "adj_pcg": r["it_adj"],
+                          "refinements": r["res"].outer}}
+    del pl, r
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_gn(args):
@@ -407,6 +471,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain fp64 per-source baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the fp64 reference solve")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the extra workloads reported beside the main one (the 48-problem "
+                         "batch and the layered 3.29M-node step)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -486,7 +553,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64 (bf16 multigrid storage)",
         "data": "synthetic (seeded Gaussian-random-field log-conductivity, generated on the host)",
         "config": {"workload": wl.name, "cells": list(wl.mesh.shape_cells),
                    "nodes": wl.mesh.n_nodes, "sources": nS, "data": wl.survey.nD,
@@ -555,6 +622,15 @@ def main():
 
     out["api_level"] = api_level(pl, args.rtol)
     log(f"[bench] drop-in API {out['api_level']}")
+
+    if args.workload == "large" and not args.no_extra:
+        # the other workloads of the benchmark set, measured in the same run
+        out["layered_160x160x96_256src"] = layered_result(args, dev)
+        log(f"[bench] layered step {out['layered_160x160x96_256src']}")
+        b = batch_result(args)
+        out["uneven_batch_48"] = {k: b[k] for k in ("value", "unit", "config", "fields_ms",
+                                                    "jtvec_ms", "concurrency_speedup")}
+        log(f"[bench] uneven batch {out['uneven_batch_48']}")
 
     if not args.no_cpu:
         log("[bench] CPU oracle baseline ...")
