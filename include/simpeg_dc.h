@@ -196,6 +196,17 @@ int sdc_jtvec_contract(sdc_handle h, const double* u_dev, const double* lam_dev,
                        const int32_t* ucol_dev, int per_column, double* out_dev, int64_t ldo,
                        void* stream);
 
+/* The column-summed contraction of sdc_jtvec_contract (per_column = 0) without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its per-column-group partial sums live in a fixed 32 MiB device array of the library, one per
+ * GPU, held until the process ends; groups beyond what it holds are summed in further passes in
+ * the same order, so the result equals sdc_jtvec_contract's bit for bit.  Limit: meshes of at
+ * most 4,194,304 cells (larger ones return SDC_ERR_ARG).  All calls on one GPU share that array:
+ * calls that overlap on one GPU (other streams, other handles, other threads) race. */
+int sdc_jtvec_contract_noalloc(sdc_handle h, const double* u_dev, const double* lam_dev, int ncol,
+                               const int32_t* ucol_dev, double* out_dev, void* stream);
+
 /* out_dev[c] += sum_j wsq[j] * g_{j,c}^2 with g_j the contraction above for column j (paired
  * with u column ucol[j]): the diagonal of J^T W^T W J before the map derivative, accumulated
  * over chunks of data columns without materialising J (getJtJdiag,

@@ -103,6 +103,7 @@ _SIGS = {
     "sdc_solve": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(SolveOpts), C.POINTER(SolveStats), _vp]),
     "sdc_jvec_rhs": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "sdc_jtvec_contract": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int64, _vp]),
+    "sdc_jtvec_contract_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp]),
     "sdc_jtj_diag_accumulate": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
     "sdc_gather": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sdc_scatter": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),

@@ -1419,16 +1419,20 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_jtvec(Geo g, const double* __rest
   }
 }
 
-// out[c] (ACC: out[c] +) = sum over the ngroups partials of cell c, in group order
-template <bool ACC>
+// Sum of the ngroups partials of cell c, in group order.
+//   MODE 0: out[c] = sum;  MODE 1: out[c] += sum;
+//   MODE 2: out[c] = (...((out[c] + p_0) + p_1) ...) -- the running sum of earlier groups
+//           continued, so that groups reduced in several passes add up in the same order (and
+//           to the same bits) as one pass over all of them.
+template <int MODE>
 __global__ __launch_bounds__(SDC_BLOCK) void k_jtvec_reduce(const double* __restrict__ part,
                                                             int ngroups, uint32_t nC,
                                                             double* __restrict__ out) {
   const uint32_t c = blockIdx.x * SDC_BLOCK + threadIdx.x;
   if (c >= nC) return;
-  double t = 0.0;
+  double t = MODE == 2 ? out[c] : 0.0;
   for (int g = 0; g < ngroups; ++g) t += ld_once(part + (size_t)g * nC + c);
-  out[c] = ACC ? out[c] + t : t;
+  out[c] = MODE == 1 ? out[c] + t : t;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -9,9 +9,18 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
+
+// Device scratch of the allocation-free adjoint contraction (sdc_jtvec_contract_noalloc): the
+// per-column-group partial sums, SDC_SCRATCH_DOUBLES doubles (32 MiB).  Every GPU the library
+// runs on holds its own copy until the process ends.  Host code never passes this name to a
+// kernel (on the host it is only a stand-in for the symbol): it uses the per-device address
+// that hipGetSymbolAddress returned at sdc_create.
+#define SDC_SCRATCH_DOUBLES (4u << 20)
+static __device__ __align__(256) double scratch[SDC_SCRATCH_DOUBLES];
 
 namespace {
 
@@ -1180,6 +1189,30 @@ int build_levels(sdc_handle_s* h, hipStream_t st);
 // adjoint contraction: columns per work-item group, and the partial-sum buffer for ng groups
 constexpr int kJtvecGroupCols = SDC_JT_COLS;
 int jtvec_groups(int ncol) { return (ncol + kJtvecGroupCols - 1) / kJtvecGroupCols; }
+
+// Device address of `scratch` on each device, resolved once per device with that device
+// current (sdc_create); null until then.
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_dev;
+
+int scratch_resolve(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0) return fail(SDC_ERR_ARG, "negative device ordinal");
+  if ((size_t)device >= g_scratch_dev.size()) g_scratch_dev.resize((size_t)device + 1, nullptr);
+  if (g_scratch_dev[device]) return SDC_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(SDC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_dev[device] = (double*)p;
+  return SDC_OK;
+}
+
+double* scratch_on(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && (size_t)device < g_scratch_dev.size() ? g_scratch_dev[device] : nullptr;
+}
+
 int jtvec_partials(sdc_handle_s* h, int ng, size_t nC, double** part) {
   const size_t need = (size_t)ng * nC;
   if (h->jpart_cap < need) {
@@ -1246,6 +1279,7 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
        !d->beta_zmax))
     return fail(SDC_ERR_ARG, "Robin boundary needs the six beta arrays");
   DeviceGuard dg(d->device);
+  if (int rc0 = scratch_resolve(d->device)) return rc0;
   sdc_handle_s* h = new sdc_handle_s();
   h->device = d->device;
   h->bc = d->bc_type;
@@ -1734,8 +1768,53 @@ int sdc_jtvec_contract(sdc_handle h, const double* u, const double* lam, int nco
   prof_end(h, SDC_PROF_JTVEC, st,
            (double)l0.nN * ncol * 16 + (double)nC * 8 * (per_column ? ncol : 1));
   if (part)
-    hipLaunchKernelGGL(k_jtvec_reduce<false>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st,
+    hipLaunchKernelGGL(k_jtvec_reduce<0>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st,
                        (const double*)part, ng, (uint32_t)nC, out);
+  HIPCK(hipGetLastError());
+  return SDC_OK;
+}
+
+int sdc_jtvec_contract_noalloc(sdc_handle h, const double* u, const double* lam, int ncol,
+                               const int32_t* ucol, double* out, void* stream) {
+  if (!h || !u || !lam || !out || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
+  const int64_t nC = (int64_t)h->nx * h->ny * h->nz;
+  if (nC > (int64_t)SDC_SCRATCH_DOUBLES)
+    return fail(SDC_ERR_ARG, "sdc_jtvec_contract_noalloc: the mesh has " + std::to_string(nC) +
+                                 " cells, more than the " + std::to_string(SDC_SCRATCH_DOUBLES) +
+                                 " its fixed scratch buffer holds (use sdc_jtvec_contract)");
+  DeviceGuard dg(h->device);
+  double* scr = scratch_on(h->device);
+  if (!scr) return fail(SDC_ERR_STATE, "scratch buffer not resolved on the handle's device");
+  hipStream_t st = (hipStream_t)stream;
+  const Geo geo = make_geo(h, true);
+  const size_t ld = h->L[0].ld;
+  const int ng = jtvec_groups(ncol);
+  if (ng == 1) {
+    hipLaunchKernelGGL(k_jtvec<0>, dim3(nblk(nC), 1), dim3(SDC_BLOCK), 0, st, geo, u, lam, ld,
+                       ncol, (const int*)ucol, out, (size_t)nC, (const double*)nullptr,
+                       (double*)nullptr);
+    HIPCK(hipGetLastError());
+    return SDC_OK;
+  }
+  // as many column groups per pass as the scratch holds partials for; every pass overwrites
+  // the partials of all its groups, and its reduction continues the running sum in out
+  const int per_pass = (int)std::min<int64_t>(ng, (int64_t)SDC_SCRATCH_DOUBLES / nC);
+  for (int g0 = 0; g0 < ng; g0 += per_pass) {
+    const int gn = std::min(per_pass, ng - g0);
+    const int cbeg = g0 * kJtvecGroupCols;
+    const int nc = std::min(ncol - cbeg, gn * kJtvecGroupCols);
+    const double* up = ucol ? u : u + (size_t)cbeg * ld;
+    const int32_t* uc = ucol ? ucol + cbeg : nullptr;
+    hipLaunchKernelGGL(k_jtvec<0>, dim3(nblk(nC), (uint32_t)gn), dim3(SDC_BLOCK), 0, st, geo, up,
+                       lam + (size_t)cbeg * ld, ld, nc, (const int*)uc, out, (size_t)nC,
+                       (const double*)nullptr, scr);
+    if (g0 == 0)
+      hipLaunchKernelGGL(k_jtvec_reduce<0>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st,
+                         (const double*)scr, gn, (uint32_t)nC, out);
+    else
+      hipLaunchKernelGGL(k_jtvec_reduce<2>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st,
+                         (const double*)scr, gn, (uint32_t)nC, out);
+  }
   HIPCK(hipGetLastError());
   return SDC_OK;
 }
@@ -1753,7 +1832,7 @@ int sdc_jtj_diag_accumulate(sdc_handle h, const double* u, const double* lam, in
   hipLaunchKernelGGL(k_jtvec<2>, dim3(nblk(nC), (uint32_t)ng), dim3(SDC_BLOCK), 0, st, geo, u,
                      lam, h->L[0].ld, ncol, (const int*)ucol, out, (size_t)nC, wsq, part);
   if (part)
-    hipLaunchKernelGGL(k_jtvec_reduce<true>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st,
+    hipLaunchKernelGGL(k_jtvec_reduce<1>, dim3(nblk(nC)), dim3(SDC_BLOCK), 0, st,
                        (const double*)part, ng, (uint32_t)nC, out);
   HIPCK(hipGetLastError());
   return SDC_OK;
