@@ -27,6 +27,31 @@
 
 #define SDC_NODE_PROLOGUE(NN) SDC_NODE_PROLOGUE_R(NN, 1)
 
+// Control block of the mixed PCG (device ints), read and written by the kernels so that the
+// iteration runs without a host decision: STOP ends the solve (every kernel of an iteration
+// returns at once), UPD asks for a reliable update in the current iteration, XFRESH / UFRESH
+// mark x as already folded into u / u as not yet written, IT counts iterations, LAST is the
+// iteration of the last update, REASON says why the solve stopped (1 converged, 2 iteration
+// limit, 3 stalled) and OUTER counts reliable updates.
+#define SDC_CTL_STOP 0
+#define SDC_CTL_UPD 1
+#define SDC_CTL_XFRESH 2
+#define SDC_CTL_UFRESH 3
+#define SDC_CTL_IT 4
+#define SDC_CTL_LAST 5
+#define SDC_CTL_REASON 6
+#define SDC_CTL_OUTER 7
+#define SDC_CTL_N 8
+// doubles beside it: the largest true relative residual of the previous update, of the last one
+#define SDC_DCTL_MPREV 0
+#define SDC_DCTL_MREL 1
+#define SDC_DCTL_N 2
+
+// a reliable update runs in this iteration
+__device__ __forceinline__ bool ctl_update(const int* ctl) {
+  return ctl[SDC_CTL_STOP] == 0 && ctl[SDC_CTL_UPD] != 0;
+}
+
 // ------------------------------------------------------------------------------------------
 // operator application: y = A x, optional per-column partial sums of x.y
 
@@ -564,15 +589,17 @@ __device__ __forceinline__ double block_reduce1(double v) {
   return t;
 }
 
-// x += alpha p ; r -= alpha q ; partial |r|^2
+// x += alpha p ; r -= alpha q ; partial |r|^2.  x is read from the device pointer table xp[0],
+// so that a recorded iteration serves any solution buffer.
 template <typename T>
-__global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* __restrict__ x, const T* __restrict__ p,
+__global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* const* xp, const T* __restrict__ p,
                                                           T* __restrict__ r, const T* __restrict__ q,
                                                           const double* __restrict__ alpha,
                                                           size_t ld, uint32_t nN,
                                                           double* __restrict__ partial,
                                                           const int* __restrict__ done) {
   if (done && *done) return;
+  T* __restrict__ x = xp[0];
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
   const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
@@ -604,7 +631,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* __restrict__ x, con
 // q = A z + beta q, so the stencil reads z (the preconditioner output, possibly bf16) instead of
 // p, and p, q are updated pointwise.  The x update of the previous step, x += alpha_prev p_old,
 // rides along (it needs p_old, which is read here anyway); partial sums of p.q per column.
-// FIRST: p = z, q = A z, no x update.  xfresh: x was folded away and counts as zero (not read).
+// FIRST: p = z, q = A z, no x update.  ctl[XFRESH]: x was folded away and counts as zero (not
+// read).
 template <typename Z, int R, bool FIRST>
 __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* __restrict__ z,
                                                         float* __restrict__ p,
@@ -613,7 +641,10 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* 
                                                         const double* __restrict__ beta,
                                                         const double* __restrict__ alpha_prev,
                                                         size_t ld, int ncol,
-                                                        double* __restrict__ partial, int xfresh) {
+                                                        double* __restrict__ partial,
+                                                        const int* __restrict__ ctl) {
+  if (ctl[SDC_CTL_STOP]) return;
+  const int xfresh = ctl[SDC_CTL_XFRESH];
   SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   double acc[R];
 #pragma unroll
@@ -659,7 +690,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
                                                          const double* __restrict__ alpha,
                                                          size_t ld, uint32_t nN,
                                                          double* __restrict__ partial,
-                                                         __bf16* __restrict__ r16) {
+                                                         __bf16* __restrict__ r16,
+                                                         const int* __restrict__ done) {
+  if (done && *done) return;
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
   const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
@@ -692,18 +725,22 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
   if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
 }
 
-// u += x + alpha p (fold the fp32 partial solution, including the pending x update of the last
-// step, into the fp64 solution).  x is not cleared: the next k_apply_pq treats it as zero.
-// xfresh: x counts as zero (not read).
-// UFRESH: u holds nothing yet (first fold of a solve): u = x + alpha p without reading u, and the
-// column padding [nN, ld) is zeroed, so the solve needs no separate clearing pass (the grid then
-// covers ld entries per column instead of nN).
-template <bool UFRESH>
-__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_p4(double* __restrict__ u,
-                                                          const float* __restrict__ x,
-                                                          const float* __restrict__ p,
-                                                          const double* __restrict__ alpha,
-                                                          size_t ld, uint32_t nN, int xfresh) {
+// u += x + alpha p: the fp32 partial solution, including the pending x update of the last step,
+// folded into the fp64 solution.  Runs only when the control block asks for a reliable update;
+// x is not cleared (XFRESH then tells the next k_apply_pq to treat it as zero, and XFRESH set
+// before the fold means x holds nothing yet and is not read).  UFRESH: u holds nothing yet (first
+// fold of a solve): u = x + alpha p without reading u, and the column padding [nN, ld) is zeroed,
+// so the solve needs no separate clearing pass.  u comes from the pointer table (u = qu[1]); the
+// grid covers ld entries per column, 4 consecutive nodes per work-item.
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_ctl(const double* const* qu,
+                                                           const float* __restrict__ x,
+                                                           const float* __restrict__ p,
+                                                           const double* __restrict__ alpha,
+                                                           size_t ld, uint32_t nN,
+                                                           const int* __restrict__ ctl) {
+  if (!ctl_update(ctl)) return;
+  double* __restrict__ u = const_cast<double*>(qu[1]);
+  const bool ufresh = ctl[SDC_CTL_UFRESH] != 0, xfresh = ctl[SDC_CTL_XFRESH] != 0;
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
   const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
@@ -712,21 +749,131 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_p4(double* __restrict__ u
     const vec4_t<float> pv = *(const vec4_t<float>*)(p + o + n0);
     const vec4_t<float> xv =
         (xfresh ? vec4_t<float>(0.0f) : *(const vec4_t<float>*)(x + o + n0)) + al * pv;
-    vec4_t<double> uv = UFRESH ? vec4_t<double>(0.0) : *(const vec4_t<double>*)(u + o + n0);
+    vec4_t<double> uv = ufresh ? vec4_t<double>(0.0) : *(const vec4_t<double>*)(u + o + n0);
     uv.x += (double)xv.x;
     uv.y += (double)xv.y;
     uv.z += (double)xv.z;
     uv.w += (double)xv.w;
     *(vec4_t<double>*)(u + o + n0) = uv;
   } else {
-    const uint32_t end = UFRESH ? (uint32_t)ld : nN;
+    const uint32_t end = ufresh ? (uint32_t)ld : nN;
     for (uint32_t n = n0; n < end && n < n0 + 4; ++n) {
       if (n < nN)
-        u[o + n] = (UFRESH ? 0.0 : u[o + n]) + (double)((xfresh ? 0.0f : x[o + n]) + al * p[o + n]);
+        u[o + n] = (ufresh ? 0.0 : u[o + n]) + (double)((xfresh ? 0.0f : x[o + n]) + al * p[o + n]);
       else
         u[o + n] = 0.0;
     }
   }
+}
+
+// Reliable-update decision after an iteration of the mixed PCG (one work-group): counts the
+// iteration and asks for an update (replace the fp32 residual by the fp64 true residual, fold x
+// into u) when every active column's fp32 residual |r|^2 passed tol^2 |q|^2, some active column's
+// dropped by delta^2 since its last replacement (rmax), the iteration limit is reached, or
+// `window` iterations passed without an update.  From the second iteration on, x holds the
+// lagged update of k_apply_pq, so it is no longer fresh.
+__global__ __launch_bounds__(SDC_BLOCK) void k_decide(int* __restrict__ ctl,
+                                                      const double* __restrict__ rr,
+                                                      const double* __restrict__ qn2,
+                                                      const double* __restrict__ rmax,
+                                                      const int* __restrict__ active, int ncol,
+                                                      double tol2, double delta2, int maxit,
+                                                      int window) {
+  if (ctl[SDC_CTL_STOP]) return;
+  int cand = 1, drop = 0;
+  for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
+    if (!active[c]) continue;
+    if (rr[c] > tol2 * qn2[c]) cand = 0;
+    if (rr[c] <= delta2 * rmax[c]) drop = 1;
+  }
+  cand = __syncthreads_and(cand);
+  drop = __syncthreads_or(drop);
+  if (threadIdx.x == 0) {
+    const int it = ctl[SDC_CTL_IT] + 1;
+    ctl[SDC_CTL_IT] = it;
+    if (it > 1) ctl[SDC_CTL_XFRESH] = 0;
+    ctl[SDC_CTL_UPD] =
+        (cand || drop || it >= maxit || it - ctl[SDC_CTL_LAST] >= window) ? 1 : 0;
+  }
+}
+
+// After a reliable update (one work-group): the lagged x update is cleared with alpha, the
+// solve stops when no column is open (converged), at the iteration limit, or when the largest
+// true relative residual failed to halve since the previous update (the fp32 recursion stalled;
+// the caller continues in fp64).  rmax and active were set by k_fin_true.
+__global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ctl,
+                                                            double* __restrict__ dctl,
+                                                            const double* __restrict__ orr,
+                                                            const double* __restrict__ qn2,
+                                                            const int* __restrict__ active,
+                                                            double* __restrict__ alpha, int ncol,
+                                                            int maxit) {
+  if (!ctl_update(ctl)) return;
+  __shared__ double sm[SDC_BLOCK / 64];
+  __shared__ int sn[SDC_BLOCK / 64];
+  double m = 0.0;
+  int nact = 0;
+  for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
+    alpha[c] = 0.0;
+    nact += active[c] != 0;
+    if (qn2[c] > 0.0) m = fmax(m, sqrt(orr[c] / qn2[c]));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    m = fmax(m, __shfl_xor(m, off, 64));
+    nact += __shfl_xor(nact, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sm[threadIdx.x >> 6] = m;
+    sn[threadIdx.x >> 6] = nact;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < SDC_BLOCK / 64; ++w) {
+      m = fmax(m, sm[w]);
+      nact += sn[w];
+    }
+    const int it = ctl[SDC_CTL_IT];
+    ctl[SDC_CTL_OUTER] += 1;
+    ctl[SDC_CTL_UPD] = 0;
+    ctl[SDC_CTL_XFRESH] = 1;
+    ctl[SDC_CTL_UFRESH] = 0;
+    dctl[SDC_DCTL_MREL] = m;
+    int reason = 0;
+    if (nact == 0)
+      reason = 1;
+    else if (it >= maxit)
+      reason = 2;
+    else if (m > 0.5 * dctl[SDC_DCTL_MPREV])
+      reason = 3;
+    if (reason) {
+      ctl[SDC_CTL_REASON] = reason;
+      ctl[SDC_CTL_STOP] = 1;
+    } else {
+      dctl[SDC_DCTL_MPREV] = m;
+      ctl[SDC_CTL_LAST] = it;
+    }
+  }
+}
+
+// Start of a device-driven solve: the control block, and q, u in the pointer table.
+__global__ void k_ctl_init(int* __restrict__ ctl, double* __restrict__ dctl,
+                           const double** __restrict__ qu, const double* q, double* u) {
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < SDC_CTL_N; ++i) ctl[i] = 0;
+    ctl[SDC_CTL_XFRESH] = 1;
+    ctl[SDC_CTL_UFRESH] = 1;
+    dctl[SDC_DCTL_MPREV] = 1.0;
+    dctl[SDC_DCTL_MREL] = 0.0;
+    qu[0] = q;
+    qu[1] = u;
+  }
+}
+
+// xp[0] = x (the solution buffer of a device-driven PCG)
+template <typename T>
+__global__ void k_set_ptr(T** __restrict__ xp, T* x) {
+  if (threadIdx.x == 0) xp[0] = x;
 }
 
 // p = z + beta p (FIRST: p = z)
@@ -780,7 +927,9 @@ __device__ __forceinline__ double block_sum(const double* __restrict__ part, uin
 // out[c] = sum of partials; mode 1 additionally initialises the PCG state
 __global__ __launch_bounds__(SDC_BLOCK) void k_fin_norm(const double* __restrict__ partial,
                                                         uint32_t nbx, double* __restrict__ out,
-                                                        int* __restrict__ active, int init) {
+                                                        int* __restrict__ active, int init,
+                                                        const int* __restrict__ done) {
+  if (done && *done) return;
   const int c = blockIdx.x;
   const double v = block_sum(partial + (size_t)c * nbx, nbx);
   if (threadIdx.x == 0) {
@@ -802,15 +951,16 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_alpha(const double* __restric
   if (threadIdx.x == 0) alpha[c] = (active[c] && pq > 0.0) ? rz[c] / pq : 0.0;
 }
 
-// convergence test |r| <= tol |b| per column; the last work-group publishes the number of
-// still-active columns and the all-converged flag (device copy and host-mapped copy)
+// convergence test |r| <= tol |b| per column; the last work-group counts the iteration
+// (counters[4]) and publishes the number of still-active columns and the stop flag: all
+// converged, or maxit iterations done (device copy and host-mapped copy)
 __global__ __launch_bounds__(SDC_BLOCK) void k_fin_rr(const double* __restrict__ partial,
                                                       uint32_t nbx,
                                                       const double* __restrict__ bnorm2,
                                                       double tol2, double* __restrict__ rr,
                                                       int* __restrict__ active,
                                                       int* __restrict__ counters, int* done,
-                                                      int* host_flag, int maxit_reached) {
+                                                      int* host_flag, int maxit) {
   if (*done) return;
   const int c = blockIdx.x;
   const double v = block_sum(partial + (size_t)c * nbx, nbx);
@@ -824,7 +974,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_rr(const double* __restrict__
     const int ticket = atomicAdd(&counters[1], 1);
     if (ticket == (int)gridDim.x - 1) {
       const int nact = atomicAdd(&counters[0], 0);
-      const int fin = (nact == 0 || maxit_reached) ? 1 : 0;
+      const int it = counters[4] + 1;
+      counters[4] = it;
+      const int fin = (nact == 0 || it >= maxit) ? 1 : 0;
       *done = fin;
       counters[2] = nact;
       counters[0] = 0;
@@ -891,14 +1043,24 @@ struct PinFix {
 // qmask (per node, never null): with U_ZERO it is set to 1 on every node where some column of
 // q is non-zero (the caller clears it first); otherwise q is read only on flagged nodes.  A
 // right-hand side of point sources touches a few nodes, so the fp64 residual then streams u.
+// ctl non-null: runs only when the control block asks for a reliable update, and q, u are read
+// from qu[0], qu[1] (device pointer table) instead of the arguments, so that a recorded
+// iteration serves every solve.
 template <typename TO, int R, bool U_ZERO>
-__global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const double* __restrict__ q,
-                                                           const double* __restrict__ u,
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const double* q,
+                                                           const double* u,
                                                            TO* __restrict__ rout, size_t ld,
                                                            int ncol, PinFix pf,
                                                            double* __restrict__ partial,
                                                            unsigned* __restrict__ qmask,
-                                                           __bf16* __restrict__ r16) {
+                                                           __bf16* __restrict__ r16,
+                                                           const int* __restrict__ ctl,
+                                                           const double* const* qu) {
+  if (ctl) {
+    if (!ctl_update(ctl)) return;
+    q = qu[0];
+    u = qu[1];
+  }
   SDC_NODE_PROLOGUE_R(s.nN, s.remap)
   double acc[R];
 #pragma unroll
@@ -944,13 +1106,15 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
 }
 
 // true-residual check after a reliable update: rr = |q - A u|^2, rmax = rr,
-// active = rr > tol^2 |q|^2
+// active = rr > tol^2 |q|^2 (ctl non-null: only when a reliable update runs)
 __global__ __launch_bounds__(SDC_BLOCK) void k_fin_true(const double* __restrict__ partial,
                                                         uint32_t nbx,
                                                         const double* __restrict__ qn2,
                                                         double tol2, double* __restrict__ rr,
                                                         double* __restrict__ rmax,
-                                                        int* __restrict__ active) {
+                                                        int* __restrict__ active,
+                                                        const int* __restrict__ ctl) {
+  if (ctl && !ctl_update(ctl)) return;
   const int c = blockIdx.x;
   const double v = block_sum(partial + (size_t)c * nbx, nbx);
   if (threadIdx.x == 0) {

@@ -143,7 +143,8 @@ struct Work {
   double *alpha = nullptr, *beta = nullptr, *rz = nullptr, *rr = nullptr, *bn2 = nullptr,
          *aux = nullptr;
   int* active = nullptr;
-  int* counters = nullptr;  // [0] active count, [1] ticket, [2] last active count, [3] done
+  int* counters = nullptr;  // [0] active count, [1] ticket, [2] last active count, [3] done,
+                            // [4] iterations
   // refinement (outer) state: |q|^2 in aux, |q - A u|^2 in orr, open columns in omask
   double* orr = nullptr;
   int* omask = nullptr;
@@ -152,8 +153,20 @@ struct Work {
   unsigned* qmask = nullptr;
   // bfloat16 copy of r, the finest-level multigrid input (vcycle_bf16 = 3 or 4)
   __bf16* r16 = nullptr;
+  // device-driven mixed PCG: control block (SDC_CTL_N ints, then SDC_DCTL_N doubles in the same
+  // allocation), per-column true residual at the last update, and the (q, u) pointer table
+  double* ctlmem = nullptr;
+  int* ctl = nullptr;
+  double* dctl = nullptr;
+  double* rmax = nullptr;
+  const double** qu = nullptr;
+  // plain PCG: the solution buffer, read by k_update_xr4 through this device pointer
+  T** xtab = nullptr;
   void release() {
     dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(r16);
+    dfree(ctlmem); dfree(rmax); dfree(qu); dfree(xtab);
+    ctl = nullptr;
+    dctl = nullptr;
     for (auto* v : b) dfree(v);
     for (auto* v : xs) dfree(v);
     for (auto* v : xa) dfree(v);
@@ -198,6 +211,7 @@ struct sdc_handle_s {
   Work<float> w32;
   Work<double> w64;
   int* h_flag = nullptr;  // pinned host copy of the convergence flag
+  double* h_ctl = nullptr;  // pinned host copy of the mixed PCG control block
   // recorded finest-level V-cycles (sdc_params.graph): one executable graph per input, output,
   // column count and storage variant, launched on the caller's stream
   struct GraphRec {
@@ -514,13 +528,20 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   CK(dmalloc(&w.alpha, cap)); CK(dmalloc(&w.beta, cap)); CK(dmalloc(&w.rz, cap));
   CK(dmalloc(&w.rr, cap)); CK(dmalloc(&w.bn2, cap)); CK(dmalloc(&w.aux, cap));
   CK(dmalloc(&w.active, cap));
-  CK(dmalloc(&w.counters, 4));
-  HIPCK(hipMemset(w.counters, 0, 4 * sizeof(int)));
+  CK(dmalloc(&w.counters, 8));
+  HIPCK(hipMemset(w.counters, 0, 8 * sizeof(int)));
   CK(dmalloc(&w.orr, cap));
   CK(dmalloc(&w.omask, cap));
   CK(dmalloc(&w.ocount, 1));
   CK(dmalloc(&w.qmask, ld0));
   if (want16) CK(dmalloc(&w.r16, n0));
+  static_assert(SDC_CTL_N * sizeof(int) == 4 * sizeof(double), "control block layout");
+  CK(dmalloc(&w.ctlmem, 4 + SDC_DCTL_N));
+  w.ctl = (int*)w.ctlmem;
+  w.dctl = w.ctlmem + 4;
+  CK(dmalloc(&w.rmax, cap));
+  CK(dmalloc(&w.qu, 2));
+  CK(dmalloc(&w.xtab, 1));
   w.ncap = cap;
   return SDC_OK;
 }
@@ -914,6 +935,9 @@ int graph_run(sdc_handle_s* h, const void* b, void* x, int ncol, int variant, hi
 
 // Preconditioned CG on ncol columns: rhs in W.r (overwritten by the residual), solution in x.
 // Each column stops when |r| <= tol |b|.  col_mask (device, may be null) excludes columns.
+// The convergence test and the iteration count live on the device (k_fin_rr), and every kernel
+// of an iteration returns at once after the stop flag is set, so check_every iterations at a
+// time run without a host synchronisation -- as one recorded graph when sdc_params.graph is set.
 template <typename T>
 int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_mask, int* iters,
         hipStream_t st) {
@@ -926,11 +950,12 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   const size_t len = l0.ld * (size_t)ncol;
   int* done = W.counters + 3;
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(len)), blk, 0, st, x, len);
-  hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)4);
+  hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)8);
+  hipLaunchKernelGGL(k_set_ptr<T>, dim3(1), dim3(64), 0, st, W.xtab, x);
   DISPATCH_R(R, hipLaunchKernelGGL((k_dot<T, R>), g, blk, 0, st, (const T*)W.r, (const T*)W.r,
                                    l0.ld, l0.nN, ncol, W.partial, (const int*)nullptr));
   hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.bn2,
-                     W.active, 1);
+                     W.active, 1, (const int*)nullptr);
   if (col_mask)
     hipLaunchKernelGGL(k_and_active, dim3(nblk(ncol)), blk, 0, st, W.active, col_mask, ncol);
   CK(graph_run(h, W.r, W.z, ncol, 8 + (int)sizeof(T), st, [&](hipStream_t gs) {
@@ -942,45 +967,52 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
                      (const double*)W.beta, l0.ld, l0.nN, (const int*)done);
   const Sten<T> s = make_sten<T>(h, 0, 1);
   const double tol2 = tol * tol;
-  const int check = std::max(1, (int)h->prm.check_every);
-  int it = 0;
-  bool finished = false;
-  for (it = 1; it <= maxit && !finished; ++it) {
-    prof_begin(h, SDC_PROF_APPLY, st);
-    DISPATCH_R(R, hipLaunchKernelGGL((k_apply<T, R, true>), g, blk, 0, st, s, (const T*)W.p, W.q,
-                                     l0.ld, ncol, W.partial, (const int*)done));
-    prof_end(h, SDC_PROF_APPLY, st,
+  // one iteration: q = A p, alpha, x += alpha p and r -= alpha q, the convergence test, z = M r,
+  // beta, p = z + beta p -- every launch through launch() so that it can be recorded
+  auto iteration = [&](hipStream_t s2) -> int {
+    prof_begin(h, SDC_PROF_APPLY, s2);
+    DISPATCH_R(R, launch(h, (k_apply<T, R, true>), g, blk, 0, s2, s, (const T*)W.p, W.q, l0.ld,
+                         ncol, W.partial, (const int*)done));
+    prof_end(h, SDC_PROF_APPLY, s2,
              (double)l0.nN * 4 * sizeof(T) + (double)l0.nN * ncol * 2 * sizeof(T));
-    hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
-                       (const double*)W.rz, W.alpha, (const int*)W.active, (const int*)done);
-    prof_begin(h, SDC_PROF_UPDATE, st);
-    hipLaunchKernelGGL(k_update_xr4<T>, grid4(l0.nN, ncol), blk, 0, st, x, (const T*)W.p, W.r,
-                       (const T*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
-                       (const int*)done);
-    prof_end(h, SDC_PROF_UPDATE, st, (double)l0.nN * ncol * 6 * sizeof(T));
-    hipLaunchKernelGGL(k_fin_rr, dim3(ncol), blk, 0, st, (const double*)W.partial, nblk4(l0.nN),
-                       (const double*)W.bn2, tol2, W.rr, W.active, W.counters, done,
-                       (int*)nullptr, (int)(it == maxit));
-    if (it % check == 0 || it == maxit) {
-      HIPCK(hipMemcpyAsync(h->h_flag, done, sizeof(int), hipMemcpyDeviceToHost, st));
-      HIPCK(hipStreamSynchronize(st));
-      if (h->h_flag[0]) {
-        finished = true;
-        break;
-      }
-    }
-    CK(graph_run(h, W.r, W.z, ncol, 8 + (int)sizeof(T), st, [&](hipStream_t gs) {
-      return vcycle0<T>(h, W.r, W.z, ncol, true, done, gs);
-    }));
-    hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
-                       W.beta, (const int*)W.active, 0, (const int*)done);
-    prof_begin(h, SDC_PROF_UPDATE_P, st);
-    hipLaunchKernelGGL((k_update_p4<T, false>), grid4(l0.nN, ncol), blk, 0, st, (const T*)W.z, W.p,
-                       (const double*)W.beta, l0.ld, l0.nN, (const int*)done);
-    prof_end(h, SDC_PROF_UPDATE_P, st, (double)l0.nN * ncol * 3 * sizeof(T));
+    launch(h, k_fin_alpha, dim3(ncol), blk, 0, s2, (const double*)W.partial, nbx,
+           (const double*)W.rz, W.alpha, (const int*)W.active, (const int*)done);
+    prof_begin(h, SDC_PROF_UPDATE, s2);
+    launch(h, k_update_xr4<T>, grid4(l0.nN, ncol), blk, 0, s2, (T* const*)W.xtab, (const T*)W.p,
+           W.r, (const T*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
+           (const int*)done);
+    prof_end(h, SDC_PROF_UPDATE, s2, (double)l0.nN * ncol * 6 * sizeof(T));
+    launch(h, k_fin_rr, dim3(ncol), blk, 0, s2, (const double*)W.partial, nblk4(l0.nN),
+           (const double*)W.bn2, tol2, W.rr, W.active, W.counters, done, (int*)nullptr, maxit);
+    CK(vcycle0<T>(h, W.r, W.z, ncol, true, done, s2));
+    launch(h, k_fin_beta, dim3(ncol), blk, 0, s2, (const double*)W.partial, nbx, W.rz, W.beta,
+           (const int*)W.active, 0, (const int*)done);
+    prof_begin(h, SDC_PROF_UPDATE_P, s2);
+    launch(h, k_update_p4<T, false>, grid4(l0.nN, ncol), blk, 0, s2, (const T*)W.z, W.p,
+           (const double*)W.beta, l0.ld, l0.nN, (const int*)done);
+    prof_end(h, SDC_PROF_UPDATE_P, s2, (double)l0.nN * ncol * 3 * sizeof(T));
+    return SDC_OK;
+  };
+  const int K = std::max(1, (int)h->prm.check_every);
+  const bool rec = h->prm.graph && h->prof_mask == 0;
+  const int variant = 2000 + 16 * K + (int)sizeof(T);
+  for (int it = 0;; it += K) {
+    auto block = [&](hipStream_t s2) -> int {
+      for (int k = 0; k < K; ++k) CK(iteration(s2));
+      return SDC_OK;
+    };
+    if (rec)
+      CK(graph_run(h, W.counters, (void*)W.partial, ncol, variant, st, block));
+    else
+      CK(block(st));
+    HIPCK(hipMemcpyAsync(h->h_flag, W.counters + 3, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (h->h_flag[0]) break;
+    // the device stops at maxit by itself; this bound only guards the host loop
+    if (it > maxit + K) return fail(SDC_ERR_STATE, "internal: PCG did not stop");
   }
   HIPCK(hipGetLastError());
-  if (iters) *iters = std::min(it, maxit);
+  if (iters) *iters = std::min(h->h_flag[1], maxit);
   return SDC_OK;
 }
 
@@ -988,6 +1020,10 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
 // and the fp32 residual is replaced by the fp64 true residual q - A u whenever every active
 // column's fp32 residual passed the tolerance, or some column's residual dropped by
 // reliable_delta since its last replacement.  Columns stop on the true residual only.
+// Every decision is taken on the device (k_decide, k_after_update on the control block W.ctl):
+// an iteration is a fixed sequence of launches whose kernels return at once when the control
+// block says the solve stopped or no update is due, so check_every iterations at a time run
+// without a host synchronisation -- as one recorded graph when sdc_params.graph is set.
 int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sdc_solve_opts* o,
                  sdc_solve_stats& S, hipStream_t st) {
   CK(ensure_work<float>(h, ncol));
@@ -1009,28 +1045,25 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const Sten<float> s1 = make_sten<float>(h, 0, 1);
   const double tol2 = o->tol * o->tol;
   const double delta2 = h->prm.reliable_delta * h->prm.reliable_delta;
+  const int maxit = o->max_iter;
   // u is not cleared: the first fold writes it (and its column padding) without reading it
   hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
   // vcycle_bf16 = 3, 4: the multigrid reads a bfloat16 copy of r, written with r
   __bf16* const r16 =
       ((h->prm.vcycle_bf16 == 3 || h->prm.vcycle_bf16 == 4) && h->L.size() > 1) ? W.r16 : nullptr;
   DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
-                                   (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask, r16));
-  // aux = |q|^2 and active = |q| > 0; orr = |q|^2
+                                   (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask, r16,
+                                   (const int*)nullptr, (const double* const*)nullptr));
+  // aux = |q|^2 and active = |q| > 0; rmax = |q|^2 (the residual at the last replacement)
   hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
-                     W.active, 1);
-  hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
-                     W.active, 0);
-  std::vector<double> h_qn(ncol), h_rr(ncol), h_rmax(ncol);
+                     W.active, 1, (const int*)nullptr);
+  hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rmax,
+                     W.active, 0, (const int*)nullptr);
+  std::vector<double> h_qn(ncol);
   HIPCK(hipMemcpyAsync(h_qn.data(), W.aux, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCK(hipStreamSynchronize(st));
-  h_rmax = h_qn;
-  std::vector<char> act(ncol);
   int nact = 0;
-  for (int c = 0; c < ncol; ++c) {
-    act[c] = h_qn[c] > 0.0;
-    nact += act[c];
-  }
+  for (int c = 0; c < ncol; ++c) nact += h_qn[c] > 0.0;
   S.outer = 0;
   S.iters = 0;
   S.max_rel_res = 0.0;
@@ -1039,62 +1072,53 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
     return SDC_OK;
   }
-  // the finalisers test the convergence flag counters[3]; a previous solve may have left it set
-  hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)4);
-  // Iteration: z = M r (z kept in the V-cycle storage type), beta, p = z + beta p and
-  // q = A z + beta q in one pass (plus the lagged x += alpha p), alpha, r -= alpha q.  A reliable
-  // update folds x + alpha p into u, so the lagged update is cleared with alpha.
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, W.ctl, W.dctl, W.qu, q, u);
   // vcycle_bf16 = 1, 4: the preconditioned residual z is bfloat16
   const bool zb = (h->prm.vcycle_bf16 == 1 || h->prm.vcycle_bf16 == 4) && h->L.size() > 1;
   __bf16* const z16 = (__bf16*)W.xb0;
-  const int* done = W.counters + 3;
-  const int check = std::max(1, (int)h->prm.check_every);
-  const int maxit = o->max_iter;
+  const int* stop = W.ctl + SDC_CTL_STOP;
   const double bz = zb ? 2.0 : 4.0;
-  bool xfresh = true;  // x holds nothing that is not yet in u (its memory is not read)
-  bool ufresh = true;  // u has not been written yet (the first fold sets it)
   // Stall guard: a reliable update is forced after kStallWindow iterations without one, and the
   // solve gives up (converged = 0, the caller continues in fp64) when an update fails to halve
   // the largest true relative residual of the previous one.  This happens when the fp32
   // recursion no longer tracks the true residual (conductivity contrasts of 1e4 and more).
   constexpr int kStallWindow = 25;
-  double m_prev = 1.0;
-  int last_update = 0;
-  int it;
-  for (it = 1; it <= maxit; ++it) {
-    const bool first = it == 1;
-    if (zb && r16) {
-      CK(graph_run(h, r16, z16, ncol, 4, st, [&](hipStream_t s) {
-        return vcycle<float, __bf16, __bf16, __bf16>(h, 0, r16, z16, ncol, true, nullptr, s);
-      }));
-    } else if (zb) {
-      CK(graph_run(h, W.r, z16, ncol, 1, st, [&](hipStream_t s) {
-        return vcycle<float, __bf16, float, __bf16>(h, 0, W.r, z16, ncol, true, nullptr, s);
-      }));
-    } else if (r16) {
-      CK(graph_run(h, r16, W.z, ncol, 3, st, [&](hipStream_t s) {
-        return vcycle<float, __bf16, __bf16, float>(h, 0, r16, W.z, ncol, true, nullptr, s);
-      }));
-    } else {
-      CK(graph_run(h, W.r, W.z, ncol, h->prm.vcycle_bf16 ? 2 : 0, st, [&](hipStream_t s) {
-        return vcycle0<float>(h, W.r, W.z, ncol, true, nullptr, s);
-      }));
-    }
-    hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
-                       W.beta, (const int*)W.active, first ? 1 : 0, done);
+  // fp64 stencil of the true residual: refine_rhs_per_thread columns per work-item (default 8;
+  // its coefficients are fp64, so more columns per work-item save more traffic than in the fp32
+  // kernels)
+  const int R64 = h->prm.refine_rhs_per_thread > 0 ? h->prm.refine_rhs_per_thread
+                  : ncol >= 8                      ? 8
+                                                   : R;
+  // Iteration: z = M r (z kept in the V-cycle storage type), beta, p = z + beta p and
+  // q = A z + beta q in one pass (plus the lagged x += alpha p), alpha, r -= alpha q, the update
+  // decision, and the predicated reliable update (fold x + alpha p into u, fp64 true residual).
+  // Every launch goes through launch() so that the whole iteration can be recorded.
+  bool fold_fresh = true;  // profiled runs only: the next fold writes u without reading it
+  auto iteration = [&](bool first, hipStream_t s) -> int {
+    int rc = SDC_OK;
+    if (zb && r16)
+      rc = vcycle<float, __bf16, __bf16, __bf16>(h, 0, r16, z16, ncol, true, stop, s);
+    else if (zb)
+      rc = vcycle<float, __bf16, float, __bf16>(h, 0, W.r, z16, ncol, true, stop, s);
+    else if (r16)
+      rc = vcycle<float, __bf16, __bf16, float>(h, 0, r16, W.z, ncol, true, stop, s);
+    else
+      rc = vcycle0<float>(h, W.r, W.z, ncol, true, stop, s);
+    CK(rc);
+    launch(h, k_fin_beta, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx, W.rz, W.beta,
+           (const int*)W.active, first ? 1 : 0, stop);
     // the class covers the steady-state launches (the first one skips p, q and x)
     const int atag = first ? -1 : SDC_PROF_APPLY;
-    prof_begin(h, atag, st);
+    prof_begin(h, atag, s);
 #define SDC_APPLY_PQ(ZT, ZP)                                                                   \
     if (first) {                                                                               \
-      DISPATCH_R(R, hipLaunchKernelGGL((k_apply_pq<ZT, R, true>), g, blk, 0, st, s1, ZP, W.p,  \
-                                       W.q, W.x, (const double*)W.beta,                        \
-                                       (const double*)W.alpha, l0.ld, ncol, W.partial, 1));    \
+      DISPATCH_R(R, launch(h, (k_apply_pq<ZT, R, true>), g, blk, 0, s, s1, ZP, W.p, W.q, W.x,  \
+                           (const double*)W.beta, (const double*)W.alpha, l0.ld, ncol,         \
+                           W.partial, (const int*)W.ctl));                                     \
     } else {                                                                                   \
-      DISPATCH_R(R, hipLaunchKernelGGL((k_apply_pq<ZT, R, false>), g, blk, 0, st, s1, ZP, W.p, \
-                                       W.q, W.x, (const double*)W.beta,                        \
-                                       (const double*)W.alpha, l0.ld, ncol, W.partial,         \
-                                       xfresh ? 1 : 0));                                       \
+      DISPATCH_R(R, launch(h, (k_apply_pq<ZT, R, false>), g, blk, 0, s, s1, ZP, W.p, W.q, W.x, \
+                           (const double*)W.beta, (const double*)W.alpha, l0.ld, ncol,         \
+                           W.partial, (const int*)W.ctl));                                     \
     }
     if (zb) {
       SDC_APPLY_PQ(__bf16, (const __bf16*)z16)
@@ -1102,81 +1126,81 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       SDC_APPLY_PQ(float, (const float*)W.z)
     }
 #undef SDC_APPLY_PQ
-    prof_end(h, atag, st,
-             (double)l0.nN * 16 + (double)l0.nN * ncol * (bz + (xfresh ? 20.0 : 24.0)));
-    if (!first) xfresh = false;
-    hipLaunchKernelGGL(k_fin_alpha, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
-                       (const double*)W.rz, W.alpha, (const int*)W.active, done);
-    prof_begin(h, SDC_PROF_UPDATE, st);
-    if (r16) {
-      hipLaunchKernelGGL(k_update_r4<true>, grid4(l0.nN, ncol), blk, 0, st, W.r,
-                         (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16);
-    } else {
-      hipLaunchKernelGGL(k_update_r4<false>, grid4(l0.nN, ncol), blk, 0, st, W.r,
-                         (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16);
+    // algorithmic bytes of the steady state (x read and written)
+    prof_end(h, atag, s, (double)l0.nN * 16 + (double)l0.nN * ncol * (bz + 24.0));
+    launch(h, k_fin_alpha, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx,
+           (const double*)W.rz, W.alpha, (const int*)W.active, stop);
+    prof_begin(h, SDC_PROF_UPDATE, s);
+    if (r16)
+      launch(h, k_update_r4<true>, grid4(l0.nN, ncol), blk, 0, s, W.r, (const float*)W.q,
+             (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
+    else
+      launch(h, k_update_r4<false>, grid4(l0.nN, ncol), blk, 0, s, W.r, (const float*)W.q,
+             (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
+    prof_end(h, SDC_PROF_UPDATE, s, (double)l0.nN * ncol * (r16 ? 14 : 12));
+    launch(h, k_fin_norm, dim3(ncol), blk, 0, s, (const double*)W.partial, nblk4(l0.nN), W.rr,
+           W.active, 0, stop);
+    launch(h, k_decide, dim3(1), blk, 0, s, W.ctl, (const double*)W.rr, (const double*)W.aux,
+           (const double*)W.rmax, (const int*)W.active, ncol, tol2, delta2, maxit,
+           kStallWindow);
+    // reliable update (every kernel below returns at once unless k_decide asked for it).  A
+    // profiled run brackets the fold and the fp64 residual only when they do work, so it reads
+    // the decision on the host (graphs are off while profiling).
+    bool upd = true;
+    if (prof_on(h, SDC_PROF_REFINE) || prof_on(h, SDC_PROF_FOLD)) {
+      HIPCK(hipMemcpyAsync(h->h_ctl, W.ctlmem, SDC_CTL_N * sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCK(hipStreamSynchronize(s));
+      const int* hc = (const int*)h->h_ctl;
+      upd = hc[SDC_CTL_STOP] == 0 && hc[SDC_CTL_UPD] != 0;
+      fold_fresh = hc[SDC_CTL_UFRESH] != 0;
     }
-    prof_end(h, SDC_PROF_UPDATE, st, (double)l0.nN * ncol * (r16 ? 14 : 12));
-    hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nblk4(l0.nN),
-                       W.rr, W.active, 0);
-    if (it % check == 0 || it == maxit) {
-      HIPCK(hipMemcpyAsync(h_rr.data(), W.rr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
-      HIPCK(hipStreamSynchronize(st));
-      bool cand = true, drop = false;
-      for (int c = 0; c < ncol; ++c) {
-        if (!act[c]) continue;
-        if (h_rr[c] > tol2 * h_qn[c]) cand = false;
-        if (h_rr[c] <= delta2 * h_rmax[c]) drop = true;
-      }
-      if (cand || drop || it == maxit || it - last_update >= kStallWindow) {
-        prof_begin(h, SDC_PROF_FOLD, st);
-        if (ufresh) {
-          hipLaunchKernelGGL(k_ir_fold_p4<true>, grid4((uint32_t)l0.ld, ncol), blk, 0, st, u,
-                             (const float*)W.x, (const float*)W.p, (const double*)W.alpha, l0.ld,
-                             l0.nN, xfresh ? 1 : 0);
-        } else {
-          hipLaunchKernelGGL(k_ir_fold_p4<false>, grid4(l0.nN, ncol), blk, 0, st, u,
-                             (const float*)W.x, (const float*)W.p, (const double*)W.alpha, l0.ld,
-                             l0.nN, xfresh ? 1 : 0);
-        }
-        const double fold_bytes = (double)l0.nN * ncol * (ufresh ? 16 : 24);
-        ufresh = false;
-        prof_end(h, SDC_PROF_FOLD, st, fold_bytes);
-        hipLaunchKernelGGL(k_zero<double>, dim3(nblk(ncol)), blk, 0, st, W.alpha, (size_t)ncol);
-        xfresh = true;
-        prof_begin(h, SDC_PROF_REFINE, st);
-        // fp64 stencil: refine_rhs_per_thread columns per work-item (default 8; its coefficients
-        // are fp64, so more columns per work-item save more traffic than in the fp32 kernels)
-        const int R64 = h->prm.refine_rhs_per_thread > 0 ? h->prm.refine_rhs_per_thread
-                        : ncol >= 8                      ? 8
-                                                         : R;
-        DISPATCH_R(R64, hipLaunchKernelGGL((k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64),
-                                           blk, 0, st, s2, q, (const double*)u, W.r, l0.ld, ncol,
-                                           pf, W.partial, (unsigned*)W.qmask, r16));
-        // algorithmic bytes: fp64 coefficients, per column u64 read and r32 write; q is read
-        // on source nodes only (not counted)
-        prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
-        hipLaunchKernelGGL(k_fin_true, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
-                           (const double*)W.aux, tol2, W.orr, W.rr, W.active);
-        HIPCK(hipMemcpyAsync(h_rr.data(), W.orr, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCK(hipStreamSynchronize(st));
-        S.outer += 1;
-        nact = 0;
-        double m = 0.0;
-        for (int c = 0; c < ncol; ++c) {
-          h_rmax[c] = h_rr[c];
-          act[c] = h_qn[c] > 0.0 && h_rr[c] > tol2 * h_qn[c];
-          nact += act[c];
-          if (h_qn[c] > 0.0) m = std::max(m, std::sqrt(h_rr[c] / h_qn[c]));
-        }
-        S.max_rel_res = m;
-        if (nact == 0 || it == maxit || m > 0.5 * m_prev) break;
-        m_prev = m;
-        last_update = it;
-      }
-    }
+    prof_begin(h, upd ? SDC_PROF_FOLD : -1, s);
+    launch(h, k_ir_fold_ctl, grid4((uint32_t)l0.ld, ncol), blk, 0, s,
+           (const double* const*)W.qu, (const float*)W.x, (const float*)W.p,
+           (const double*)W.alpha, l0.ld, l0.nN, (const int*)W.ctl);
+    prof_end(h, upd ? SDC_PROF_FOLD : -1, s, (double)l0.nN * ncol * (fold_fresh ? 16 : 24));
+    prof_begin(h, upd ? SDC_PROF_REFINE : -1, s);
+    DISPATCH_R(R64, launch(h, (k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64), blk, 0,
+                           s, s2, (const double*)nullptr, (const double*)nullptr, W.r, l0.ld,
+                           ncol, pf, W.partial, (unsigned*)W.qmask, r16, (const int*)W.ctl,
+                           (const double* const*)W.qu));
+    // algorithmic bytes: fp64 coefficients, per column u64 read and r32 write; q is read on
+    // source nodes only (not counted)
+    prof_end(h, upd ? SDC_PROF_REFINE : -1, s, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
+    launch(h, k_fin_true, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx,
+           (const double*)W.aux, tol2, W.orr, W.rmax, W.active, (const int*)W.ctl);
+    launch(h, k_after_update, dim3(1), blk, 0, s, W.ctl, W.dctl, (const double*)W.orr,
+           (const double*)W.aux, (const int*)W.active, W.alpha, ncol, maxit);
+    return SDC_OK;
+  };
+  // the first iteration (FIRST variants) is launched; later ones run check_every at a time,
+  // then the host reads the control block
+  CK(iteration(true, st));
+  const int K = std::max(1, (int)h->prm.check_every);
+  // profiled kernel classes are bracketed by events, which cannot sit inside a graph
+  const bool rec = h->prm.graph && h->prof_mask == 0;
+  const int variant = 1000 + 16 * K + (zb ? 1 : 0) + (r16 ? 2 : 0) + 4 * (R64 == 8);
+  const int* h_ctl = (const int*)h->h_ctl;
+  for (int done_it = 1;; done_it += K) {
+    auto block = [&](hipStream_t s) -> int {
+      for (int k = 0; k < K; ++k) CK(iteration(false, s));
+      return SDC_OK;
+    };
+    if (rec)
+      CK(graph_run(h, W.ctl, (void*)W.partial, ncol, variant, st, block));
+    else
+      CK(block(st));
+    HIPCK(hipMemcpyAsync(h->h_ctl, W.ctlmem, (4 + SDC_DCTL_N) * sizeof(double),
+                         hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (h_ctl[SDC_CTL_STOP]) break;
+    // the device stops at maxit by itself; this bound only guards the host loop
+    if (done_it > maxit + K) return fail(SDC_ERR_STATE, "internal: mixed PCG did not stop");
   }
-  S.iters = std::min(it, maxit);
-  S.converged = nact == 0;
+  S.iters = std::min(h_ctl[SDC_CTL_IT], maxit);
+  S.outer = h_ctl[SDC_CTL_OUTER];
+  S.max_rel_res = h->h_ctl[4 + SDC_DCTL_MREL];
+  S.converged = h_ctl[SDC_CTL_REASON] == 1;
   HIPCK(hipGetLastError());
   return SDC_OK;
 }
@@ -1353,6 +1377,11 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
     h->h_flag = nullptr;
     return bail(fail(SDC_ERR_ALLOC, "hipHostMalloc failed"));
   }
+  if (hipHostMalloc((void**)&h->h_ctl, (4 + SDC_DCTL_N) * sizeof(double), hipHostMallocDefault) !=
+      hipSuccess) {
+    h->h_ctl = nullptr;
+    return bail(fail(SDC_ERR_ALLOC, "hipHostMalloc failed"));
+  }
   // the fused coarse tail may use all 160 KB of LDS (set once, outside any graph capture)
   if (hipFuncSetAttribute((const void*)k_tail_vcycle<__bf16>,
                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
@@ -1376,6 +1405,7 @@ int sdc_destroy(sdc_handle h) {
   h->w32.release();
   h->w64.release();
   if (h->h_flag) (void)hipHostFree(h->h_flag);
+  if (h->h_ctl) (void)hipHostFree(h->h_ctl);
   drop_graphs(h);
   for (auto& p : h->prof)
     for (auto e : p.ev) (void)hipEventDestroy(e);
@@ -1609,16 +1639,16 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, true>), g, blk, 0, st, s2, q,
                                      (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask,
-                                     (__bf16*)nullptr));
+                                     (__bf16*)nullptr, (const int*)nullptr, (const double* const*)nullptr));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
-                       W.active, 0);
+                       W.active, 0, (const int*)nullptr);
     CK(pcg<double>(h, u, ncol, o->tol, o->max_iter, nullptr, iters, st));
     S.outer = 1;
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
                                      (const double*)u, W.z, l0.ld, ncol, pf, W.partial,
-                                     W.qmask, (__bf16*)nullptr));
+                                     W.qmask, (__bf16*)nullptr, (const int*)nullptr, (const double* const*)nullptr));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
-                       W.active, 0);
+                       W.active, 0, (const int*)nullptr);
     CK(rel_max(W.orr, W.aux));
     // converged when every column met the PCG test (recursive residual)
     HIPCK(hipMemcpyAsync(h->h_flag, W.counters + 2, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1642,7 +1672,7 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
       const Work<float>& W32 = h->w32;
       DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
                                        (const double*)u, W.r, l0.ld, ncol, pf, W.partial,
-                                       W32.qmask, (__bf16*)nullptr));
+                                       W32.qmask, (__bf16*)nullptr, (const int*)nullptr, (const double* const*)nullptr));
       const double tin = std::min(0.5, 0.5 * o->tol / std::max(S.max_rel_res, 1e-300));
       int it = 0;
       CK(pcg<double>(h, W.x, ncol, tin, o->max_iter, nullptr, &it, st));
@@ -1651,9 +1681,9 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
                                        (const double*)W.x, l0.ld, l0.nN, ncol));
       DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<double, R, false>), g, blk, 0, st, s2, q,
                                        (const double*)u, W.z, l0.ld, ncol, pf, W.partial,
-                                       W32.qmask, (__bf16*)nullptr));
+                                       W32.qmask, (__bf16*)nullptr, (const int*)nullptr, (const double* const*)nullptr));
       hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
-                         W.active, 0);
+                         W.active, 0, (const int*)nullptr);
       CK(rel_max(W.orr, W32.aux));
       S.outer += 1;
       S.converged = S.max_rel_res <= o->tol ? 1 : 0;
@@ -1669,21 +1699,21 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
     hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
     DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, true>), g, blk, 0, st, s2, q,
                                      (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask,
-                                     (__bf16*)nullptr));
+                                     (__bf16*)nullptr, (const int*)nullptr, (const double* const*)nullptr));
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
-                       W.active, 0);
+                       W.active, 0, (const int*)nullptr);
     hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.orr,
-                       W.active, 0);
+                       W.active, 0, (const int*)nullptr);
     const double tol2 = o->tol * o->tol;
     for (int outer = 0;; ++outer) {
       if (outer > 0) {
         prof_begin(h, SDC_PROF_REFINE, st);
         DISPATCH_R(R, hipLaunchKernelGGL((k_ir_residual<float, R, false>), g, blk, 0, st, s2, q,
                                          (const double*)u, W.r, l0.ld, ncol, pf, W.partial,
-                                         W.qmask, (__bf16*)nullptr));
+                                         W.qmask, (__bf16*)nullptr, (const int*)nullptr, (const double* const*)nullptr));
         prof_end(h, SDC_PROF_REFINE, st, (double)l0.nN * 32 + (double)l0.nN * ncol * 20);
         hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx,
-                           W.orr, W.active, 0);
+                           W.orr, W.active, 0, (const int*)nullptr);
       }
       hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.ocount, (size_t)1);
       hipLaunchKernelGGL(k_outer_check, dim3(nblk(ncol)), blk, 0, st, (const double*)W.orr,
