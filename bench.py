@@ -325,9 +325,10 @@ def _range(name):
 
 def batch_result(args):
     """48 independent problems (uneven sizes / source counts) behind MetaSimulation: fields +
-    Jtvec for all of them, one stream after the other vs concurrently on 8 streams.  Multigrid
-    settings come from the autotuner (tuned during the one-stream warm-up, cached per mesh shape
-    and column bucket).  Returns the result dict."""
+    Jtvec for all of them, one stream after the other vs concurrently on 8 streams.  Problems of
+    one mesh size form a group solved as operator sets of one engine (one batched solve per
+    group).  Multigrid settings come from the autotuner (tuned during the one-stream warm-up,
+    cached per mesh shape and column bucket).  Returns the result dict."""
     import torch
 
     from simpeg_amd import maps, workloads
@@ -362,12 +363,16 @@ def batch_result(args):
         t2 = time.perf_counter()
         res[ns] = (t1 - t0, t2 - t1)
         log(f"[bench] batch streams={ns}: fields {1e3*(t1-t0):.1f} ms, Jtvec {1e3*(t2-t1):.1f} ms")
+    from simpeg_amd.meta import _SetGroup
+
+    groups = sorted((len(u.members) for u in meta._units if isinstance(u, _SetGroup)), reverse=True)
     out = {"metric": "uneven batch: source-solves/s (fields of 48 problems)", "unit": "source-solves/s",
            "value": round(nsrc / res[8][0], 2), "n_gpus": 1, "higher_is_better": True,
            "config": {"workload": "uneven_batch_48", "problems": len(batch), "sources": nsrc,
                       "node_columns": int(sum(wl.mesh.n_nodes * wl.survey.nSrc for wl in batch))},
            "fields_ms": {str(k): round(1e3 * v[0], 1) for k, v in res.items()},
            "jtvec_ms": {str(k): round(1e3 * v[1], 1) for k, v in res.items()},
+           "same_mesh_groups": groups,
            "autotuned": True,
            "concurrency_speedup": round(sum(res[1]) / sum(res[8]), 2)}
     return out

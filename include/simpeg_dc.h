@@ -81,7 +81,8 @@ typedef struct {
   double cheb_ratio;           /* lower bound = cheb_lmax / cheb_ratio */
   int32_t rhs_per_thread;      /* stencil kernels: 0 = automatic (4), else 1, 2, 4 or 8 columns
                                   per work-item */
-  int32_t check_every;         /* PCG iterations between host convergence polls */
+  int32_t check_every;         /* PCG iterations recorded per graph replay (or launched) between
+                                  two host reads of the device-side stop flag */
   double reliable_delta;       /* mixed mode: replace the fp32 residual by the fp64 true residual
                                   once it dropped by this factor (0 < delta < 1) */
   int32_t xfer_rhs_per_thread; /* grid transfers: 0 = automatic (8), else 1, 2, 4 or 8 */
@@ -161,6 +162,15 @@ int sdc_set_params(sdc_handle h, const sdc_params* p);
  * `self.Ainv = self.solver(A, **self.solver_opts)` (resistivity/simulation.py:108). */
 int sdc_set_sigma(sdc_handle h, const double* sigma_dev, void* stream);
 
+/* Several operators on one mesh ("operator sets"): set s is built from the cell conductivity
+ * sigma_dev + s * nC (device fp64, nset x nC), with every multigrid level and coarse inverse, in
+ * one pass over all sets.  Replaces, for MetaSimulation's sub-simulations that share a mesh
+ * (meta/simulation.py:13-330), their separate getA + factorisation calls
+ * (resistivity/simulation.py:108, 523-555).  Column solves then pick their set with
+ * sdc_solve_sets; sdc_solve, sdc_apply and sdc_vcycle use set 0.  Growing the number of sets
+ * reallocates the per-set arrays. */
+int sdc_set_sigma_sets(sdc_handle h, int nset, const double* sigma_dev, void* stream);
+
 /* Set the finest level's operator directly (device fp64, nN each): cx/cy/cz = conductance of the
  * +x/+y/+z edge of every node (0 where there is no such edge), bd = the rest of the diagonal
  * (Robin term).  For the solver plugin point, which receives an assembled operator instead of
@@ -181,6 +191,16 @@ int sdc_vcycle(sdc_handle h, int dtype, const void* b_dev, void* z_dev, int ncol
  * (resistivity/simulation.py:111, 181, 245).  stats may be NULL. */
 int sdc_solve(sdc_handle h, const double* q_dev, double* u_dev, int ncol,
               const sdc_solve_opts* opts, sdc_solve_stats* stats, void* stream);
+
+/* sdc_solve with column c solved against operator set colset[c] (host array of ncol entries,
+ * sets built by sdc_set_sigma_sets; each aligned group of 8 columns must use one set, so pad a
+ * set's columns to a multiple of 8 with zero right-hand sides).  One batched solve for the
+ * sources of many same-mesh problems: `Ainv * RHS` of every sub-simulation of a MetaSimulation
+ * (resistivity/simulation.py:111, 181, 245; meta/simulation.py:194-330).  stats cover all
+ * columns. */
+int sdc_solve_sets(sdc_handle h, const double* q_dev, double* u_dev, int ncol,
+                   const int32_t* colset, const sdc_solve_opts* opts, sdc_solve_stats* stats,
+                   void* stream);
 
 /* out = -A(dsig) u  (no Neumann row replacement): the Jvec right-hand side -getADeriv(u, v)
  * with dsig = sigmaDeriv @ v (resistivity/simulation.py:557-565, 177-181). */

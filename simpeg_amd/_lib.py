@@ -97,10 +97,13 @@ _SIGS = {
     "sdc_default_params": (C.c_int, [C.POINTER(Params)]),
     "sdc_set_params": (C.c_int, [_vp, C.POINTER(Params)]),
     "sdc_set_sigma": (C.c_int, [_vp, _vp, _vp]),
+    "sdc_set_sigma_sets": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "sdc_set_coefficients": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sdc_apply": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     "sdc_vcycle": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     "sdc_solve": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(SolveOpts), C.POINTER(SolveStats), _vp]),
+    "sdc_solve_sets": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.POINTER(SolveOpts),
+                                 C.POINTER(SolveStats), _vp]),
     "sdc_jvec_rhs": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "sdc_jtvec_contract": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int64, _vp]),
     "sdc_jtvec_contract_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp]),

@@ -31,6 +31,7 @@ CANDIDATES = [
     dict(_BASE, cheb_ratio=4.0),
     dict(_BASE, nu_pre=3, nu_post=3),
     dict(_BASE, check_every=2),
+    dict(_BASE, check_every=4),
     dict(_BASE, rhs_per_thread=8, xfer_rhs_per_thread=8),
     dict(_BASE, vcycle_bf16=2),
     # few columns: the fused coarse tail runs one work-group per column, so it can idle most

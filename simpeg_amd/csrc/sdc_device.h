@@ -59,7 +59,25 @@ struct Sten {
   FastDiv fx, fy;
   int pin0;
   int remap;  // 1: XCD-contiguous work-group order
+  // Operator sets (several operators on one mesh, one per column group): column c uses the
+  // arrays at offset cset[c] * sstride.  cset null: one operator.
+  const int32_t* cset;
+  uint32_t sstride;
 };
+
+// The operator of column c (every column of a work-item's group shares one set).
+template <typename T>
+__device__ __forceinline__ Sten<T> sten_col(Sten<T> s, int c) {
+  if (s.cset) {
+    const size_t o = (size_t)s.cset[c] * s.sstride;
+    s.cx += o;
+    s.cy += o;
+    s.cz += o;
+    s.dg += o;
+    s.idg += o;
+  }
+  return s;
+}
 
 // Per-node stencil data.  The neighbour indices of a missing neighbour (domain edge) are the
 // node itself and its weight is 0, so stencil loads need no branches and stay in bounds.
@@ -101,6 +119,16 @@ __device__ __forceinline__ NodeC<T> node_coef(const Sten<T>& s, uint32_t n) {
   c.wzm = bz * (T)(hzm && !(p1 && n == s.plane));
   c.d = s.dg[n];
   return c;
+}
+
+// Flags and counters written by an earlier kernel of the same stream (convergence flags, the
+// PCG control block) are read from L2 with device-scope loads, never through a compute unit's
+// scalar cache, which may still hold the line from before that kernel wrote it.
+__device__ __forceinline__ int ld_flag(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_flag(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Loads/stores of operands that a kernel touches exactly once (right-hand sides, search

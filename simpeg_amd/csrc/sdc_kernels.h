@@ -32,7 +32,9 @@
 // returns at once), UPD asks for a reliable update in the current iteration, XFRESH / UFRESH
 // mark x as already folded into u / u as not yet written, IT counts iterations, LAST is the
 // iteration of the last update, REASON says why the solve stopped (1 converged, 2 iteration
-// limit, 3 stalled) and OUTER counts reliable updates.
+// limit, 3 stalled) and OUTER counts reliable updates; MAXIT and WINDOW are the iteration limit
+// and the stall window of the solve.  The solve's parameters live here, not in kernel
+// arguments, so that a recorded iteration serves solves with any tolerance or limit.
 #define SDC_CTL_STOP 0
 #define SDC_CTL_UPD 1
 #define SDC_CTL_XFRESH 2
@@ -41,27 +43,38 @@
 #define SDC_CTL_LAST 5
 #define SDC_CTL_REASON 6
 #define SDC_CTL_OUTER 7
-#define SDC_CTL_N 8
-// doubles beside it: the largest true relative residual of the previous update, of the last one
+#define SDC_CTL_MAXIT 8
+#define SDC_CTL_WINDOW 9
+#define SDC_CTL_N 16
+// doubles after the ints: the largest true relative residual of the previous update and of the
+// last one, tol^2 and reliable_delta^2
 #define SDC_DCTL_MPREV 0
 #define SDC_DCTL_MREL 1
-#define SDC_DCTL_N 2
+#define SDC_DCTL_TOL2 2
+#define SDC_DCTL_DELTA2 3
+#define SDC_DCTL_N 4
+
+__device__ __forceinline__ double* ctl_dbl(int* ctl) { return (double*)(ctl + SDC_CTL_N); }
+__device__ __forceinline__ const double* ctl_dbl(const int* ctl) {
+  return (const double*)(ctl + SDC_CTL_N);
+}
 
 // a reliable update runs in this iteration
 __device__ __forceinline__ bool ctl_update(const int* ctl) {
-  return ctl[SDC_CTL_STOP] == 0 && ctl[SDC_CTL_UPD] != 0;
+  return ld_flag(ctl + SDC_CTL_STOP) == 0 && ld_flag(ctl + SDC_CTL_UPD) != 0;
 }
 
 // ------------------------------------------------------------------------------------------
 // operator application: y = A x, optional per-column partial sums of x.y
 
 template <typename T, int R, bool DOT>
-__global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> s, const T* __restrict__ x,
+__global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> sa, const T* __restrict__ x,
                                                      T* __restrict__ y, size_t ld, int ncol,
                                                      double* __restrict__ partial,
                                                      const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
+  if (done && ld_flag(done)) return;
+  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
+  const Sten<T> s = sten_col(sa, c0);
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -82,12 +95,13 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> s, const T* __restr
 
 // r = b - A x.  b in type B, x and r in the V-cycle storage type X, arithmetic in T.
 template <typename T, typename B, typename X, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const B* __restrict__ b,
+__global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> sa, const B* __restrict__ b,
                                                         const X* __restrict__ x,
                                                         X* __restrict__ res, size_t ld, int ncol,
                                                         const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
+  if (done && ld_flag(done)) return;
+  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
+  const Sten<T> s = sten_col(sa, c0);
   if (!live) return;
   const NodeC<T> c = node_coef(s, n);
   T xc[R], v[R], bv[R];
@@ -101,11 +115,12 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> s, const B* __re
 
 // First Chebyshev step from a zero guess: x = c0 * D^-1 b
 template <typename T, typename B, typename X, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const B* __restrict__ b,
+__global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> sa, const B* __restrict__ b,
                                                        X* __restrict__ x, size_t ld, int ncol,
                                                        T coef, const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
+  if (done && ld_flag(done)) return;
+  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
+  const Sten<T> s = sten_col(sa, c0);
   if (!live) return;
   const T w = coef * s.idg[n];
 #pragma unroll
@@ -126,13 +141,14 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> s, const B* __res
 // b in type B, x and xprev in the storage type X, xout in type O (O = T on the last step of
 // the finest level, which hands z back to PCG).
 template <typename T, typename B, typename X, typename O, int R, int PREV, bool DOT>
-__global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const B* __restrict__ b,
+__global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> sa, const B* __restrict__ b,
                                                     const X* __restrict__ x, const X* xprev,
                                                     O* xout, size_t ld, int ncol, T c1, T c2,
                                                     T cz, double* __restrict__ partial,
                                                     const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
+  if (done && ld_flag(done)) return;
+  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
+  const Sten<T> s = sten_col(sa, c0);
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -168,12 +184,13 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> s, const B* __restri
 //   x1 = c0 D^-1 b (recomputed at the 7 stencil points, never stored), r1 = b - A x1,
 //   x2 = x1 + c1 x1 + c2 D^-1 r1.  Writes x2 only (a following step recomputes x1 from b).
 template <typename T, typename B, typename X, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_smooth_pair(Sten<T> s, const B* __restrict__ b,
+__global__ __launch_bounds__(SDC_BLOCK) void k_smooth_pair(Sten<T> sa, const B* __restrict__ b,
                                                            X* __restrict__ xout, size_t ld,
                                                            int ncol, T cz, T c1, T c2,
                                                            const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
+  if (done && ld_flag(done)) return;
+  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
+  const Sten<T> s = sten_col(sa, c0);
   if (!live) return;
   NodeC<T> c = node_coef(s, n);
   const T g = s.idg[n];
@@ -233,7 +250,22 @@ struct Xfer {
   const T* wx;
   const T* wy;
   const T* wz;
+  // operator sets: column c uses wx + cset[c] * wxs (and likewise wy, wz); cset null: one set
+  const int32_t* cset;
+  uint32_t wxs, wys, wzs;
 };
+
+// The transfer of column c (operator-dependent weights of its operator set).
+template <typename T>
+__device__ __forceinline__ Xfer<T> xfer_col(Xfer<T> t, int c) {
+  if (t.cset && t.wx) {
+    const size_t k = (size_t)t.cset[c];
+    t.wx += k * t.wxs;
+    t.wy += k * t.wys;
+    t.wz += k * t.wzs;
+  }
+  return t;
+}
 
 // Weights of the 8 coarse nodes around fine node n = (i, j, k); corner a + 2b + 4c is
 // (I0 + a, J0 + b, K0 + c).  P = Px Py Pz makes the corner weight the product
@@ -321,9 +353,14 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_interp_weights(Xfer<double> t, ui
                                                               uint32_t g1, uint32_t g2,
                                                               const double* __restrict__ c,
                                                               double* __restrict__ w64,
-                                                              float* __restrict__ w32) {
+                                                              float* __restrict__ w32,
+                                                              size_t cstride, size_t wstride) {
   const size_t n = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
   if (n >= (size_t)g0 * g1 * g2) return;
+  // operator set blockIdx.z: its conductances and its weights
+  c += (size_t)blockIdx.z * cstride;
+  w64 += (size_t)blockIdx.z * wstride;
+  w32 += (size_t)blockIdx.z * wstride;
   const uint32_t p0 = (uint32_t)(n % g0);
   const size_t q = n / g0;
   const uint32_t p1 = (uint32_t)(q % g1);
@@ -352,11 +389,12 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_interp_weights(Xfer<double> t, ui
 // (restrict_weights); a missing neighbour (domain edge or coincident coarse node) has weight 0
 // and its index is clamped into the grid.
 template <typename T, typename X, int R>
-__global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> t, const X* __restrict__ rf,
+__global__ __launch_bounds__(SDC_BLOCK) void k_restrict(Xfer<T> ta, const X* __restrict__ rf,
                                                         X* __restrict__ rc, size_t ldf, size_t ldc,
                                                         int ncol, const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE_R(t.cN, t.remap)
+  if (done && ld_flag(done)) return;
+  SDC_NODE_PROLOGUE_R(ta.cN, ta.remap)
+  const Xfer<T> t = xfer_col(ta, c0);
   if (!live) return;
   const uint32_t rr = fdiv(n, t.cfx);
   const uint32_t I = n - rr * t.cn1x;
@@ -406,6 +444,8 @@ struct AxisXfer {
   uint32_t a0, a1, a2, b0, b1, b2, nout;
   FastDiv f0, f1;  // division by b0 and b1
   int remap;
+  const int32_t* cset;  // operator sets: wop + cset[c] * wstride (null: one set)
+  uint32_t wstride;
 };
 
 // OPD: operator-dependent weights wop (else the geometric wlo / whi alone).  The weight loads
@@ -420,7 +460,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const i
                                                              X* __restrict__ out, size_t ldi,
                                                              size_t ldo, int ncol,
                                                              const int* __restrict__ done) {
-  if (done && *done) return;
+  if (done && ld_flag(done)) return;
   SDC_NODE_PROLOGUE_R(t.nout, t.remap)
   if (!live) return;
   const uint32_t q = fdiv(n, t.f0);
@@ -440,6 +480,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const i
   // operator-dependent weights (wop, over this pass's input grid): the left neighbour gives
   // its weight of the right coarse node, the right neighbour 1 minus its own
   T wl = wlo[C], wh = whi[C];
+  if (OPD && t.cset) wop += (size_t)t.cset[c0] * t.wstride;
   if (OPD) {
     wl = wop[il] * (T)(wl != T(0));
     wh = (T(1) - wop[ih]) * (T)(wh != T(0));
@@ -466,12 +507,13 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const i
 // (none depends on another load), then the weights and coarse values addressed through the
 // parents -- two memory round trips per wavefront.
 template <typename T, typename X, int R, bool OPD>
-__global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* __restrict__ xc,
+__global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> ta, const X* __restrict__ xc,
                                                            X* __restrict__ xf, size_t ldf,
                                                            size_t ldc, int ncol,
                                                            const int* __restrict__ done) {
-  if (done && *done) return;
-  SDC_NODE_PROLOGUE_R(t.fN, t.remap)
+  if (done && ld_flag(done)) return;
+  SDC_NODE_PROLOGUE_R(ta.fN, ta.remap)
+  const Xfer<T> t = xfer_col(ta, c0);
   if (!live) return;
   const uint32_t rr = fdiv(n, t.ffx);
   const uint32_t i = n - rr * t.fn1x;
@@ -533,15 +575,19 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> t, const X* _
 
 // coarsest level: x = Ainv b (dense inverse, n x n; symmetric, so column m is read as row m
 // and consecutive work-items read consecutive addresses)
+// (operator sets: column col uses ainv + cset[col] * astride)
 template <typename T, typename B, typename O>
 __global__ __launch_bounds__(SDC_BLOCK) void k_dense_solve(const T* __restrict__ ainv, int nn,
                                                            const B* __restrict__ b,
                                                            O* __restrict__ x, size_t ld, int ncol,
-                                                           const int* __restrict__ done) {
-  if (done && *done) return;
+                                                           const int* __restrict__ done,
+                                                           const int32_t* __restrict__ cset,
+                                                           size_t astride) {
+  if (done && ld_flag(done)) return;
   const int row = blockIdx.x * SDC_BLOCK + threadIdx.x;
   const int col = blockIdx.y;
   if (row >= nn || col >= ncol) return;
+  if (cset) ainv += (size_t)cset[col] * astride;
   const B* bc = b + (size_t)col * ld;
   T acc = T(0);
   for (int m = 0; m < nn; ++m) acc += ainv[(size_t)m * nn + row] * (T)bc[m];
@@ -556,7 +602,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_dot(const T* __restrict__ a, cons
                                                    size_t ld, uint32_t nN, int ncol,
                                                    double* __restrict__ partial,
                                                    const int* __restrict__ done) {
-  if (done && *done) return;
+  if (done && ld_flag(done)) return;
   SDC_NODE_PROLOGUE(nN)
   double acc[R];
 #pragma unroll
@@ -598,7 +644,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* const* xp, const T*
                                                           size_t ld, uint32_t nN,
                                                           double* __restrict__ partial,
                                                           const int* __restrict__ done) {
-  if (done && *done) return;
+  if (done && ld_flag(done)) return;
   T* __restrict__ x = xp[0];
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
@@ -632,9 +678,10 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* const* xp, const T*
 // p, and p, q are updated pointwise.  The x update of the previous step, x += alpha_prev p_old,
 // rides along (it needs p_old, which is read here anyway); partial sums of p.q per column.
 // FIRST: p = z, q = A z, no x update.  ctl[XFRESH]: x was folded away and counts as zero (not
-// read).
+// read).  check_stop: return at once when ctl[STOP] is set (only needed when several iterations
+// run between host reads of the stop flag).
 template <typename Z, int R, bool FIRST>
-__global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* __restrict__ z,
+__global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z* __restrict__ z,
                                                         float* __restrict__ p,
                                                         float* __restrict__ q,
                                                         float* __restrict__ x,
@@ -642,10 +689,12 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> s, const Z* 
                                                         const double* __restrict__ alpha_prev,
                                                         size_t ld, int ncol,
                                                         double* __restrict__ partial,
-                                                        const int* __restrict__ ctl) {
-  if (ctl[SDC_CTL_STOP]) return;
-  const int xfresh = ctl[SDC_CTL_XFRESH];
-  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
+                                                        const int* __restrict__ ctl,
+                                                        int check_stop) {
+  if (check_stop && ld_flag(ctl + SDC_CTL_STOP)) return;
+  const int xfresh = ld_flag(ctl + SDC_CTL_XFRESH);
+  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
+  const Sten<float> s = sten_col(sa, c0);
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -692,7 +741,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
                                                          double* __restrict__ partial,
                                                          __bf16* __restrict__ r16,
                                                          const int* __restrict__ done) {
-  if (done && *done) return;
+  if (done && ld_flag(done)) return;
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
   const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
@@ -740,7 +789,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_ctl(const double* const* 
                                                            const int* __restrict__ ctl) {
   if (!ctl_update(ctl)) return;
   double* __restrict__ u = const_cast<double*>(qu[1]);
-  const bool ufresh = ctl[SDC_CTL_UFRESH] != 0, xfresh = ctl[SDC_CTL_XFRESH] != 0;
+  const bool ufresh = ld_flag(ctl + SDC_CTL_UFRESH) != 0,
+             xfresh = ld_flag(ctl + SDC_CTL_XFRESH) != 0;
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
   const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
@@ -776,10 +826,11 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_decide(int* __restrict__ ctl,
                                                       const double* __restrict__ rr,
                                                       const double* __restrict__ qn2,
                                                       const double* __restrict__ rmax,
-                                                      const int* __restrict__ active, int ncol,
-                                                      double tol2, double delta2, int maxit,
-                                                      int window) {
-  if (ctl[SDC_CTL_STOP]) return;
+                                                      const int* __restrict__ active, int ncol) {
+  if (ld_flag(ctl + SDC_CTL_STOP)) return;
+  const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
+  const double delta2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_DELTA2);
+  const int maxit = ld_flag(ctl + SDC_CTL_MAXIT), window = ld_flag(ctl + SDC_CTL_WINDOW);
   int cand = 1, drop = 0;
   for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
     if (!active[c]) continue;
@@ -789,11 +840,11 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_decide(int* __restrict__ ctl,
   cand = __syncthreads_and(cand);
   drop = __syncthreads_or(drop);
   if (threadIdx.x == 0) {
-    const int it = ctl[SDC_CTL_IT] + 1;
+    const int it = ld_flag(ctl + SDC_CTL_IT) + 1;
     ctl[SDC_CTL_IT] = it;
     if (it > 1) ctl[SDC_CTL_XFRESH] = 0;
     ctl[SDC_CTL_UPD] =
-        (cand || drop || it >= maxit || it - ctl[SDC_CTL_LAST] >= window) ? 1 : 0;
+        (cand || drop || it >= maxit || it - ld_flag(ctl + SDC_CTL_LAST) >= window) ? 1 : 0;
   }
 }
 
@@ -802,13 +853,13 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_decide(int* __restrict__ ctl,
 // true relative residual failed to halve since the previous update (the fp32 recursion stalled;
 // the caller continues in fp64).  rmax and active were set by k_fin_true.
 __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ctl,
-                                                            double* __restrict__ dctl,
                                                             const double* __restrict__ orr,
                                                             const double* __restrict__ qn2,
                                                             const int* __restrict__ active,
-                                                            double* __restrict__ alpha, int ncol,
-                                                            int maxit) {
+                                                            double* __restrict__ alpha,
+                                                            int ncol) {
   if (!ctl_update(ctl)) return;
+  const int maxit = ld_flag(ctl + SDC_CTL_MAXIT);
   __shared__ double sm[SDC_BLOCK / 64];
   __shared__ int sn[SDC_BLOCK / 64];
   double m = 0.0;
@@ -833,8 +884,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ct
       m = fmax(m, sm[w]);
       nact += sn[w];
     }
-    const int it = ctl[SDC_CTL_IT];
-    ctl[SDC_CTL_OUTER] += 1;
+    double* dctl = ctl_dbl(ctl);
+    const int it = ld_flag(ctl + SDC_CTL_IT);
+    ctl[SDC_CTL_OUTER] = ld_flag(ctl + SDC_CTL_OUTER) + 1;
     ctl[SDC_CTL_UPD] = 0;
     ctl[SDC_CTL_XFRESH] = 1;
     ctl[SDC_CTL_UFRESH] = 0;
@@ -844,7 +896,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ct
       reason = 1;
     else if (it >= maxit)
       reason = 2;
-    else if (m > 0.5 * dctl[SDC_DCTL_MPREV])
+    else if (m > 0.5 * ld_flag(dctl + SDC_DCTL_MPREV))
       reason = 3;
     if (reason) {
       ctl[SDC_CTL_REASON] = reason;
@@ -856,24 +908,38 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ct
   }
 }
 
-// Start of a device-driven solve: the control block, and q, u in the pointer table.
-__global__ void k_ctl_init(int* __restrict__ ctl, double* __restrict__ dctl,
-                           const double** __restrict__ qu, const double* q, double* u) {
+// Start of a device-driven solve: the control block with the solve's parameters, and q, u in
+// the pointer table.
+__global__ void k_ctl_init(int* __restrict__ ctl, const double** __restrict__ qu,
+                           const double* q, double* u, double tol2, double delta2, int maxit,
+                           int window) {
   if (threadIdx.x == 0) {
     for (int i = 0; i < SDC_CTL_N; ++i) ctl[i] = 0;
     ctl[SDC_CTL_XFRESH] = 1;
     ctl[SDC_CTL_UFRESH] = 1;
-    dctl[SDC_DCTL_MPREV] = 1.0;
-    dctl[SDC_DCTL_MREL] = 0.0;
+    ctl[SDC_CTL_MAXIT] = maxit;
+    ctl[SDC_CTL_WINDOW] = window;
+    double* d = ctl_dbl(ctl);
+    d[SDC_DCTL_MPREV] = 1.0;
+    d[SDC_DCTL_MREL] = 0.0;
+    d[SDC_DCTL_TOL2] = tol2;
+    d[SDC_DCTL_DELTA2] = delta2;
     qu[0] = q;
     qu[1] = u;
   }
 }
 
-// xp[0] = x (the solution buffer of a device-driven PCG)
+// Start of a plain PCG: counters cleared, the iteration limit in counters[5] and tol^2 in the
+// control block's double area (read by k_fin_rr), x in the pointer table.
 template <typename T>
-__global__ void k_set_ptr(T** __restrict__ xp, T* x) {
-  if (threadIdx.x == 0) xp[0] = x;
+__global__ void k_pcg_init(int* __restrict__ counters, int* __restrict__ ctl,
+                           T** __restrict__ xp, T* x, double tol2, int maxit) {
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 8; ++i) counters[i] = 0;
+    counters[5] = maxit;
+    ctl_dbl(ctl)[SDC_DCTL_TOL2] = tol2;
+    xp[0] = x;
+  }
 }
 
 // p = z + beta p (FIRST: p = z)
@@ -882,7 +948,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_p4(const T* __restrict__ z
                                                          const double* __restrict__ beta,
                                                          size_t ld, uint32_t nN,
                                                          const int* __restrict__ done) {
-  if (done && *done) return;
+  if (done && ld_flag(done)) return;
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
   const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
@@ -929,7 +995,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_norm(const double* __restrict
                                                         uint32_t nbx, double* __restrict__ out,
                                                         int* __restrict__ active, int init,
                                                         const int* __restrict__ done) {
-  if (done && *done) return;
+  if (done && ld_flag(done)) return;
   const int c = blockIdx.x;
   const double v = block_sum(partial + (size_t)c * nbx, nbx);
   if (threadIdx.x == 0) {
@@ -945,23 +1011,26 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_alpha(const double* __restric
                                                          double* __restrict__ alpha,
                                                          const int* __restrict__ active,
                                                          const int* __restrict__ done) {
-  if (*done) return;
+  if (done && ld_flag(done)) return;
   const int c = blockIdx.x;
   const double pq = block_sum(partial + (size_t)c * nbx, nbx);
   if (threadIdx.x == 0) alpha[c] = (active[c] && pq > 0.0) ? rz[c] / pq : 0.0;
 }
 
-// convergence test |r| <= tol |b| per column; the last work-group counts the iteration
-// (counters[4]) and publishes the number of still-active columns and the stop flag: all
-// converged, or maxit iterations done (device copy and host-mapped copy)
+// convergence test |r| <= tol |b| per column (tol^2 from the control block ctl); the last
+// work-group counts the iteration (counters[4]) and publishes the number of still-active columns
+// and the stop flag: all converged, or counters[5] iterations done (device copy and host-mapped
+// copy)
 __global__ __launch_bounds__(SDC_BLOCK) void k_fin_rr(const double* __restrict__ partial,
                                                       uint32_t nbx,
                                                       const double* __restrict__ bnorm2,
-                                                      double tol2, double* __restrict__ rr,
+                                                      const int* __restrict__ ctl,
+                                                      double* __restrict__ rr,
                                                       int* __restrict__ active,
                                                       int* __restrict__ counters, int* done,
-                                                      int* host_flag, int maxit) {
-  if (*done) return;
+                                                      int* host_flag) {
+  if (ld_flag(done)) return;
+  const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
   const int c = blockIdx.x;
   const double v = block_sum(partial + (size_t)c * nbx, nbx);
   if (threadIdx.x == 0) {
@@ -974,9 +1043,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_rr(const double* __restrict__
     const int ticket = atomicAdd(&counters[1], 1);
     if (ticket == (int)gridDim.x - 1) {
       const int nact = atomicAdd(&counters[0], 0);
-      const int it = counters[4] + 1;
+      const int it = ld_flag(counters + 4) + 1;
       counters[4] = it;
-      const int fin = (nact == 0 || it >= maxit) ? 1 : 0;
+      const int fin = (nact == 0 || it >= ld_flag(counters + 5)) ? 1 : 0;
       *done = fin;
       counters[2] = nact;
       counters[0] = 0;
@@ -996,7 +1065,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_beta(const double* __restrict
                                                         double* __restrict__ beta,
                                                         const int* __restrict__ active, int first,
                                                         const int* __restrict__ done) {
-  if (*done) return;
+  if (done && ld_flag(done)) return;
   const int c = blockIdx.x;
   const double v = block_sum(partial + (size_t)c * nbx, nbx);
   if (threadIdx.x == 0) {
@@ -1047,7 +1116,7 @@ struct PinFix {
 // from qu[0], qu[1] (device pointer table) instead of the arguments, so that a recorded
 // iteration serves every solve.
 template <typename TO, int R, bool U_ZERO>
-__global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const double* q,
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> sa, const double* q,
                                                            const double* u,
                                                            TO* __restrict__ rout, size_t ld,
                                                            int ncol, PinFix pf,
@@ -1061,7 +1130,14 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> s, const
     q = qu[0];
     u = qu[1];
   }
-  SDC_NODE_PROLOGUE_R(s.nN, s.remap)
+  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
+  const Sten<double> s = sten_col(sa, c0);
+  if (sa.cset) {
+    const size_t o = (size_t)sa.cset[c0] * sa.sstride;
+    pf.cx += o;
+    pf.cy += o;
+    pf.cz += o;
+  }
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -1115,6 +1191,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_true(const double* __restrict
                                                         int* __restrict__ active,
                                                         const int* __restrict__ ctl) {
   if (ctl && !ctl_update(ctl)) return;
+  if (ctl) tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
   const int c = blockIdx.x;
   const double v = block_sum(partial + (size_t)c * nbx, nbx);
   if (threadIdx.x == 0) {
@@ -1160,7 +1237,8 @@ struct Geo {
 };
 
 // Edge conductances of the +x/+y/+z edges of every node and the Robin diagonal, from a cell
-// property sig (sigma, or a perturbation of it; A is linear in sigma):
+// property sig (sigma, or a perturbation of it; A is linear in sigma).  blockIdx.z selects an
+// operator set: sig + z * sig_stride in, arrays + z * ostride out.
 //   x edge (i,j,k)-(i+1,j,k): sum over the <=4 cells (i, j', k') touching it of
 //   sig * hy_j' * hz_k' / (4 hx_i), i.e. M_e = diag(A_v sig) with A_v[e,c] = V_c/4 divided by hx_i^2
 //   Robin: b_n = sum over boundary faces f containing n of beta_f * sig(cell of f)
@@ -1169,9 +1247,17 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coef_from_sigma(Geo g, const doub
                                                                double* __restrict__ cy,
                                                                double* __restrict__ cz,
                                                                double* __restrict__ bd,
-                                                               double scale) {
+                                                               double scale, size_t sig_stride,
+                                                               size_t ostride) {
   const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
   if (n >= g.nN) return;
+  // operator set blockIdx.z: its cell property and its coefficient arrays
+  const size_t zi = (size_t)blockIdx.z * sig_stride, zo = (size_t)blockIdx.z * ostride;
+  sig += zi;
+  cx += zo;
+  cy += zo;
+  cz += zo;
+  bd += zo;
   const uint32_t r = fdiv(n, g.fx);
   const int i = (int)(n - r * g.n1x);
   const int k = (int)fdiv(r, g.fy);
@@ -1262,9 +1348,17 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coef_diag(uint32_t n1x, uint32_t 
                                                          const double* __restrict__ cz,
                                                          const double* __restrict__ bd,
                                                          double* __restrict__ dg,
-                                                         double* __restrict__ idg, int pin) {
+                                                         double* __restrict__ idg, int pin,
+                                                         size_t stride) {
   const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
   if (n >= nN) return;
+  const size_t zo = (size_t)blockIdx.z * stride;  // operator set
+  cx += zo;
+  cy += zo;
+  cz += zo;
+  bd += zo;
+  dg += zo;
+  idg += zo;
   const uint32_t r = fdiv(n, fx);
   const uint32_t i = n - r * n1x;
   const uint32_t k = fdiv(r, fy);
@@ -1292,9 +1386,20 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coarsen_coef(Xfer<double> t,
                                                             double* __restrict__ ccy,
                                                             double* __restrict__ ccz,
                                                             double* __restrict__ cbd,
-                                                            uint32_t cn1z) {
+                                                            uint32_t cn1z, size_t fstride,
+                                                            size_t cstride) {
   const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
   if (n >= t.cN) return;
+  // operator set blockIdx.z: fine arrays + z * fstride, coarse arrays + z * cstride
+  const size_t zf = (size_t)blockIdx.z * fstride, zc = (size_t)blockIdx.z * cstride;
+  fcx += zf;
+  fcy += zf;
+  fcz += zf;
+  fbd += zf;
+  ccx += zc;
+  ccy += zc;
+  ccz += zc;
+  cbd += zc;
   const uint32_t rr = fdiv(n, t.cfx);
   const uint32_t I = n - rr * t.cn1x;
   const uint32_t K = fdiv(rr, t.cfy);
@@ -1368,16 +1473,27 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coarsen_coef(Xfer<double> t,
 }
 
 template <typename TO>
+// b = a in type TO, len entries of operator set blockIdx.z (arrays + z * stride)
 __global__ __launch_bounds__(SDC_BLOCK) void k_cast(const double* __restrict__ a, TO* __restrict__ b,
-                                                    size_t len) {
+                                                    size_t len, size_t stride) {
   const size_t i = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
-  if (i < len) b[i] = (TO)a[i];
+  const size_t z = (size_t)blockIdx.z * stride;
+  if (i < len) b[z + i] = (TO)a[z + i];
 }
 
-// dense operator of the coarsest level (row-major nn x nn), pin = 1: identity row/column 0
-__global__ __launch_bounds__(SDC_BLOCK) void k_dense_build(Sten<double> s, double* __restrict__ a) {
+// dense operator of the coarsest level (row-major nn x nn), pin = 1: identity row/column 0;
+// operator set blockIdx.z: coefficients + z * s.sstride, matrix + z * astride
+__global__ __launch_bounds__(SDC_BLOCK) void k_dense_build(Sten<double> s, double* __restrict__ a,
+                                                           size_t astride) {
   const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
   if (n >= s.nN) return;
+  const size_t zo = (size_t)blockIdx.z * s.sstride;
+  s.cx += zo;
+  s.cy += zo;
+  s.cz += zo;
+  s.dg += zo;
+  s.idg += zo;
+  a += (size_t)blockIdx.z * astride;
   const size_t N = s.nN;
   double* row = a + (size_t)n * N;
   for (size_t m = 0; m < N; ++m) row[m] = 0.0;
@@ -1393,10 +1509,13 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_dense_build(Sten<double> s, doubl
 
 // ainv64 <- (ainv64 + ainv64^T)/2 and its fp32 copy
 __global__ __launch_bounds__(SDC_BLOCK) void k_symmetrize(double* __restrict__ a,
-                                                          float* __restrict__ a32, int nn) {
+                                                          float* __restrict__ a32, int nn,
+                                                          size_t astride) {
   const size_t e = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
   const size_t N = nn;
   if (e >= N * N) return;
+  a += (size_t)blockIdx.y * astride;  // operator set blockIdx.y
+  a32 += (size_t)blockIdx.y * astride;
   const size_t r = e / N, c = e % N;
   if (c < r) return;
   const double v = 0.5 * (a[r * N + c] + a[c * N + r]);
@@ -1406,11 +1525,16 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_symmetrize(double* __restrict__ a
   a32[c * N + r] = (float)v;
 }
 
-// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one work-group of 1024.
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one work-group of 1024 per
+// operator set (blockIdx.x; matrices + set * astride, scratch + set * nn).
 // a: nn x nn (destroyed), inv: nn x nn output, f: nn scratch for the elimination factors.
 // Each pivot step updates all nn*nn entries of both matrices in parallel.
 __global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ a, double* __restrict__ inv,
-                                                       double* __restrict__ f, int nn) {
+                                                       double* __restrict__ f, int nn,
+                                                       size_t astride) {
+  a += (size_t)blockIdx.x * astride;
+  inv += (size_t)blockIdx.x * astride;
+  f += (size_t)blockIdx.x * nn;
   const int tid = threadIdx.x;
   const int N = nn;
   const int NN = N * N;

@@ -98,8 +98,10 @@ struct XferDev {
   T *rlx = nullptr, *rly = nullptr, *rlz = nullptr, *rhx = nullptr, *rhy = nullptr, *rhz = nullptr;
   int *pix = nullptr, *piy = nullptr, *piz = nullptr;
   T *pwx = nullptr, *pwy = nullptr, *pwz = nullptr;
-  // operator-dependent interpolation weights (Xfer::wx), recomputed with the conductances
+  // operator-dependent interpolation weights (Xfer::wx), recomputed with the conductances;
+  // one array per operator set, wxs / wys / wzs entries apart
   T *wx = nullptr, *wy = nullptr, *wz = nullptr;
+  size_t wxs = 0, wys = 0, wzs = 0;
   void release() {
     dfree(rix); dfree(riy); dfree(riz);
     dfree(rlx); dfree(rly); dfree(rlz); dfree(rhx); dfree(rhy); dfree(rhz);
@@ -113,6 +115,8 @@ struct Level {
   Axis ax[3];
   uint32_t n1x = 0, n1y = 0, n1z = 0, nN = 0, plane = 0;
   size_t ld = 0;
+  // coefficient arrays hold one operator per set, sstride entries apart (round_ld(nN))
+  size_t sstride = 0;
   double *cx64 = nullptr, *cy64 = nullptr, *cz64 = nullptr, *bd64 = nullptr, *dg64 = nullptr,
          *idg64 = nullptr;
   float *cx32 = nullptr, *cy32 = nullptr, *cz32 = nullptr, *dg32 = nullptr, *idg32 = nullptr;
@@ -127,6 +131,9 @@ struct Level {
     x32.release();
   }
 };
+
+// doubles taken by the int part of a control block
+constexpr int kCtlInts = SDC_CTL_N * (int)sizeof(int) / (int)sizeof(double);
 
 template <typename T>
 struct Work {
@@ -195,6 +202,15 @@ struct sdc_handle_s {
   double* ainv64 = nullptr;
   float* ainv32 = nullptr;
   double *gj_a = nullptr, *gj_f = nullptr;
+  size_t astride = 0;  // entries between the operator sets' dense inverses
+  // operator sets: capacity of the per-set arrays, sets built by the last sdc_set_sigma_sets
+  // (1 after sdc_set_sigma / sdc_set_coefficients), and during sdc_solve_sets the device map
+  // column -> set that every coefficient-reading kernel follows (null otherwise)
+  int nset_cap = 1;
+  int nset = 1;
+  int32_t* cset_dev = nullptr;
+  int cset_cap = 0;
+  const int32_t* cur_cset = nullptr;
   // scratch coefficient arrays for A(dsigma) (Jvec right-hand side)
   double *tcx = nullptr, *tcy = nullptr, *tcz = nullptr, *tbd = nullptr, *tdg = nullptr,
          *tidg = nullptr;
@@ -327,24 +343,28 @@ Sten<T> make_sten(const sdc_handle_s* h, int l, int pin_mode);
 template <>
 Sten<double> make_sten<double>(const sdc_handle_s* h, int l, int pin_mode) {
   const Level& v = h->L[l];
-  Sten<double> s;
+  Sten<double> s{};
   s.cx = v.cx64; s.cy = v.cy64; s.cz = v.cz64; s.dg = v.dg64; s.idg = v.idg64;
   s.n1x = v.n1x; s.n1y = v.n1y; s.n1z = v.n1z; s.nN = v.nN; s.plane = v.plane;
   s.fx = make_fastdiv(v.n1x); s.fy = make_fastdiv(v.n1y);
   s.pin0 = (h->bc == SDC_BC_NEUMANN) ? pin_mode : 0;
   s.remap = h->prm.xcd_remap != 0;
+  s.cset = h->cur_cset;
+  s.sstride = (uint32_t)v.sstride;
   return s;
 }
 
 template <>
 Sten<float> make_sten<float>(const sdc_handle_s* h, int l, int pin_mode) {
   const Level& v = h->L[l];
-  Sten<float> s;
+  Sten<float> s{};
   s.cx = v.cx32; s.cy = v.cy32; s.cz = v.cz32; s.dg = v.dg32; s.idg = v.idg32;
   s.n1x = v.n1x; s.n1y = v.n1y; s.n1z = v.n1z; s.nN = v.nN; s.plane = v.plane;
   s.fx = make_fastdiv(v.n1x); s.fy = make_fastdiv(v.n1y);
   s.pin0 = (h->bc == SDC_BC_NEUMANN) ? pin_mode : 0;
   s.remap = h->prm.xcd_remap != 0;
+  s.cset = h->cur_cset;
+  s.sstride = (uint32_t)v.sstride;
   return s;
 }
 
@@ -360,7 +380,7 @@ Xfer<T> make_xfer(const sdc_handle_s* h, int l) {
   const Level& f = h->L[l];
   const Level& c = h->L[l + 1];
   const XferDev<T>& d = xfer_dev<T>(f);
-  Xfer<T> t;
+  Xfer<T> t{};
   t.rix = d.rix; t.riy = d.riy; t.riz = d.riz;
   t.rlx = d.rlx; t.rly = d.rly; t.rlz = d.rlz;
   t.rhx = d.rhx; t.rhy = d.rhy; t.rhz = d.rhz;
@@ -374,6 +394,10 @@ Xfer<T> make_xfer(const sdc_handle_s* h, int l) {
   t.wx = h->prm.interp ? d.wx : nullptr;
   t.wy = h->prm.interp ? d.wy : nullptr;
   t.wz = h->prm.interp ? d.wz : nullptr;
+  t.cset = h->cur_cset;
+  t.wxs = (uint32_t)d.wxs;
+  t.wys = (uint32_t)d.wys;
+  t.wzs = (uint32_t)d.wzs;
   return t;
 }
 
@@ -422,12 +446,31 @@ void fill_level_dims(Level& v) {
   v.ld = round_ld(v.nN);
 }
 
-int alloc_level_arrays(Level& v) {
-  const size_t n = v.nN;
+void free_level_arrays(Level& v) {
+  dfree(v.cx64); dfree(v.cy64); dfree(v.cz64); dfree(v.bd64); dfree(v.dg64); dfree(v.idg64);
+  dfree(v.cx32); dfree(v.cy32); dfree(v.cz32); dfree(v.dg32); dfree(v.idg32);
+}
+
+int alloc_level_arrays(Level& v, int nset) {
+  free_level_arrays(v);
+  v.sstride = round_ld(v.nN);
+  const size_t n = v.sstride * (size_t)nset;
   CK(dmalloc(&v.cx64, n)); CK(dmalloc(&v.cy64, n)); CK(dmalloc(&v.cz64, n));
   CK(dmalloc(&v.bd64, n)); CK(dmalloc(&v.dg64, n)); CK(dmalloc(&v.idg64, n));
   CK(dmalloc(&v.cx32, n)); CK(dmalloc(&v.cy32, n)); CK(dmalloc(&v.cz32, n));
   CK(dmalloc(&v.dg32, n)); CK(dmalloc(&v.idg32, n));
+  return SDC_OK;
+}
+
+// operator-dependent interpolation weights of the transfer below level f, for nset sets
+int alloc_weights(Level& f, int nset) {
+  const size_t k = (size_t)nset;
+  dfree(f.x64.wx); dfree(f.x64.wy); dfree(f.x64.wz);
+  dfree(f.x32.wx); dfree(f.x32.wy); dfree(f.x32.wz);
+  CK(dmalloc(&f.x64.wx, f.x64.wxs * k)); CK(dmalloc(&f.x64.wy, f.x64.wys * k));
+  CK(dmalloc(&f.x64.wz, f.x64.wzs * k));
+  CK(dmalloc(&f.x32.wx, f.x32.wxs * k)); CK(dmalloc(&f.x32.wy, f.x32.wys * k));
+  CK(dmalloc(&f.x32.wz, f.x32.wzs * k));
   return SDC_OK;
 }
 
@@ -469,8 +512,10 @@ int make_transfer(Level& f, Level& c, std::vector<int> (&fidx)[3]) {
   // Pz over (coarse x, coarse y, fine z)
   const size_t nwx = f.nN, nwy = (size_t)c.n1x * f.n1y * f.n1z,
                nwz = (size_t)c.n1x * c.n1y * f.n1z;
-  CK(dmalloc(&f.x64.wx, nwx)); CK(dmalloc(&f.x64.wy, nwy)); CK(dmalloc(&f.x64.wz, nwz));
-  CK(dmalloc(&f.x32.wx, nwx)); CK(dmalloc(&f.x32.wy, nwy)); CK(dmalloc(&f.x32.wz, nwz));
+  f.x64.wxs = f.x32.wxs = round_ld(nwx);
+  f.x64.wys = f.x32.wys = round_ld(nwy);
+  f.x64.wzs = f.x32.wzs = round_ld(nwz);
+  CK(alloc_weights(f, 1));
   f.has_coarse = true;
   return SDC_OK;
 }
@@ -535,10 +580,10 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   CK(dmalloc(&w.ocount, 1));
   CK(dmalloc(&w.qmask, ld0));
   if (want16) CK(dmalloc(&w.r16, n0));
-  static_assert(SDC_CTL_N * sizeof(int) == 4 * sizeof(double), "control block layout");
-  CK(dmalloc(&w.ctlmem, 4 + SDC_DCTL_N));
+  static_assert(SDC_CTL_N * sizeof(int) == kCtlInts * sizeof(double), "control block layout");
+  CK(dmalloc(&w.ctlmem, kCtlInts + SDC_DCTL_N));
   w.ctl = (int*)w.ctlmem;
-  w.dctl = w.ctlmem + 4;
+  w.dctl = w.ctlmem + kCtlInts;
   CK(dmalloc(&w.rmax, cap));
   CK(dmalloc(&w.qu, 2));
   CK(dmalloc(&w.xtab, 1));
@@ -650,7 +695,7 @@ template <typename S>
 int tail_vcycle(sdc_handle_s* h, const S* b, S* x, int ncol, const int* done, hipStream_t st) {
   const int nl = (int)h->L.size();
   const int lt = h->tail_level;
-  TailDesc d;
+  TailDesc d{};
   d.nlev = nl - lt;
   uint32_t off = 0, ymax = 0;
   for (int k = 0; k < d.nlev; ++k) {
@@ -666,6 +711,8 @@ int tail_vcycle(sdc_handle_s* h, const S* b, S* x, int ncol, const int* done, hi
   d.offY = off;
   d.ainv = h->ainv32;
   d.nn = (int)h->L[nl - 1].nN;
+  d.cset = h->cur_cset;
+  d.astride = (uint32_t)h->astride;
   d.nu_pre = h->prm.nu_coarse > 0 ? h->prm.nu_coarse : h->prm.nu_pre;
   d.nu_post = h->prm.nu_coarse > 0 ? h->prm.nu_coarse : h->prm.nu_post;
   const Cheb cc = cheb_coefs(h->prm);
@@ -700,7 +747,7 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
     const T* ainv = (const T*)(std::is_same<T, float>::value ? (const void*)h->ainv32
                                                               : (const void*)h->ainv64);
     launch(h, (k_dense_solve<T, B, O>), dim3(nblk(lv.nN), ncol), dim3(SDC_BLOCK), 0, st,
-                       ainv, (int)lv.nN, b, x, lv.ld, ncol, done);
+                       ainv, (int)lv.nN, b, x, lv.ld, ncol, done, h->cur_cset, h->astride);
     if constexpr (std::is_same<B, T>::value && std::is_same<O, T>::value) {
       if (dot_out) {
         DISPATCH_R(R, launch(h, (k_dot<T, R>), grid2(lv.nN, ncol, R), dim3(SDC_BLOCK), 0,
@@ -829,7 +876,7 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
         const int a = axes[p];
         const bool lastp = p == na - 1;
         S* dst = lastp ? cb : (src == res ? oth : res);
-        AxisXfer ax;
+        AxisXfer ax{};
         ax.a0 = dims[0]; ax.a1 = dims[1]; ax.a2 = dims[2];
         dims[a] = cdims[a];
         ax.b0 = dims[0]; ax.b1 = dims[1]; ax.b2 = dims[2];
@@ -837,6 +884,8 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
         ax.f0 = make_fastdiv(ax.b0);
         ax.f1 = make_fastdiv(ax.b1);
         ax.remap = h->prm.xcd_remap != 0;
+        ax.cset = h->cur_cset;
+        ax.wstride = a == 0 ? t.wxs : a == 1 ? t.wys : t.wzs;
         const size_t ldo = lastp ? cl.ld : lv.ld;
         const int* fi = a == 0 ? t.rix : a == 1 ? t.riy : t.riz;
         const T* wl = a == 0 ? t.rlx : a == 1 ? t.rly : t.rlz;
@@ -907,6 +956,8 @@ int graph_run(sdc_handle_s* h, const void* b, void* x, int ncol, int variant, hi
                                    (1u << SDC_PROF_PAIR) | (1u << SDC_PROF_RESIDUAL) |
                                    (1u << SDC_PROF_RESTRICT) | (1u << SDC_PROF_PROLONG);
   if (!h->prm.graph || (h->prof_mask & vcycle_tags)) return fn(st);
+  // kernels recorded in operator-set mode carry the column -> set map
+  if (h->cur_cset) variant |= 1 << 24;
   for (const auto& g : h->graphs)
     if (g.b == b && g.x == x && g.ncol == ncol && g.variant == variant) {
       HIPCK(hipGraphLaunch(g.exec, st));
@@ -949,9 +1000,14 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   const uint32_t nbx = nblk(l0.nN);
   const size_t len = l0.ld * (size_t)ncol;
   int* done = W.counters + 3;
+  const int K = std::max(1, (int)h->prm.check_every);
+  // k_fin_rr sets the stop flag; the other kernels check it (a device-scope load at their start)
+  // only when several iterations run between two host reads, since with one the kernels after
+  // the test change nothing that the solve returns (x is final once r passed the test)
+  const int* vdone = K > 1 ? done : nullptr;
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(len)), blk, 0, st, x, len);
-  hipLaunchKernelGGL(k_zero<int>, dim3(1), blk, 0, st, W.counters, (size_t)8);
-  hipLaunchKernelGGL(k_set_ptr<T>, dim3(1), dim3(64), 0, st, W.xtab, x);
+  hipLaunchKernelGGL(k_pcg_init<T>, dim3(1), dim3(64), 0, st, W.counters, W.ctl, W.xtab, x,
+                     tol * tol, maxit);
   DISPATCH_R(R, hipLaunchKernelGGL((k_dot<T, R>), g, blk, 0, st, (const T*)W.r, (const T*)W.r,
                                    l0.ld, l0.nN, ncol, W.partial, (const int*)nullptr));
   hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.bn2,
@@ -959,41 +1015,40 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   if (col_mask)
     hipLaunchKernelGGL(k_and_active, dim3(nblk(ncol)), blk, 0, st, W.active, col_mask, ncol);
   CK(graph_run(h, W.r, W.z, ncol, 8 + (int)sizeof(T), st, [&](hipStream_t gs) {
-    return vcycle0<T>(h, W.r, W.z, ncol, true, done, gs);
+    return vcycle0<T>(h, W.r, W.z, ncol, true, nullptr, gs);
   }));
   hipLaunchKernelGGL(k_fin_beta, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rz,
-                     W.beta, (const int*)W.active, 1, (const int*)done);
+                     W.beta, (const int*)W.active, 1, (const int*)nullptr);
   hipLaunchKernelGGL((k_update_p4<T, true>), grid4(l0.nN, ncol), blk, 0, st, (const T*)W.z, W.p,
-                     (const double*)W.beta, l0.ld, l0.nN, (const int*)done);
+                     (const double*)W.beta, l0.ld, l0.nN, (const int*)nullptr);
   const Sten<T> s = make_sten<T>(h, 0, 1);
-  const double tol2 = tol * tol;
   // one iteration: q = A p, alpha, x += alpha p and r -= alpha q, the convergence test, z = M r,
   // beta, p = z + beta p -- every launch through launch() so that it can be recorded
   auto iteration = [&](hipStream_t s2) -> int {
     prof_begin(h, SDC_PROF_APPLY, s2);
     DISPATCH_R(R, launch(h, (k_apply<T, R, true>), g, blk, 0, s2, s, (const T*)W.p, W.q, l0.ld,
-                         ncol, W.partial, (const int*)done));
+                         ncol, W.partial, vdone));
     prof_end(h, SDC_PROF_APPLY, s2,
              (double)l0.nN * 4 * sizeof(T) + (double)l0.nN * ncol * 2 * sizeof(T));
     launch(h, k_fin_alpha, dim3(ncol), blk, 0, s2, (const double*)W.partial, nbx,
-           (const double*)W.rz, W.alpha, (const int*)W.active, (const int*)done);
+           (const double*)W.rz, W.alpha, (const int*)W.active, vdone);
     prof_begin(h, SDC_PROF_UPDATE, s2);
     launch(h, k_update_xr4<T>, grid4(l0.nN, ncol), blk, 0, s2, (T* const*)W.xtab, (const T*)W.p,
            W.r, (const T*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
-           (const int*)done);
+           vdone);
     prof_end(h, SDC_PROF_UPDATE, s2, (double)l0.nN * ncol * 6 * sizeof(T));
     launch(h, k_fin_rr, dim3(ncol), blk, 0, s2, (const double*)W.partial, nblk4(l0.nN),
-           (const double*)W.bn2, tol2, W.rr, W.active, W.counters, done, (int*)nullptr, maxit);
-    CK(vcycle0<T>(h, W.r, W.z, ncol, true, done, s2));
+           (const double*)W.bn2, (const int*)W.ctl, W.rr, W.active, W.counters, done,
+           (int*)nullptr);
+    CK(vcycle0<T>(h, W.r, W.z, ncol, true, vdone, s2));
     launch(h, k_fin_beta, dim3(ncol), blk, 0, s2, (const double*)W.partial, nbx, W.rz, W.beta,
-           (const int*)W.active, 0, (const int*)done);
+           (const int*)W.active, 0, vdone);
     prof_begin(h, SDC_PROF_UPDATE_P, s2);
     launch(h, k_update_p4<T, false>, grid4(l0.nN, ncol), blk, 0, s2, (const T*)W.z, W.p,
-           (const double*)W.beta, l0.ld, l0.nN, (const int*)done);
+           (const double*)W.beta, l0.ld, l0.nN, vdone);
     prof_end(h, SDC_PROF_UPDATE_P, s2, (double)l0.nN * ncol * 3 * sizeof(T));
     return SDC_OK;
   };
-  const int K = std::max(1, (int)h->prm.check_every);
   const bool rec = h->prm.graph && h->prof_mask == 0;
   const int variant = 2000 + 16 * K + (int)sizeof(T);
   for (int it = 0;; it += K) {
@@ -1072,17 +1127,23 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
     return SDC_OK;
   }
-  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, W.ctl, W.dctl, W.qu, q, u);
-  // vcycle_bf16 = 1, 4: the preconditioned residual z is bfloat16
-  const bool zb = (h->prm.vcycle_bf16 == 1 || h->prm.vcycle_bf16 == 4) && h->L.size() > 1;
-  __bf16* const z16 = (__bf16*)W.xb0;
-  const int* stop = W.ctl + SDC_CTL_STOP;
-  const double bz = zb ? 2.0 : 4.0;
   // Stall guard: a reliable update is forced after kStallWindow iterations without one, and the
   // solve gives up (converged = 0, the caller continues in fp64) when an update fails to halve
   // the largest true relative residual of the previous one.  This happens when the fp32
   // recursion no longer tracks the true residual (conductivity contrasts of 1e4 and more).
   constexpr int kStallWindow = 25;
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, W.ctl, W.qu, q, u, tol2, delta2, maxit,
+                     kStallWindow);
+  // vcycle_bf16 = 1, 4: the preconditioned residual z is bfloat16
+  const bool zb = (h->prm.vcycle_bf16 == 1 || h->prm.vcycle_bf16 == 4) && h->L.size() > 1;
+  __bf16* const z16 = (__bf16*)W.xb0;
+  const int K = std::max(1, (int)h->prm.check_every);
+  // Early exits after the stop flag only matter when several iterations run between two host
+  // reads of it (check_every > 1); the flag is read with a device-scope load at the start of
+  // every kernel that checks it, which costs latency, so with one iteration per read the
+  // kernels that cannot change u skip the check (the update kernels always read the block).
+  const int* stop = K > 1 ? W.ctl + SDC_CTL_STOP : nullptr;
+  const double bz = zb ? 2.0 : 4.0;
   // fp64 stencil of the true residual: refine_rhs_per_thread columns per work-item (default 8;
   // its coefficients are fp64, so more columns per work-item save more traffic than in the fp32
   // kernels)
@@ -1114,11 +1175,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     if (first) {                                                                               \
       DISPATCH_R(R, launch(h, (k_apply_pq<ZT, R, true>), g, blk, 0, s, s1, ZP, W.p, W.q, W.x,  \
                            (const double*)W.beta, (const double*)W.alpha, l0.ld, ncol,         \
-                           W.partial, (const int*)W.ctl));                                     \
+                           W.partial, (const int*)W.ctl, K > 1 ? 1 : 0));                      \
     } else {                                                                                   \
       DISPATCH_R(R, launch(h, (k_apply_pq<ZT, R, false>), g, blk, 0, s, s1, ZP, W.p, W.q, W.x, \
                            (const double*)W.beta, (const double*)W.alpha, l0.ld, ncol,         \
-                           W.partial, (const int*)W.ctl));                                     \
+                           W.partial, (const int*)W.ctl, K > 1 ? 1 : 0));                      \
     }
     if (zb) {
       SDC_APPLY_PQ(__bf16, (const __bf16*)z16)
@@ -1141,8 +1202,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     launch(h, k_fin_norm, dim3(ncol), blk, 0, s, (const double*)W.partial, nblk4(l0.nN), W.rr,
            W.active, 0, stop);
     launch(h, k_decide, dim3(1), blk, 0, s, W.ctl, (const double*)W.rr, (const double*)W.aux,
-           (const double*)W.rmax, (const int*)W.active, ncol, tol2, delta2, maxit,
-           kStallWindow);
+           (const double*)W.rmax, (const int*)W.active, ncol);
     // reliable update (every kernel below returns at once unless k_decide asked for it).  A
     // profiled run brackets the fold and the fp64 residual only when they do work, so it reads
     // the decision on the host (graphs are off while profiling).
@@ -1169,14 +1229,13 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     prof_end(h, upd ? SDC_PROF_REFINE : -1, s, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
     launch(h, k_fin_true, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx,
            (const double*)W.aux, tol2, W.orr, W.rmax, W.active, (const int*)W.ctl);
-    launch(h, k_after_update, dim3(1), blk, 0, s, W.ctl, W.dctl, (const double*)W.orr,
-           (const double*)W.aux, (const int*)W.active, W.alpha, ncol, maxit);
+    launch(h, k_after_update, dim3(1), blk, 0, s, W.ctl, (const double*)W.orr,
+           (const double*)W.aux, (const int*)W.active, W.alpha, ncol);
     return SDC_OK;
   };
   // the first iteration (FIRST variants) is launched; later ones run check_every at a time,
   // then the host reads the control block
   CK(iteration(true, st));
-  const int K = std::max(1, (int)h->prm.check_every);
   // profiled kernel classes are bracketed by events, which cannot sit inside a graph
   const bool rec = h->prm.graph && h->prof_mask == 0;
   const int variant = 1000 + 16 * K + (zb ? 1 : 0) + (r16 ? 2 : 0) + 4 * (R64 == 8);
@@ -1190,7 +1249,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       CK(graph_run(h, W.ctl, (void*)W.partial, ncol, variant, st, block));
     else
       CK(block(st));
-    HIPCK(hipMemcpyAsync(h->h_ctl, W.ctlmem, (4 + SDC_DCTL_N) * sizeof(double),
+    HIPCK(hipMemcpyAsync(h->h_ctl, W.ctlmem, (kCtlInts + SDC_DCTL_N) * sizeof(double),
                          hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     if (h_ctl[SDC_CTL_STOP]) break;
@@ -1199,7 +1258,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   }
   S.iters = std::min(h_ctl[SDC_CTL_IT], maxit);
   S.outer = h_ctl[SDC_CTL_OUTER];
-  S.max_rel_res = h->h_ctl[4 + SDC_DCTL_MREL];
+  S.max_rel_res = h->h_ctl[kCtlInts + SDC_DCTL_MREL];
   S.converged = h_ctl[SDC_CTL_REASON] == 1;
   HIPCK(hipGetLastError());
   return SDC_OK;
@@ -1208,7 +1267,35 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
 }  // namespace
 
 namespace {
-int build_levels(sdc_handle_s* h, hipStream_t st);
+int build_levels(sdc_handle_s* h, hipStream_t st, int nset);
+
+// dense inverses (and the Gauss-Jordan scratch) of the coarsest level for nset operator sets
+int alloc_coarse_inverse(sdc_handle_s* h, int nset) {
+  dfree(h->ainv64); dfree(h->ainv32); dfree(h->gj_a); dfree(h->gj_f);
+  const size_t k = (size_t)nset, nc = (size_t)h->ncoarse;
+  CK(dmalloc(&h->ainv64, h->astride * k));
+  CK(dmalloc(&h->ainv32, h->astride * k));
+  CK(dmalloc(&h->gj_a, h->astride * k));
+  CK(dmalloc(&h->gj_f, nc * k));
+  return SDC_OK;
+}
+
+// Room for nset operator sets: the per-set arrays (coefficients of every level, interpolation
+// weights, coarsest inverses) are reallocated when the capacity grows; their contents are then
+// undefined until the next sdc_set_sigma_sets.
+int ensure_sets(sdc_handle_s* h, int nset) {
+  if (nset <= h->nset_cap) return SDC_OK;
+  HIPCK(hipDeviceSynchronize());
+  drop_graphs(h);
+  for (size_t l = 0; l < h->L.size(); ++l) {
+    CK(alloc_level_arrays(h->L[l], nset));
+    if (l + 1 < h->L.size()) CK(alloc_weights(h->L[l], nset));
+  }
+  CK(alloc_coarse_inverse(h, nset));
+  h->nset_cap = nset;
+  h->sigma_set = false;
+  return SDC_OK;
+}
 
 // adjoint contraction: columns per work-item group, and the partial-sum buffer for ng groups
 constexpr int kJtvecGroupCols = SDC_JT_COLS;
@@ -1363,12 +1450,10 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
   h->wlevel = pick_wlevel(h);
   pick_tail(h);
   for (auto& v : h->L)
-    if ((rc = alloc_level_arrays(v))) return bail(rc);
+    if ((rc = alloc_level_arrays(v, 1))) return bail(rc);
   h->ncoarse = (int)h->L.back().nN;
-  const size_t nc = h->ncoarse;
-  if ((rc = dmalloc(&h->ainv64, nc * nc)) || (rc = dmalloc(&h->ainv32, nc * nc)) ||
-      (rc = dmalloc(&h->gj_a, nc * nc)) || (rc = dmalloc(&h->gj_f, nc)))
-    return bail(rc);
+  h->astride = round_ld((size_t)h->ncoarse * h->ncoarse);
+  if ((rc = alloc_coarse_inverse(h, 1))) return bail(rc);
   const size_t n0 = h->L[0].nN;
   if ((rc = dmalloc(&h->tcx, n0)) || (rc = dmalloc(&h->tcy, n0)) || (rc = dmalloc(&h->tcz, n0)) ||
       (rc = dmalloc(&h->tbd, n0)) || (rc = dmalloc(&h->tdg, n0)) || (rc = dmalloc(&h->tidg, n0)))
@@ -1377,7 +1462,7 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
     h->h_flag = nullptr;
     return bail(fail(SDC_ERR_ALLOC, "hipHostMalloc failed"));
   }
-  if (hipHostMalloc((void**)&h->h_ctl, (4 + SDC_DCTL_N) * sizeof(double), hipHostMallocDefault) !=
+  if (hipHostMalloc((void**)&h->h_ctl, (kCtlInts + SDC_DCTL_N) * sizeof(double), hipHostMallocDefault) !=
       hipSuccess) {
     h->h_ctl = nullptr;
     return bail(fail(SDC_ERR_ALLOC, "hipHostMalloc failed"));
@@ -1402,6 +1487,7 @@ int sdc_destroy(sdc_handle h) {
   dfree(h->ainv64); dfree(h->ainv32); dfree(h->gj_a); dfree(h->gj_f);
   dfree(h->tcx); dfree(h->tcy); dfree(h->tcz); dfree(h->tbd); dfree(h->tdg); dfree(h->tidg);
   dfree(h->jpart);
+  dfree(h->cset_dev);
   h->w32.release();
   h->w64.release();
   if (h->h_flag) (void)hipHostFree(h->h_flag);
@@ -1460,14 +1546,21 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
 }
 
 int sdc_set_sigma(sdc_handle h, const double* sigma, void* stream) {
+  return sdc_set_sigma_sets(h, 1, sigma, stream);
+}
+
+int sdc_set_sigma_sets(sdc_handle h, int nset, const double* sigma, void* stream) {
   if (!h || !sigma) return fail(SDC_ERR_ARG, "null argument");
+  if (nset < 1 || nset > 4096) return fail(SDC_ERR_ARG, "nset must be in [1, 4096]");
   DeviceGuard dg(h->device);
+  CK(ensure_sets(h, nset));
   hipStream_t st = (hipStream_t)stream;
   const Geo g = make_geo(h);
   Level& l0 = h->L[0];
-  hipLaunchKernelGGL(k_coef_from_sigma, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, g, sigma,
-                     l0.cx64, l0.cy64, l0.cz64, l0.bd64, 1.0);
-  return build_levels(h, st);
+  const size_t nC = (size_t)h->nx * h->ny * h->nz;
+  hipLaunchKernelGGL(k_coef_from_sigma, dim3(nblk(l0.nN), 1, (uint32_t)nset), dim3(SDC_BLOCK), 0,
+                     st, g, sigma, l0.cx64, l0.cy64, l0.cz64, l0.bd64, 1.0, nC, l0.sstride);
+  return build_levels(h, st, nset);
 }
 
 int sdc_set_coefficients(sdc_handle h, const double* cx, const double* cy, const double* cz,
@@ -1481,7 +1574,7 @@ int sdc_set_coefficients(sdc_handle h, const double* cx, const double* cy, const
   HIPCK(hipMemcpyAsync(l0.cy64, cy, bytes, hipMemcpyDeviceToDevice, st));
   HIPCK(hipMemcpyAsync(l0.cz64, cz, bytes, hipMemcpyDeviceToDevice, st));
   HIPCK(hipMemcpyAsync(l0.bd64, bd, bytes, hipMemcpyDeviceToDevice, st));
-  return build_levels(h, st);
+  return build_levels(h, st, 1);
 }
 
 }  // extern "C"
@@ -1490,14 +1583,17 @@ namespace {
 
 // Everything that follows the finest level's edge conductances and Robin diagonal: coarse
 // conductances, diagonals, fp32 copies and the coarsest dense inverse.
-int build_levels(sdc_handle_s* h, hipStream_t st) {
+int build_levels(sdc_handle_s* h, hipStream_t st, int nset) {
+  // every launch covers all nset operator sets (grid z, or one work-group per set)
+  const uint32_t z = (uint32_t)nset;
   for (size_t l = 0; l + 1 < h->L.size(); ++l) {
     Level& f = h->L[l];
     Level& c = h->L[l + 1];
     const Xfer<double> t = make_xfer<double>(h, (int)l);
-    hipLaunchKernelGGL(k_coarsen_coef, dim3(nblk(c.nN)), dim3(SDC_BLOCK), 0, st, t,
+    hipLaunchKernelGGL(k_coarsen_coef, dim3(nblk(c.nN), 1, z), dim3(SDC_BLOCK), 0, st, t,
                        (const double*)f.cx64, (const double*)f.cy64, (const double*)f.cz64,
-                       (const double*)f.bd64, c.cx64, c.cy64, c.cz64, c.bd64, c.n1z);
+                       (const double*)f.bd64, c.cx64, c.cy64, c.cz64, c.bd64, c.n1z, f.sstride,
+                       c.sstride);
   }
   // operator-dependent interpolation weights of every transfer (used when sdc_params.interp = 1)
   for (size_t l = 0; l + 1 < h->L.size(); ++l) {
@@ -1506,36 +1602,42 @@ int build_levels(sdc_handle_s* h, hipStream_t st) {
     const Xfer<double> t = make_xfer<double>(h, (int)l);
     const size_t nwx = f.nN, nwy = (size_t)c.n1x * f.n1y * f.n1z,
                  nwz = (size_t)c.n1x * c.n1y * f.n1z;
-    hipLaunchKernelGGL(k_interp_weights<0>, dim3(nblk(nwx)), dim3(SDC_BLOCK), 0, st, t, f.n1x,
-                       f.n1y, f.n1z, (const double*)f.cx64, f.x64.wx, f.x32.wx);
-    hipLaunchKernelGGL(k_interp_weights<1>, dim3(nblk(nwy)), dim3(SDC_BLOCK), 0, st, t, c.n1x,
-                       f.n1y, f.n1z, (const double*)f.cy64, f.x64.wy, f.x32.wy);
-    hipLaunchKernelGGL(k_interp_weights<2>, dim3(nblk(nwz)), dim3(SDC_BLOCK), 0, st, t, c.n1x,
-                       c.n1y, f.n1z, (const double*)f.cz64, f.x64.wz, f.x32.wz);
+    hipLaunchKernelGGL(k_interp_weights<0>, dim3(nblk(nwx), 1, z), dim3(SDC_BLOCK), 0, st, t,
+                       f.n1x, f.n1y, f.n1z, (const double*)f.cx64, f.x64.wx, f.x32.wx, f.sstride,
+                       f.x64.wxs);
+    hipLaunchKernelGGL(k_interp_weights<1>, dim3(nblk(nwy), 1, z), dim3(SDC_BLOCK), 0, st, t,
+                       c.n1x, f.n1y, f.n1z, (const double*)f.cy64, f.x64.wy, f.x32.wy, f.sstride,
+                       f.x64.wys);
+    hipLaunchKernelGGL(k_interp_weights<2>, dim3(nblk(nwz), 1, z), dim3(SDC_BLOCK), 0, st, t,
+                       c.n1x, c.n1y, f.n1z, (const double*)f.cz64, f.x64.wz, f.x32.wz, f.sstride,
+                       f.x64.wzs);
   }
   const int pin = h->bc == SDC_BC_NEUMANN ? 1 : 0;
   for (auto& v : h->L) {
-    hipLaunchKernelGGL(k_coef_diag, dim3(nblk(v.nN)), dim3(SDC_BLOCK), 0, st, v.n1x, v.n1y, v.nN,
-                       make_fastdiv(v.n1x), make_fastdiv(v.n1y), (const double*)v.cx64,
+    hipLaunchKernelGGL(k_coef_diag, dim3(nblk(v.nN), 1, z), dim3(SDC_BLOCK), 0, st, v.n1x, v.n1y,
+                       v.nN, make_fastdiv(v.n1x), make_fastdiv(v.n1y), (const double*)v.cx64,
                        (const double*)v.cy64, (const double*)v.cz64, (const double*)v.bd64,
-                       v.dg64, v.idg64, pin);
-    const dim3 gg(nblk(v.nN));
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cx64, v.cx32, (size_t)v.nN);
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cy64, v.cy32, (size_t)v.nN);
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cz64, v.cz32, (size_t)v.nN);
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.dg64, v.dg32, (size_t)v.nN);
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.idg64, v.idg32, (size_t)v.nN);
+                       v.dg64, v.idg64, pin, v.sstride);
+    const dim3 gg(nblk(v.nN), 1, z);
+    const size_t n = v.nN, ss = v.sstride;
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cx64, v.cx32, n, ss);
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cy64, v.cy32, n, ss);
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cz64, v.cz32, n, ss);
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.dg64, v.dg32, n, ss);
+    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.idg64, v.idg32, n, ss);
   }
   // coarsest level: dense inverse
   const int lc = (int)h->L.size() - 1;
   const Sten<double> sc = make_sten<double>(h, lc, 1);
   const size_t nc = h->ncoarse;
-  hipLaunchKernelGGL(k_dense_build, dim3(nblk(nc)), dim3(SDC_BLOCK), 0, st, sc, h->gj_a);
-  hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), 0, st, h->gj_a, h->ainv64, h->gj_f,
-                     (int)nc);
-  hipLaunchKernelGGL(k_symmetrize, dim3(nblk(nc * nc)), dim3(SDC_BLOCK), 0, st, h->ainv64,
-                     h->ainv32, (int)nc);
+  hipLaunchKernelGGL(k_dense_build, dim3(nblk(nc), 1, z), dim3(SDC_BLOCK), 0, st, sc, h->gj_a,
+                     h->astride);
+  hipLaunchKernelGGL(k_gauss_jordan, dim3(z), dim3(1024), 0, st, h->gj_a, h->ainv64, h->gj_f,
+                     (int)nc, h->astride);
+  hipLaunchKernelGGL(k_symmetrize, dim3(nblk(nc * nc), z), dim3(SDC_BLOCK), 0, st, h->ainv64,
+                     h->ainv32, (int)nc, h->astride);
   HIPCK(hipGetLastError());
+  h->nset = nset;
   h->sigma_set = true;
   h->mixed_stalled = false;
   return SDC_OK;
@@ -1745,6 +1847,38 @@ int sdc_solve(sdc_handle h, const double* q, double* u, int ncol, const sdc_solv
   return SDC_OK;
 }
 
+int sdc_solve_sets(sdc_handle h, const double* q, double* u, int ncol, const int32_t* colset,
+                   const sdc_solve_opts* o, sdc_solve_stats* stats, void* stream) {
+  if (!h || !colset) return fail(SDC_ERR_ARG, "null argument");
+  if (ncol < 1) return fail(SDC_ERR_ARG, "ncol must be >= 1");
+  if (!h->sigma_set) return fail(SDC_ERR_STATE, "sdc_set_sigma_sets must be called first");
+  for (int c = 0; c < ncol; ++c) {
+    if (colset[c] < 0 || colset[c] >= h->nset)
+      return fail(SDC_ERR_ARG, "colset[" + std::to_string(c) + "] = " + std::to_string(colset[c]) +
+                                   " outside the " + std::to_string(h->nset) + " operator sets");
+    if (colset[c] != colset[c & ~7])
+      return fail(SDC_ERR_ARG, "every aligned group of 8 columns must use one operator set "
+                               "(column " + std::to_string(c) + ")");
+  }
+  DeviceGuard dg(h->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (h->cset_cap < ncol) {
+    // recorded iterations captured the old array's address
+    HIPCK(hipStreamSynchronize(st));
+    drop_graphs(h);
+    dfree(h->cset_dev);
+    h->cset_cap = 0;
+    CK(dmalloc(&h->cset_dev, (size_t)ncol));
+    h->cset_cap = ncol;
+  }
+  HIPCK(hipMemcpyAsync(h->cset_dev, colset, (size_t)ncol * sizeof(int32_t), hipMemcpyHostToDevice,
+                       st));
+  h->cur_cset = h->cset_dev;
+  const int rc = sdc_solve(h, q, u, ncol, o, stats, stream);
+  h->cur_cset = nullptr;
+  return rc;
+}
+
 int sdc_jvec_rhs(sdc_handle h, const double* dsig, const double* u, double* out, int ncol,
                  void* stream) {
   if (!h || !dsig || !u || !out || ncol < 1) return fail(SDC_ERR_ARG, "bad argument");
@@ -1753,12 +1887,12 @@ int sdc_jvec_rhs(sdc_handle h, const double* dsig, const double* u, double* out,
   const Level& l0 = h->L[0];
   const Geo geo = make_geo(h, true);
   hipLaunchKernelGGL(k_coef_from_sigma, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, geo, dsig,
-                     h->tcx, h->tcy, h->tcz, h->tbd, -1.0);
+                     h->tcx, h->tcy, h->tcz, h->tbd, -1.0, (size_t)0, (size_t)0);
   hipLaunchKernelGGL(k_coef_diag, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, l0.n1x, l0.n1y,
                      l0.nN, make_fastdiv(l0.n1x), make_fastdiv(l0.n1y), (const double*)h->tcx,
                      (const double*)h->tcy, (const double*)h->tcz, (const double*)h->tbd, h->tdg,
-                     h->tidg, 0);
-  Sten<double> s;
+                     h->tidg, 0, (size_t)0);
+  Sten<double> s{};
   s.cx = h->tcx; s.cy = h->tcy; s.cz = h->tcz; s.dg = h->tdg; s.idg = h->tidg;
   s.n1x = l0.n1x; s.n1y = l0.n1y; s.n1z = l0.n1z; s.nN = l0.nN; s.plane = l0.plane;
   s.fx = make_fastdiv(l0.n1x); s.fy = make_fastdiv(l0.n1y);

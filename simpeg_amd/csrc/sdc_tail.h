@@ -27,6 +27,9 @@ struct TailDesc {
   uint32_t offY;     // LDS offset of the shared scratch vector (size of the largest level)
   const float* ainv; // coarsest dense inverse, nn x nn, symmetric
   int nn;
+  const int32_t* cset;  // operator sets: column c uses set cset[c] (null: one set); the levels'
+                        // Sten / Xfer carry their own strides, the dense inverse astride
+  uint32_t astride;
   int nu_pre, nu_post;
   float c1[16], c2[16];
 };
@@ -139,10 +142,18 @@ template <typename S>
 __global__ __launch_bounds__(1024) void k_tail_vcycle(TailDesc d, const S* __restrict__ b,
                                                       S* __restrict__ x, size_t ld,
                                                       const int* __restrict__ done) {
-  if (done && *done) return;
+  if (done && ld_flag(done)) return;
   extern __shared__ float lds[];
   const size_t col = blockIdx.x;
   float* y = lds + d.offY;
+  // the operator set of this column: every level's coefficients and weights, the dense inverse
+  if (d.cset) {
+    for (int l = 0; l < d.nlev; ++l) {
+      d.lv[l].s = sten_col(d.lv[l].s, (int)col);
+      if (l + 1 < d.nlev) d.lv[l].t = xfer_col(d.lv[l].t, (int)col);
+    }
+    d.ainv += (size_t)d.cset[col] * d.astride;
+  }
   {
     const TailLevel& L0 = d.lv[0];
     float* B = lds + L0.offB;

@@ -147,6 +147,44 @@ class NodalEngine:
         _lib.check(self.lib.sdc_set_sigma(self.h, _lib.ptr(s), self._stream()), "sdc_set_sigma")
         self._sigma = s
 
+    def set_sigma_sets(self, sigmas):
+        """Build one operator per row of ``sigmas`` (nset, nC) on this mesh (operator sets, for
+        solve_sets)."""
+        if isinstance(sigmas, torch.Tensor):
+            s = sigmas.to(self.device, torch.float64)
+        else:
+            s = torch.as_tensor(np.asarray(sigmas, dtype=np.float64)).to(self.device)
+        if s.ndim != 2 or s.shape[1] != self.nC:
+            raise ValueError(f"sigmas must have shape (nset, {self.nC}), got {tuple(s.shape)}")
+        s = s.contiguous()
+        _lib.check(self.lib.sdc_set_sigma_sets(self.h, int(s.shape[0]), _lib.ptr(s),
+                                               self._stream()), "sdc_set_sigma_sets")
+        self._sigma = s
+        return s
+
+    def solve_sets(self, q, colset, ncol=None, mode="mixed", tol=1e-10, inner_tol=1e-5,
+                   max_outer=8, max_iter=500, u=None, raise_on_fail=True):
+        """solve() with column c against operator set colset[c] (sets from set_sigma_sets; every
+        aligned group of 8 columns must use one set)."""
+        ncol = q.shape[0] if ncol is None else ncol
+        cs = np.ascontiguousarray(np.asarray(colset, dtype=np.int32).ravel())
+        if cs.size < ncol:
+            raise ValueError(f"colset has {cs.size} entries, {ncol} needed")
+        if u is None:
+            u = self.block(q.shape[0], zero=False)
+        o = _lib.SolveOpts()
+        o.mode = {"fp64": _lib.SDC_MODE_FP64, "mixed": _lib.SDC_MODE_MIXED,
+                  "mixed_ir": _lib.SDC_MODE_MIXED_IR}[mode]
+        o.tol, o.inner_tol, o.max_outer, o.max_iter = tol, inner_tol, max_outer, max_iter
+        st = _lib.SolveStats()
+        rc = self.lib.sdc_solve_sets(self.h, _lib.ptr(q), _lib.ptr(u), ncol,
+                                     cs.ctypes.data_as(C.c_void_p), C.byref(o), C.byref(st),
+                                     self._stream())
+        res = SolveResult(st.outer, st.iters, st.max_rel_res, bool(st.converged), st.fp64_iters)
+        if rc != _lib.SDC_ERR_NOCONV or raise_on_fail:
+            _lib.check(rc, "sdc_solve_sets")
+        return u, res
+
     def set_coefficients(self, cx, cy, cz, bd):
         """Set the finest-level operator from its edge conductances (+x/+y/+z edge of every node)
         and the remaining diagonal instead of from sigma (host or device arrays of nN)."""
@@ -218,13 +256,16 @@ class NodalEngine:
                                              C.byref(by)), "sdc_profile_read")
         return int(n.value), float(ms.value), float(by.value)
 
-    def jvec_rhs(self, dsig, u, ncol=None):
+    def jvec_rhs(self, dsig, u, ncol=None, out=None):
         ncol = u.shape[0] if ncol is None else ncol
         if isinstance(dsig, torch.Tensor):
             ds = dsig.to(self.device, torch.float64).contiguous()
         else:
             ds = torch.as_tensor(np.asarray(dsig, dtype=np.float64)).to(self.device).contiguous()
-        out = torch.empty_like(u)
+        if out is None:
+            out = torch.empty_like(u)
+        elif out.shape[0] < ncol or out.shape[1] != self.ld or not out.is_contiguous():
+            raise ValueError("out must be a contiguous (>= ncol, ld) block")
         _lib.check(self.lib.sdc_jvec_rhs(self.h, _lib.ptr(ds), _lib.ptr(u), _lib.ptr(out), ncol,
                                          self._stream()), "sdc_jvec_rhs")
         return out
@@ -293,9 +334,13 @@ class NodalEngine:
                    "sdc_gather")
         return d[: g.nrow]
 
-    def scatter(self, s: ScatterCSR, v, ncol):
+    def scatter(self, s: ScatterCSR, v, ncol, out=None):
+        """P^T v into ncol columns (zeroed first) of a new block, or of ``out``."""
         v = torch.as_tensor(v, dtype=torch.float64).to(self.device).contiguous()
-        out = self.block(ncol, zero=False)
+        if out is None:
+            out = self.block(ncol, zero=False)
+        elif out.shape[0] < ncol or out.shape[1] != self.ld or not out.is_contiguous():
+            raise ValueError("out must be a contiguous (>= ncol, ld) block")
         _lib.check(self.lib.sdc_scatter(self.h, s.nrow, _lib.ptr(s.dst), _lib.ptr(s.rowptr),
                                         _lib.ptr(s.src), _lib.ptr(s.w), _lib.ptr(v), _lib.ptr(out),
                                         ncol, self._stream()), "sdc_scatter")

@@ -2,12 +2,15 @@
 
 ``MetaSimulation(simulations, mappings)``: fields / dpred / Jvec are per-simulation results
 concatenated, Jtvec = sum_i mapping_i'^T Jtvec_i, getJtJdiag sums the per-simulation diagonals
-through the mappings.  On the GPU the simulations run concurrently: each one is driven from its
-own host thread on its own HIP stream (the C library releases the GIL, and kernels of different
-streams overlap on the device), largest problems first.  With 4 hardware queues per process,
-8 streams still helped on MI355X: while one problem's host thread waits on its convergence poll
-another's kernels fill the queue (48 uneven problems: fields 150 ms with 4 streams, 130 ms with
-8, 129 ms with 12).
+through the mappings (meta/simulation.py:194-330).
+
+On the GPU, sub-simulations that share a mesh, boundary condition and solver settings form a
+group: one engine holds one operator per problem (operator sets), and each group's fields,
+Jvec and Jtvec are one batched solve over the sources of all its problems (every aligned group
+of 8 columns follows its own problem's operator).  A 48-problem batch on five mesh sizes thus
+runs five large solves instead of 48 small ones.  Groups and the remaining single simulations
+run concurrently, each driven from its own host thread on its own HIP stream (the C library
+releases the GIL), largest first.
 """
 
 from __future__ import annotations
@@ -17,6 +20,7 @@ import concurrent.futures as cf
 import numpy as np
 import torch
 
+from .engine import NodalEngine
 from .maps import Projection
 
 
@@ -32,8 +36,141 @@ def sum_adjoint(mappings, model, parts):
     return out
 
 
+def _group_key(sim):
+    """Simulations with equal keys can share one engine (same mesh, boundary and solver
+    settings); None: the simulation runs on its own engine."""
+    if sim.storeJ or sim.miniaturize or sim.surface_faces is not None:
+        return None
+    return (tuple(np.asarray(h, dtype=float).tobytes() for h in sim._widths),
+            np.asarray(sim._origin, dtype=float).tobytes(), sim._bc_type, sim.precision,
+            sim.rtol, sim.inner_rtol, sim.max_outer, sim.max_iter, str(sim._device),
+            sim._coarse_max, tuple(sorted(sim._gpu_params.items())), sim._deriv_includes_bc,
+            sim.autotune)
+
+
+class _Single:
+    """One simulation on its own engine."""
+
+    def __init__(self, i, sim):
+        self.members = [i]
+        self.sim = sim
+        self.cost = sim._nN * sim.survey.nSrc
+
+    def fields(self):
+        return [self.sim.fields(self.sim.model)]
+
+    def dpred(self, f):
+        return [self.sim.dpred(self.sim.model, f=f[0])]
+
+    def jvec(self, v, f):
+        return [self.sim.Jvec(self.sim.model, v[0], f=f[0])]
+
+    def jtvec(self, v, f):
+        return [self.sim.Jtvec(self.sim.model, v[0], f=f[0])]
+
+
+class _SetGroup:
+    """Simulations on one mesh: one engine with one operator set per simulation; the sources of
+    simulation k occupy columns offs[k] .. offs[k] + counts[k] of every block, each simulation's
+    block padded to a multiple of 8 columns with zero right-hand sides."""
+
+    def __init__(self, members, sims):
+        self.members = list(members)
+        self.sims = list(sims)
+        s0 = self.sims[0]
+        self.engine = NodalEngine(
+            s0._widths, s0._origin, bc_type="Neumann" if s0._bc_type == "Neumann" else "Robin",
+            device=s0._device, coarse_max=s0._coarse_max, params=s0._gpu_params)
+        self.counts = [s._sim_survey.nSrc for s in self.sims]
+        self.offs, colset = [], []
+        for k, n in enumerate(self.counts):
+            self.offs.append(len(colset))
+            colset += [k] * ((n + 7) // 8 * 8)
+        self.colset = np.asarray(colset, dtype=np.int32)
+        self.ncol = int(self.colset.size)
+        self.cost = sum(s._nN * n for s, n in zip(self.sims, self.counts))
+        self._q = None
+        self._tuned = False
+
+    def _views(self, block):
+        return [block[o:o + n] for o, n in zip(self.offs, self.counts)]
+
+    def _set_models(self):
+        eng = self.engine
+        eng.set_sigma_sets(np.stack([np.asarray(s.sigma, dtype=float) for s in self.sims]))
+        want = 1 if self.sims[0]._deriv_includes_bc else 0
+        if eng.params.deriv_robin != want:
+            eng.set_params(deriv_robin=want)
+
+    def _source(self):
+        if self._q is None:
+            q = self.engine.block(self.ncol)
+            for s, v in zip(self.sims, self._views(q)):
+                s._source_block(self.engine, out=v)
+            self._q = q
+        return self._q
+
+    def _solve(self, rhs):
+        s0 = self.sims[0]
+        u, res = self.engine.solve_sets(rhs, self.colset, self.ncol, mode=s0.precision,
+                                        tol=s0.rtol, inner_tol=s0.inner_rtol,
+                                        max_outer=s0.max_outer, max_iter=s0.max_iter)
+        for s in self.sims:
+            s.last_solve = res
+            s.solve_count += 1
+            s.solve_iterations += res.iters
+        return u
+
+    def fields(self):
+        from .resistivity.fields import Fields3DNodal
+
+        self._set_models()
+        q = self._source()
+        s0 = self.sims[0]
+        if s0.autotune and not self._tuned:
+            from .autotune import tune
+
+            tune(self.engine, q, self.ncol, mode="fp64" if s0.precision == "fp64" else "mixed",
+                 tol=s0.rtol)
+            self._tuned = True
+        u = self._solve(q)
+        return [Fields3DNodal(s, v, engine=self.engine) for s, v in zip(self.sims, self._views(u))]
+
+    def dpred(self, f):
+        eng = self.engine
+        return [eng.gather(s._projection(eng)["gather"], fk.u_block).cpu().numpy()
+                for s, fk in zip(self.sims, f)]
+
+    def jvec(self, v, f):
+        eng = self.engine
+        self._set_models()
+        rhs = eng.block(self.ncol)
+        for s, vk, fk, out in zip(self.sims, v, f, self._views(rhs)):
+            eng.jvec_rhs(s.sigmaDeriv @ np.asarray(vk, dtype=float), fk.u_block, out.shape[0],
+                         out=out)
+        du = self._solve(rhs)
+        return [eng.gather(s._projection(eng)["gather"], d).cpu().numpy()
+                for s, d in zip(self.sims, self._views(du))]
+
+    def jtvec(self, v, f):
+        eng = self.engine
+        self._set_models()
+        rhs = eng.block(self.ncol)
+        for s, vk, out in zip(self.sims, v, self._views(rhs)):
+            if np.size(vk) != s.survey.nD:
+                raise ValueError(f"v must have {s.survey.nD} entries, got {np.size(vk)}")
+            eng.scatter(s._projection(eng)["scatter"], np.asarray(vk, dtype=float), out.shape[0],
+                        out=out)
+        lam = self._solve(rhs)
+        parts = []
+        for s, fk, lk in zip(self.sims, f, self._views(lam)):
+            g = eng.contract(fk.u_block, lk, lk.shape[0]).cpu().numpy()
+            parts.append(-s._sigma_deriv_T(g))
+        return parts
+
+
 class MetaSimulation:
-    def __init__(self, simulations, mappings, n_streams=8):
+    def __init__(self, simulations, mappings, n_streams=8, group=True):
         if len(simulations) != len(mappings):
             raise ValueError("Must provide the same number of mappings and simulations.")
         ids = [id(s) for s in simulations]
@@ -49,8 +186,18 @@ class MetaSimulation:
         self._streams = None
         vnD = [sim.survey.nD for sim in self.simulations]
         self._data_offsets = np.cumsum(np.r_[0, vnD])
-        cost = [sim._nN * sim.survey.nSrc for sim in self.simulations]
-        self._order = list(np.argsort(cost)[::-1])
+        # work units: groups of same-mesh simulations (two or more) and single simulations
+        keys = {}
+        for i, sim in enumerate(self.simulations):
+            k = _group_key(sim) if group else None
+            keys.setdefault(k if k is not None else ("single", i), []).append(i)
+        self._units = []
+        for members in keys.values():
+            if len(members) > 1:
+                self._units.append(_SetGroup(members, [self.simulations[i] for i in members]))
+            else:
+                self._units.append(_Single(members[0], self.simulations[members[0]]))
+        self._units.sort(key=lambda u: -u.cost)
 
     @property
     def nD(self):
@@ -67,48 +214,54 @@ class MetaSimulation:
             sim.model = None if value is None else mp * self._model
 
     def _map(self, fn):
-        """fn(i) for every simulation, concurrently on separate streams; results in order."""
-        if not torch.cuda.is_available() or self.n_streams == 1:
-            return [fn(i) for i in range(len(self.simulations))]
-        if self._streams is None:
-            self._streams = [torch.cuda.Stream() for _ in range(self.n_streams)]
-        out = [None] * len(self.simulations)
+        """fn(unit) for every work unit, concurrently on separate streams (longest first);
+        returns the per-simulation results in simulation order."""
+        units = self._units
+        if not torch.cuda.is_available() or self.n_streams == 1 or len(units) == 1:
+            outs = [fn(u) for u in units]
+        else:
+            if self._streams is None:
+                self._streams = [torch.cuda.Stream() for _ in range(self.n_streams)]
+            outs = [None] * len(units)
 
-        def work(slot, idx_list):
-            with torch.cuda.stream(self._streams[slot]):
-                for i in idx_list:
-                    out[i] = fn(i)
-                torch.cuda.current_stream().synchronize()
+            def work(slot, idx_list):
+                with torch.cuda.stream(self._streams[slot]):
+                    for j in idx_list:
+                        outs[j] = fn(units[j])
+                    torch.cuda.current_stream().synchronize()
 
-        # longest-processing-time-first assignment to the stream slots
-        load = [0.0] * self.n_streams
-        plan = [[] for _ in range(self.n_streams)]
-        for i in self._order:
-            s = int(np.argmin(load))
-            plan[s].append(i)
-            load[s] += self.simulations[i]._nN * self.simulations[i].survey.nSrc
-        with cf.ThreadPoolExecutor(self.n_streams) as ex:
-            futs = [ex.submit(work, s, plan[s]) for s in range(self.n_streams) if plan[s]]
-            for fu in futs:
-                fu.result()
-        return out
+            load = [0.0] * self.n_streams
+            plan = [[] for _ in range(self.n_streams)]
+            for j, u in enumerate(units):
+                s = int(np.argmin(load))
+                plan[s].append(j)
+                load[s] += u.cost
+            with cf.ThreadPoolExecutor(self.n_streams) as ex:
+                futs = [ex.submit(work, s, plan[s]) for s in range(self.n_streams) if plan[s]]
+                for fu in futs:
+                    fu.result()
+        res = [None] * len(self.simulations)
+        for u, o in zip(units, outs):
+            for i, r in zip(u.members, o):
+                res[i] = r
+        return res
 
     def fields(self, m):
         self.model = m
-        return self._map(lambda i: self.simulations[i].fields(self.simulations[i].model))
+        return self._map(lambda u: u.fields())
 
     def dpred(self, m=None, f=None):
         if f is None:
             f = self.fields(self._model if m is None else m)
-        parts = self._map(lambda i: self.simulations[i].dpred(self.simulations[i].model, f=f[i]))
+        parts = self._map(lambda u: u.dpred([f[i] for i in u.members]))
         return np.concatenate(parts)
 
     def Jvec(self, m, v, f=None):
         self.model = m
         if f is None:
             f = self.fields(m)
-        parts = self._map(lambda i: self.simulations[i].Jvec(
-            self.simulations[i].model, self.mappings[i].deriv(self._model) @ v, f=f[i]))
+        vm = [mp.deriv(self._model) @ v for mp in self.mappings]
+        parts = self._map(lambda u: u.jvec([vm[i] for i in u.members], [f[i] for i in u.members]))
         return np.concatenate(parts)
 
     def Jtvec(self, m, v, f=None):
@@ -116,8 +269,9 @@ class MetaSimulation:
         if f is None:
             f = self.fields(m)
         v = np.asarray(v, dtype=float)
-        parts = self._map(lambda i: self.simulations[i].Jtvec(
-            self.simulations[i].model, v[self._data_offsets[i]:self._data_offsets[i + 1]], f=f[i]))
+        vs = [v[self._data_offsets[i]:self._data_offsets[i + 1]]
+              for i in range(len(self.simulations))]
+        parts = self._map(lambda u: u.jtvec([vs[i] for i in u.members], [f[i] for i in u.members]))
         return sum_adjoint(self.mappings, self._model, parts)
 
     def getJtJdiag(self, m, W=None, f=None):

@@ -25,8 +25,11 @@ class Fields3DNodal:
     }
     dtype = float
 
-    def __init__(self, simulation, u_block=None):
+    def __init__(self, simulation, u_block=None, engine=None):
         self.simulation = simulation
+        # the engine whose block holds the potentials (a MetaSimulation's shared engine, or the
+        # simulation's own)
+        self._eng = engine
         self.mesh = simulation.mesh
         # FieldsDC.survey: the mini survey when the simulation is miniaturised (fields.py:13-17)
         self.survey = simulation._sim_survey
@@ -37,7 +40,7 @@ class Fields3DNodal:
     @property
     def phi_host(self):
         if self._host is None:
-            eng = self.simulation._engine
+            eng = self._eng if self._eng is not None else self.simulation.engine
             self._host = eng.from_block(self.u_block, self.survey.nSrc)
         return self._host
 
@@ -90,6 +93,6 @@ class Fields3DNodal:
         host = self.phi_host.copy()
         value = np.asarray(value, dtype=float)
         host[:, cols] = value.reshape(host.shape[0], -1)
-        eng = self.simulation._engine
+        eng = self._eng if self._eng is not None else self.simulation.engine
         self.u_block = eng.to_block(host)
         self._host = host

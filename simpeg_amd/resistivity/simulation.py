@@ -129,6 +129,7 @@ class Simulation3DNodal:
         self._sd_cache = None
         self._sd_diag = None
         self._proj = None
+        self._proj_other = {}
         self._q = None
         self._q_block = None
         self.last_solve = None
@@ -152,6 +153,7 @@ class Simulation3DNodal:
             raise TypeError("survey must be a DC Survey")
         self._survey = value
         self._proj = None
+        self._proj_other = {}
         self._q = None
         self._q_block = None
         self._Jmatrix = None
@@ -305,11 +307,21 @@ class Simulation3DNodal:
         return u
 
     # ------------------------------------------------------------------ survey -> sparse maps
-    def _projection(self):
-        """Receiver projections (all data) and source right-hand sides as sparse COO."""
+    def _projection(self, eng=None):
+        """Receiver projections (all data) and source right-hand sides as sparse COO.  ``eng``:
+        build the device tables for another engine on the same mesh (MetaSimulation's shared
+        engines); they are cached per engine."""
+        if eng is not None and eng is not self._engine:
+            key = id(eng)
+            if key not in self._proj_other:
+                self._proj_other[key] = self._build_projection(eng)
+            return self._proj_other[key]
         if self._proj is not None:
             return self._proj
-        eng = self.engine
+        self._proj = self._build_projection(self.engine)
+        return self._proj
+
+    def _build_projection(self, eng):
         ld = eng.ld
         rows, flat, w, nodes, dsrc = [], [], [], [], []
         i0 = 0
@@ -344,9 +356,8 @@ class Simulation3DNodal:
         gather = eng.gather_csr(rp, idx, ww)
         dst, srp, ssrc, sw = coo_to_scatter(flat, rows, w)
         scatter = eng.scatter_csr(dst, srp, ssrc, sw)
-        self._proj = dict(nD=nD, rows=rows, nodes=nodes, w=w, data_src=data_src,
-                          gather=gather, scatter=scatter)
-        return self._proj
+        return dict(nD=nD, rows=rows, nodes=nodes, w=w, data_src=data_src, gather=gather,
+                    scatter=scatter)
 
     def getSourceTerm(self):
         """q (n_nodes, nSrc), Fortran ordered (resistivity/simulation.py:265-289)."""
@@ -360,23 +371,29 @@ class Simulation3DNodal:
 
     getRHS = getSourceTerm
 
-    def _source_block(self):
+    def _source_block(self, eng=None, out=None):
+        """q as a device block (nSrc, ld).  ``eng``/``out``: write it into ``out`` (a block of
+        another engine on the same mesh) instead of caching it on this simulation."""
+        if eng is not None:
+            return self._build_source_block(eng, out)
         if self._q_block is None:
-            eng = self.engine
-            dest, col, val = [], [], []
-            for i, src in enumerate(self._sim_survey.source_list):
-                locs, cur = src.electrodes()
-                idx, wt = node_interpolation(self._widths, self._origin, locs)
-                dest.append(i * eng.ld + idx.ravel())
-                val.append((cur[:, None] * wt).ravel())
-            dest = np.concatenate(dest)
-            val = np.concatenate(val)
-            keep = val != 0.0
-            dest, val = dest[keep], val[keep]
-            d, rp, sidx, sw = coo_to_scatter(dest, np.zeros(dest.size, np.int64), val)
-            sc = eng.scatter_csr(d, rp, sidx, sw)
-            self._q_block = eng.scatter(sc, np.ones(1), self._sim_survey.nSrc)
+            self._q_block = self._build_source_block(self.engine, None)
         return self._q_block
+
+    def _build_source_block(self, eng, out):
+        dest, val = [], []
+        for i, src in enumerate(self._sim_survey.source_list):
+            locs, cur = src.electrodes()
+            idx, wt = node_interpolation(self._widths, self._origin, locs)
+            dest.append(i * eng.ld + idx.ravel())
+            val.append((cur[:, None] * wt).ravel())
+        dest = np.concatenate(dest)
+        val = np.concatenate(val)
+        keep = val != 0.0
+        dest, val = dest[keep], val[keep]
+        d, rp, sidx, sw = coo_to_scatter(dest, np.zeros(dest.size, np.int64), val)
+        sc = eng.scatter_csr(d, rp, sidx, sw)
+        return eng.scatter(sc, np.ones(1), self._sim_survey.nSrc, out=out)
 
     # ------------------------------------------------------------------ public API
     def fields(self, m=None, calcJ=True):

@@ -16,7 +16,7 @@ def batch():
     return W.uneven_batch(seed=1, n_problems=5, sides=(8, 12, 16), npad=2, min_src=2, max_src=6)
 
 
-def _meta(batch, n_streams):
+def _meta(batch, n_streams, group=True):
     from simpeg_amd import maps
     from simpeg_amd.meta import MetaSimulation
     from simpeg_amd.resistivity import Simulation3DNodal
@@ -29,7 +29,7 @@ def _meta(batch, n_streams):
         sims.append(Simulation3DNodal(wl.mesh, survey=wl.survey, sigmaMap=maps.ExpMap(wl.mesh)))
         mappings.append(maps.Projection(nP, slice(offs[k], offs[k + 1])))
     m = np.concatenate([wl.model for wl in batch])
-    return MetaSimulation(sims, mappings, n_streams=n_streams), m
+    return MetaSimulation(sims, mappings, n_streams=n_streams, group=group), m
 
 
 def test_meta_concurrent_equals_individual(gpu, batch):
@@ -58,3 +58,21 @@ def test_meta_concurrent_equals_individual(gpu, batch):
         assert rel(jtw[off:off + n], orc.Jtvec(wl.model, w[doff:doff + nd])) < 1e-6
         off += n
         doff += nd
+
+
+def test_meta_same_mesh_groups_equal_separate_engines(gpu, batch):
+    """Problems that share a mesh are solved together as operator sets of one engine; the data,
+    Jvec and Jtvec equal those of one engine per problem (both solved to 1e-10)."""
+    from simpeg_amd.meta import _SetGroup
+
+    meta_g, m = _meta(batch, 4, group=True)
+    meta_s, _ = _meta(batch, 4, group=False)
+    assert any(isinstance(u, _SetGroup) for u in meta_g._units)
+    assert not any(isinstance(u, _SetGroup) for u in meta_s._units)
+    fg, fs = meta_g.fields(m), meta_s.fields(m)
+    assert rel(meta_g.dpred(m, f=fg), meta_s.dpred(m, f=fs)) < 1e-8
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(m.size)
+    w = rng.standard_normal(meta_g.nD)
+    assert rel(meta_g.Jvec(m, v, f=fg), meta_s.Jvec(m, v, f=fs)) < 1e-8
+    assert rel(meta_g.Jtvec(m, w, f=fg), meta_s.Jtvec(m, w, f=fs)) < 1e-8
