@@ -118,6 +118,10 @@ typedef struct {
                                   include the boundary term d(AvgBC sigma)/dsigma; 0 = they leave
                                   it out, as the reference's getADeriv does when the model is a
                                   resistivity (rhoMap; resistivity/simulation.py:566-570) */
+  int32_t col_loop;            /* stencil kernels of the fp32 solver: 1 = each work-item takes
+                                  rhs_per_thread columns; 8 or 16 = each work-item loops over
+                                  that many columns one at a time with its node's coefficients
+                                  held (0 = automatic) */
 } sdc_params;
 
 typedef struct {

@@ -66,6 +66,7 @@ class Params(C.Structure):
         ("wcycle_smooth", C.c_int32),
         ("nu_coarse", C.c_int32),
         ("deriv_robin", C.c_int32),
+        ("col_loop", C.c_int32),
     ]
 
 

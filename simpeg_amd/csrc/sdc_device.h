@@ -192,7 +192,8 @@ __device__ __forceinline__ void stencil_cols(const NodeC<T>& c, const X* __restr
 template <int R>
 __device__ __forceinline__ void block_partials(double (&acc)[R], double* __restrict__ partial,
                                                int c0, int ncol, uint32_t nbx, uint32_t bx) {
-  constexpr int LOG_R = R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
+  static_assert(R == 1 || R == 2 || R == 4 || R == 8 || R == 16 || R == 32, "R: power of 2 <= 32");
+  constexpr int LOG_R = R == 32 ? 5 : R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
   __shared__ double red[R][SDC_BLOCK / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double a[R];

@@ -15,15 +15,26 @@
 
 // The linear work-group id (x fastest, as the dispatcher numbers them) is remapped so that
 // every XCD works through one contiguous range of (column group, node block) pairs.
-#define SDC_NODE_PROLOGUE_R(NN, REMAP)                                          \
+#define SDC_NODE_PROLOGUE_C(NN, REMAP, CPB)                                     \
   const uint32_t nbx = gridDim.x;                                               \
   const uint32_t lin0_ = blockIdx.x + nbx * blockIdx.y;                         \
   const uint32_t lin_ = (REMAP) ? xcd_remap(lin0_, nbx * gridDim.y) : lin0_;    \
   const uint32_t by_ = lin_ / nbx;                                              \
   const uint32_t bx = lin_ - by_ * nbx;                                         \
   const uint32_t n = bx * SDC_BLOCK + threadIdx.x;                              \
-  const int c0 = (int)by_ * R;                                                  \
+  const int c0 = (int)by_ * (CPB);                                              \
   const bool live = n < (NN);
+
+#define SDC_NODE_PROLOGUE_R(NN, REMAP) SDC_NODE_PROLOGUE_C(NN, REMAP, R)
+
+// Column loops: the stencil kernels take G groups of R columns per work-item, one group after
+// the other, with the node's coefficients loaded once (G = 1: one group).  With operator sets
+// the columns of one work-item may cross a set boundary (sets are aligned to 8 columns): the
+// operator is reselected where the set changes.
+template <typename T>
+__device__ __forceinline__ bool set_changes(const Sten<T>& sa, int c, int cprev) {
+  return sa.cset && sa.cset[c] != sa.cset[cprev];
+}
 
 #define SDC_NODE_PROLOGUE(NN) SDC_NODE_PROLOGUE_R(NN, 1)
 
@@ -67,50 +78,68 @@ __device__ __forceinline__ bool ctl_update(const int* ctl) {
 // ------------------------------------------------------------------------------------------
 // operator application: y = A x, optional per-column partial sums of x.y
 
-template <typename T, int R, bool DOT>
+template <typename T, int R, bool DOT, int G = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_apply(Sten<T> sa, const T* __restrict__ x,
                                                      T* __restrict__ y, size_t ld, int ncol,
                                                      double* __restrict__ partial,
                                                      const int* __restrict__ done) {
   if (done && ld_flag(done)) return;
-  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
-  const Sten<T> s = sten_col(sa, c0);
-  double acc[R];
+  SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
+  double acc[R * G];
 #pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int r = 0; r < R * G; ++r) acc[r] = 0.0;
   if (live) {
-    const NodeC<T> c = node_coef(s, n);
-    T xc[R], v[R];
-    stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, v);
+    Sten<T> s = sten_col(sa, c0);
+    NodeC<T> c = node_coef(s, n);
+#pragma unroll 2
+    for (int g = 0; g < G; ++g) {
+      const int cg = c0 + g * R;
+      if (cg >= ncol) break;
+      if (G > 1 && g > 0 && set_changes(sa, cg, cg - R)) {
+        s = sten_col(sa, cg);
+        c = node_coef(s, n);
+      }
+      T xc[R], v[R];
+      stencil_cols<T, R>(c, x, n, ld, cg, ncol, xc, v);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (c0 + r < ncol) {
-        st_once(y + (size_t)(c0 + r) * ld + n, v[r]);
-        if (DOT) acc[r] += (double)xc[r] * (double)v[r];
+      for (int r = 0; r < R; ++r) {
+        if (cg + r < ncol) {
+          st_once(y + (size_t)(cg + r) * ld + n, v[r]);
+          if (DOT) acc[g * R + r] += (double)xc[r] * (double)v[r];
+        }
       }
     }
   }
-  if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, bx);
+  if (DOT) block_partials<R * G>(acc, partial, c0, ncol, nbx, bx);
 }
 
 // r = b - A x.  b in type B, x and r in the V-cycle storage type X, arithmetic in T.
-template <typename T, typename B, typename X, int R>
+template <typename T, typename B, typename X, int R, int G = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_residual(Sten<T> sa, const B* __restrict__ b,
                                                         const X* __restrict__ x,
                                                         X* __restrict__ res, size_t ld, int ncol,
                                                         const int* __restrict__ done) {
   if (done && ld_flag(done)) return;
-  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
-  const Sten<T> s = sten_col(sa, c0);
+  SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
   if (!live) return;
-  const NodeC<T> c = node_coef(s, n);
-  T xc[R], v[R], bv[R];
+  Sten<T> s = sten_col(sa, c0);
+  NodeC<T> c = node_coef(s, n);
+#pragma unroll 2
+  for (int g = 0; g < G; ++g) {
+    const int cg = c0 + g * R;
+    if (cg >= ncol) break;
+    if (G > 1 && g > 0 && set_changes(sa, cg, cg - R)) {
+      s = sten_col(sa, cg);
+      c = node_coef(s, n);
+    }
+    T xc[R], v[R], bv[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) bv[r] = (T)ld_once(col_ptr(b, ld, c0, r, ncol) + n);
-  stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, v);
+    for (int r = 0; r < R; ++r) bv[r] = (T)ld_once(col_ptr(b, ld, cg, r, ncol) + n);
+    stencil_cols<T, R>(c, x, n, ld, cg, ncol, xc, v);
 #pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (c0 + r < ncol) st_once(res + (size_t)(c0 + r) * ld + n, (X)(bv[r] - v[r]));
+    for (int r = 0; r < R; ++r)
+      if (cg + r < ncol) st_once(res + (size_t)(cg + r) * ld + n, (X)(bv[r] - v[r]));
+  }
 }
 
 // First Chebyshev step from a zero guess: x = c0 * D^-1 b
@@ -140,77 +169,97 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_smooth0(Sten<T> sa, const B* __re
 // DOT adds per-column partial sums of b.xout (the r.z product of PCG).
 // b in type B, x and xprev in the storage type X, xout in type O (O = T on the last step of
 // the finest level, which hands z back to PCG).
-template <typename T, typename B, typename X, typename O, int R, int PREV, bool DOT>
+template <typename T, typename B, typename X, typename O, int R, int PREV, bool DOT, int G = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_cheb(Sten<T> sa, const B* __restrict__ b,
                                                     const X* __restrict__ x, const X* xprev,
                                                     O* xout, size_t ld, int ncol, T c1, T c2,
                                                     T cz, double* __restrict__ partial,
                                                     const int* __restrict__ done) {
   if (done && ld_flag(done)) return;
-  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
-  const Sten<T> s = sten_col(sa, c0);
-  double acc[R];
+  SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
+  double acc[R * G];
 #pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int r = 0; r < R * G; ++r) acc[r] = 0.0;
   if (live) {
-    const NodeC<T> c = node_coef(s, n);
-    const T g = s.idg[n];
-    const T w = c2 * g;
-    T xc[R], v[R], bv[R], pv[R];
+    Sten<T> s = sten_col(sa, c0);
+    NodeC<T> c = node_coef(s, n);
+    T g = s.idg[n];
+#pragma unroll 2
+    for (int q = 0; q < G; ++q) {
+      const int cg = c0 + q * R;
+      if (cg >= ncol) break;
+      if (G > 1 && q > 0 && set_changes(sa, cg, cg - R)) {
+        s = sten_col(sa, cg);
+        c = node_coef(s, n);
+        g = s.idg[n];
+      }
+      const T w = c2 * g;
+      T xc[R], v[R], bv[R], pv[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      bv[r] = (T)ld_once(col_ptr(b, ld, c0, r, ncol) + n);
-      pv[r] = (PREV == 1) ? (T)col_ptr(xprev, ld, c0, r, ncol)[n] : T(0);
-    }
-    stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, v);
+      for (int r = 0; r < R; ++r) {
+        bv[r] = (T)ld_once(col_ptr(b, ld, cg, r, ncol) + n);
+        pv[r] = (PREV == 1) ? (T)col_ptr(xprev, ld, cg, r, ncol)[n] : T(0);
+      }
+      stencil_cols<T, R>(c, x, n, ld, cg, ncol, xc, v);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (c0 + r < ncol) {
-        const size_t o = (size_t)(c0 + r) * ld + n;
-        T xn = xc[r] + w * (bv[r] - v[r]);
-        if (PREV == 1) xn += c1 * (xc[r] - pv[r]);
-        if (PREV == 2) xn += c1 * (xc[r] - cz * g * bv[r]);
-        const O xs = (O)xn;
-        st_once(xout + o, xs);
-        // the product uses the value stored (z may be rounded to bf16), as the next step does
-        if (DOT) acc[r] += (double)bv[r] * (double)(T)xs;
+      for (int r = 0; r < R; ++r) {
+        if (cg + r < ncol) {
+          const size_t o = (size_t)(cg + r) * ld + n;
+          T xn = xc[r] + w * (bv[r] - v[r]);
+          if (PREV == 1) xn += c1 * (xc[r] - pv[r]);
+          if (PREV == 2) xn += c1 * (xc[r] - cz * g * bv[r]);
+          const O xs = (O)xn;
+          st_once(xout + o, xs);
+          // the product uses the value stored (z may be rounded to bf16), as the next step does
+          if (DOT) acc[q * R + r] += (double)bv[r] * (double)(T)xs;
+        }
       }
     }
   }
-  if (DOT) block_partials<R>(acc, partial, c0, ncol, nbx, bx);
+  if (DOT) block_partials<R * G>(acc, partial, c0, ncol, nbx, bx);
 }
 
 // The first two Chebyshev steps from a zero guess in one pass:
 //   x1 = c0 D^-1 b (recomputed at the 7 stencil points, never stored), r1 = b - A x1,
 //   x2 = x1 + c1 x1 + c2 D^-1 r1.  Writes x2 only (a following step recomputes x1 from b).
-template <typename T, typename B, typename X, int R>
+template <typename T, typename B, typename X, int R, int G = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_smooth_pair(Sten<T> sa, const B* __restrict__ b,
                                                            X* __restrict__ xout, size_t ld,
                                                            int ncol, T cz, T c1, T c2,
                                                            const int* __restrict__ done) {
   if (done && ld_flag(done)) return;
-  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
-  const Sten<T> s = sten_col(sa, c0);
+  SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
   if (!live) return;
-  NodeC<T> c = node_coef(s, n);
-  const T g = s.idg[n];
   // A x1 = c0 * A' b with the neighbour weights scaled by their 1/diagonal
-  c.d *= g;
-  c.wxm *= s.idg[c.ixm];
-  c.wxp *= s.idg[c.ixp];
-  c.wym *= s.idg[c.iym];
-  c.wyp *= s.idg[c.iyp];
-  c.wzm *= s.idg[c.izm];
-  c.wzp *= s.idg[c.izp];
-  T bc[R], ab[R];
-  stencil_cols<T, R>(c, b, n, ld, c0, ncol, bc, ab);
+  auto scaled = [&](const Sten<T>& s, T& g) {
+    NodeC<T> c = node_coef(s, n);
+    g = s.idg[n];
+    c.d *= g;
+    c.wxm *= s.idg[c.ixm];
+    c.wxp *= s.idg[c.ixp];
+    c.wym *= s.idg[c.iym];
+    c.wyp *= s.idg[c.iyp];
+    c.wzm *= s.idg[c.izm];
+    c.wzp *= s.idg[c.izp];
+    return c;
+  };
+  T g;
+  NodeC<T> c = scaled(sten_col(sa, c0), g);
+#pragma unroll 2
+  for (int q = 0; q < G; ++q) {
+    const int cg = c0 + q * R;
+    if (cg >= ncol) break;
+    if (G > 1 && q > 0 && set_changes(sa, cg, cg - R)) c = scaled(sten_col(sa, cg), g);
+    T bc[R], ab[R];
+    stencil_cols<T, R>(c, b, n, ld, cg, ncol, bc, ab);
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (c0 + r < ncol) {
-      const size_t o = (size_t)(c0 + r) * ld + n;
-      const T x1 = cz * g * bc[r];
-      const T r1 = bc[r] - cz * ab[r];
-      st_once(xout + o, (X)(x1 + c1 * x1 + c2 * g * r1));
+    for (int r = 0; r < R; ++r) {
+      if (cg + r < ncol) {
+        const size_t o = (size_t)(cg + r) * ld + n;
+        const T x1 = cz * g * bc[r];
+        const T r1 = bc[r] - cz * ab[r];
+        st_once(xout + o, (X)(x1 + c1 * x1 + c2 * g * r1));
+      }
     }
   }
 }
@@ -680,7 +729,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* const* xp, const T*
 // FIRST: p = z, q = A z, no x update.  ctl[XFRESH]: x was folded away and counts as zero (not
 // read).  check_stop: return at once when ctl[STOP] is set (only needed when several iterations
 // run between host reads of the stop flag).
-template <typename Z, int R, bool FIRST>
+template <typename Z, int R, bool FIRST, int G = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z* __restrict__ z,
                                                         float* __restrict__ p,
                                                         float* __restrict__ q,
@@ -693,42 +742,47 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z*
                                                         int check_stop) {
   if (check_stop && ld_flag(ctl + SDC_CTL_STOP)) return;
   const int xfresh = ld_flag(ctl + SDC_CTL_XFRESH);
-  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
-  const Sten<float> s = sten_col(sa, c0);
-  double acc[R];
+  SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
+  double acc[R * G];
 #pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int r = 0; r < R * G; ++r) acc[r] = 0.0;
   if (live) {
-    const NodeC<float> c = node_coef(s, n);
-    float zc[R], az[R], po[R], qo[R], xo[R];
+    NodeC<float> c = node_coef(sten_col(sa, c0), n);
+#pragma unroll 2
+    for (int gq = 0; gq < G; ++gq) {
+      const int cg = c0 + gq * R;
+      if (cg >= ncol) break;
+      if (G > 1 && gq > 0 && set_changes(sa, cg, cg - R)) c = node_coef(sten_col(sa, cg), n);
+      float zc[R], az[R], po[R], qo[R], xo[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (!FIRST) {
-        const size_t o = (size_t)(c0 + r < ncol ? c0 + r : ncol - 1) * ld + n;
-        po[r] = ld_once(p + o);
-        qo[r] = ld_once(q + o);
-        xo[r] = xfresh ? 0.0f : ld_once(x + o);
-      }
-    }
-    stencil_cols<float, R>(c, z, n, ld, c0, ncol, zc, az);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (c0 + r < ncol) {
-        const size_t o = (size_t)(c0 + r) * ld + n;
-        float pn = zc[r], qn = az[r];
+      for (int r = 0; r < R; ++r) {
         if (!FIRST) {
-          const float be = (float)beta[c0 + r];
-          pn += be * po[r];
-          qn += be * qo[r];
-          st_once(x + o, xo[r] + (float)alpha_prev[c0 + r] * po[r]);
+          const size_t o = (size_t)(cg + r < ncol ? cg + r : ncol - 1) * ld + n;
+          po[r] = ld_once(p + o);
+          qo[r] = ld_once(q + o);
+          xo[r] = xfresh ? 0.0f : ld_once(x + o);
         }
-        st_once(p + o, pn);
-        st_once(q + o, qn);
-        acc[r] += (double)pn * (double)qn;
+      }
+      stencil_cols<float, R>(c, z, n, ld, cg, ncol, zc, az);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (cg + r < ncol) {
+          const size_t o = (size_t)(cg + r) * ld + n;
+          float pn = zc[r], qn = az[r];
+          if (!FIRST) {
+            const float be = (float)beta[cg + r];
+            pn += be * po[r];
+            qn += be * qo[r];
+            st_once(x + o, xo[r] + (float)alpha_prev[cg + r] * po[r]);
+          }
+          st_once(p + o, pn);
+          st_once(q + o, qn);
+          acc[gq * R + r] += (double)pn * (double)qn;
+        }
       }
     }
   }
-  block_partials<R>(acc, partial, c0, ncol, nbx, bx);
+  block_partials<R * G>(acc, partial, c0, ncol, nbx, bx);
 }
 
 // r -= alpha q and per-column |r|^2 partials (4 nodes per work-item)
@@ -774,6 +828,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
   if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
 }
 
+#define SDC_FOLD_CHUNKS 4
+
 // u += x + alpha p: the fp32 partial solution, including the pending x update of the last step,
 // folded into the fp64 solution.  Runs only when the control block asks for a reliable update;
 // x is not cleared (XFRESH then tells the next k_apply_pq to treat it as zero, and XFRESH set
@@ -787,31 +843,37 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_ctl(const double* const* 
                                                            const double* __restrict__ alpha,
                                                            size_t ld, uint32_t nN,
                                                            const int* __restrict__ ctl) {
+  // most iterations make no update: this kernel then only reads the flag, so a work-group
+  // covers SDC_FOLD_CHUNKS consecutive chunks of SDC_NODES4 nodes (a smaller grid to dispatch)
   if (!ctl_update(ctl)) return;
   double* __restrict__ u = const_cast<double*>(qu[1]);
   const bool ufresh = ld_flag(ctl + SDC_CTL_UFRESH) != 0,
              xfresh = ld_flag(ctl + SDC_CTL_XFRESH) != 0;
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
-  const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
   const float al = (float)alpha[c];
-  if (n0 + 3 < nN) {
-    const vec4_t<float> pv = *(const vec4_t<float>*)(p + o + n0);
-    const vec4_t<float> xv =
-        (xfresh ? vec4_t<float>(0.0f) : *(const vec4_t<float>*)(x + o + n0)) + al * pv;
-    vec4_t<double> uv = ufresh ? vec4_t<double>(0.0) : *(const vec4_t<double>*)(u + o + n0);
-    uv.x += (double)xv.x;
-    uv.y += (double)xv.y;
-    uv.z += (double)xv.z;
-    uv.w += (double)xv.w;
-    *(vec4_t<double>*)(u + o + n0) = uv;
-  } else {
-    const uint32_t end = ufresh ? (uint32_t)ld : nN;
-    for (uint32_t n = n0; n < end && n < n0 + 4; ++n) {
-      if (n < nN)
-        u[o + n] = (ufresh ? 0.0 : u[o + n]) + (double)((xfresh ? 0.0f : x[o + n]) + al * p[o + n]);
-      else
-        u[o + n] = 0.0;
+  const uint32_t end = ufresh ? (uint32_t)ld : nN;
+#pragma unroll
+  for (int ch = 0; ch < SDC_FOLD_CHUNKS; ++ch) {
+    const uint32_t n0 =
+        ((blockIdx.x * SDC_FOLD_CHUNKS + ch) * SDC_BLOCK + threadIdx.x) * 4;
+    if (n0 + 3 < nN) {
+      const vec4_t<float> pv = *(const vec4_t<float>*)(p + o + n0);
+      const vec4_t<float> xv =
+          (xfresh ? vec4_t<float>(0.0f) : *(const vec4_t<float>*)(x + o + n0)) + al * pv;
+      vec4_t<double> uv = ufresh ? vec4_t<double>(0.0) : *(const vec4_t<double>*)(u + o + n0);
+      uv.x += (double)xv.x;
+      uv.y += (double)xv.y;
+      uv.z += (double)xv.z;
+      uv.w += (double)xv.w;
+      *(vec4_t<double>*)(u + o + n0) = uv;
+    } else {
+      for (uint32_t n = n0; n < end && n < n0 + 4; ++n) {
+        if (n < nN)
+          u[o + n] = (ufresh ? 0.0 : u[o + n]) + (double)((xfresh ? 0.0f : x[o + n]) + al * p[o + n]);
+        else
+          u[o + n] = 0.0;
+      }
     }
   }
 }
@@ -1115,11 +1177,11 @@ struct PinFix {
 // ctl non-null: runs only when the control block asks for a reliable update, and q, u are read
 // from qu[0], qu[1] (device pointer table) instead of the arguments, so that a recorded
 // iteration serves every solve.
-template <typename TO, int R, bool U_ZERO>
+template <typename TO, int R, bool U_ZERO, int G = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> sa, const double* q,
                                                            const double* u,
                                                            TO* __restrict__ rout, size_t ld,
-                                                           int ncol, PinFix pf,
+                                                           int ncol, PinFix pf0,
                                                            double* __restrict__ partial,
                                                            unsigned* __restrict__ qmask,
                                                            __bf16* __restrict__ r16,
@@ -1130,55 +1192,70 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> sa, cons
     q = qu[0];
     u = qu[1];
   }
-  SDC_NODE_PROLOGUE_R(sa.nN, sa.remap)
-  const Sten<double> s = sten_col(sa, c0);
-  if (sa.cset) {
-    const size_t o = (size_t)sa.cset[c0] * sa.sstride;
-    pf.cx += o;
-    pf.cy += o;
-    pf.cz += o;
-  }
-  double acc[R];
+  SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
+  // the operator (and the node-0 edges of the Neumann shift) of column cg's set
+  auto select = [&](int cg, PinFix& pf) {
+    pf = pf0;
+    if (sa.cset) {
+      const size_t o = (size_t)sa.cset[cg] * sa.sstride;
+      pf.cx += o;
+      pf.cy += o;
+      pf.cz += o;
+    }
+    return sten_col(sa, cg);
+  };
+  double acc[R * G];
 #pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int r = 0; r < R * G; ++r) acc[r] = 0.0;
   if (live) {
+    PinFix pf;
+    Sten<double> s = select(c0, pf);
     NodeC<double> c;
     if (!U_ZERO) c = node_coef(s, n);
-    double wfix = 0.0;
-    if (pf.on) {
-      if (n == 1) wfix = pf.cx[0];
-      else if (n == pf.n1x) wfix = pf.cy[0];
-      else if (n == pf.plane) wfix = pf.cz[0];
-    }
     // Load order: the mask word, then the 7R stencil loads of u, then the q loads that depend
     // on the mask.  Issued the other way round, the branch on the mask held the stencil loads
     // back by one memory round trip per wavefront.
     const unsigned qm = U_ZERO ? 1u : qmask[n];
-    double qv[R], uc[R], au[R];
-    if (!U_ZERO) stencil_cols<double, R>(c, u, n, ld, c0, ncol, uc, au);
-    const bool readq = qm != 0u;
-#pragma unroll
-    for (int r = 0; r < R; ++r) qv[r] = readq ? ld_once(col_ptr(q, ld, c0, r, ncol) + n) : 0.0;
     bool anyq = false;
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (c0 + r < ncol) {
-        const size_t o = (size_t)(c0 + r) * ld;
-        double rv = qv[r];
-        anyq |= rv != 0.0;
-        if (!U_ZERO) rv -= au[r];
-        acc[r] = rv * rv;
-        if (wfix != 0.0) {
-          const double r0 = U_ZERO ? q[o] : q[o] - u[o];
-          rv += wfix * r0;
-        }
-        st_once(rout + o + n, (TO)rv);
-        if (r16) r16[o + n] = (__bf16)rv;
+#pragma unroll 2
+    for (int gq = 0; gq < G; ++gq) {
+      const int cg = c0 + gq * R;
+      if (cg >= ncol) break;
+      if (G > 1 && gq > 0 && set_changes(sa, cg, cg - R)) {
+        s = select(cg, pf);
+        if (!U_ZERO) c = node_coef(s, n);
       }
+      double wfix = 0.0;
+      if (pf.on) {
+        if (n == 1) wfix = pf.cx[0];
+        else if (n == pf.n1x) wfix = pf.cy[0];
+        else if (n == pf.plane) wfix = pf.cz[0];
+      }
+      double qv[R], uc[R], au[R];
+      if (!U_ZERO) stencil_cols<double, R>(c, u, n, ld, cg, ncol, uc, au);
+      const bool readq = qm != 0u;
+#pragma unroll
+      for (int r = 0; r < R; ++r) qv[r] = readq ? ld_once(col_ptr(q, ld, cg, r, ncol) + n) : 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (cg + r < ncol) {
+          const size_t o = (size_t)(cg + r) * ld;
+          double rv = qv[r];
+          anyq |= rv != 0.0;
+          if (!U_ZERO) rv -= au[r];
+          acc[gq * R + r] = rv * rv;
+          if (wfix != 0.0) {
+            const double r0 = U_ZERO ? q[o] : q[o] - u[o];
+            rv += wfix * r0;
+          }
+          st_once(rout + o + n, (TO)rv);
+          if (r16) r16[o + n] = (__bf16)rv;
+        }
+    }
     // every writer stores the same value; the mask is cleared before this pass
     if (U_ZERO && anyq) qmask[n] = 1u;
   }
-  block_partials<R>(acc, partial, c0, ncol, nbx, bx);
+  block_partials<R * G>(acc, partial, c0, ncol, nbx, bx);
 }
 
 // true-residual check after a reliable update: rr = |q - A u|^2, rmax = rr,

@@ -651,6 +651,27 @@ int pick_rx(const sdc_handle_s* h, int ncol) {
   return ncol >= 8 ? 8 : pick_r(h, ncol);
 }
 
+// Stencil kernels of the fp32 solver: R columns per work-item, or (col_loop 8 / 16) a loop over
+// G single columns per work-item.  Automatic: 1 (see sdc_params.col_loop).
+int pick_g(const sdc_handle_s* h, int ncol) {
+  const int g = h->prm.col_loop;
+  if ((g == 8 || g == 16) && ncol >= g) return g;
+  return 1;
+}
+
+// DISPATCH_RG: constexpr R and G for a launch; G > 1 runs with R = 1
+#define DISPATCH_RG(Rv, Gv, ...)                                        \
+  if ((Gv) == 16) {                                                     \
+    constexpr int R = 1, G = 16;                                        \
+    __VA_ARGS__;                                                        \
+  } else if ((Gv) == 8) {                                               \
+    constexpr int R = 1, G = 8;                                         \
+    __VA_ARGS__;                                                        \
+  } else {                                                              \
+    constexpr int G = 1;                                                \
+    DISPATCH_R(Rv, __VA_ARGS__)                                         \
+  }
+
 #define DISPATCH_R(Rv, ...)                     \
   switch (Rv) {                                 \
     case 8: { constexpr int R = 8; __VA_ARGS__; } break; \
@@ -737,6 +758,7 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
            hipStream_t st) {
   Work<T>& W = work_of<T>(h);
   const int R = pick_r(h, ncol);
+  const int G_ = pick_g(h, ncol);
   const Level& lv = h->L[l];
   const int nl = (int)h->L.size();
   if constexpr (std::is_same<T, float>::value && std::is_same<B, S>::value &&
@@ -792,8 +814,9 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   if (nu1 >= 2) {
     // steps 0 and 1 fused
     prof_begin(h, l == 0 ? SDC_PROF_PAIR : -1, st);
-    DISPATCH_R(R, launch(h, (k_smooth_pair<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
-                                     ncol, cz, (T)cc.c1[1], (T)cc.c2[1], done));
+    DISPATCH_RG(R, G_, launch(h, (k_smooth_pair<T, B, S, R, G>), grid2(lv.nN, ncol, R * G),
+                              blk, 0, st, s, b, cur, lv.ld, ncol, cz, (T)cc.c1[1],
+                              (T)cc.c2[1], done));
     prof_end(h, l == 0 ? SDC_PROF_PAIR : -1, st, nodes * 5 * sT + nodes * cols * (sB + sS));
   } else {
     DISPATCH_R(R, launch(h, (k_smooth0<T, B, S, R>), g, blk, 0, st, s, b, cur, lv.ld,
@@ -801,13 +824,15 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   }
   for (int k = 2; k < nu1; ++k) {
     if (k == 2) {
-      DISPATCH_R(R, launch(h, (k_cheb<T, B, S, S, R, 2, false>), g, blk, 0, st, s, b,
-                                       (const S*)cur, (const S*)nullptr, oth, lv.ld, ncol,
-                                       (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
+      DISPATCH_RG(R, G_, launch(h, (k_cheb<T, B, S, S, R, 2, false, G>),
+                                grid2(lv.nN, ncol, R * G), blk, 0, st, s, b, (const S*)cur,
+                                (const S*)nullptr, oth, lv.ld, ncol, (T)cc.c1[k], (T)cc.c2[k],
+                                cz, nodot, done));
     } else {
-      DISPATCH_R(R, launch(h, (k_cheb<T, B, S, S, R, 1, false>), g, blk, 0, st, s, b,
-                                       (const S*)cur, (const S*)oth, oth, lv.ld, ncol,
-                                       (T)cc.c1[k], (T)cc.c2[k], cz, nodot, done));
+      DISPATCH_RG(R, G_, launch(h, (k_cheb<T, B, S, S, R, 1, false, G>),
+                                grid2(lv.nN, ncol, R * G), blk, 0, st, s, b, (const S*)cur,
+                                (const S*)oth, oth, lv.ld, ncol, (T)cc.c1[k], (T)cc.c2[k], cz,
+                                nodot, done));
     }
     std::swap(cur, oth);
   }
@@ -829,13 +854,13 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
       // PREV = 0 on the first step (no previous iterate), 1 after
 #define SDC_CHEB_STEP(PREV, OT, OUT)                                                           \
       if (part) {                                                                                \
-        DISPATCH_R(R, launch(h, (k_cheb<T, B, S, OT, R, PREV, true>), g, blk, 0, st, s, \
-                                         b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol, c1,  \
-                                         c2, cz, part, done));                                   \
+        DISPATCH_RG(R, G_, launch(h, (k_cheb<T, B, S, OT, R, PREV, true, G>),                    \
+                                  grid2(lv.nN, ncol, R * G), blk, 0, st, s, b, (const S*)cur,    \
+                                  (const S*)oth, OUT, lv.ld, ncol, c1, c2, cz, part, done));     \
       } else {                                                                                   \
-        DISPATCH_R(R, launch(h, (k_cheb<T, B, S, OT, R, PREV, false>), g, blk, 0, st,   \
-                                         s, b, (const S*)cur, (const S*)oth, OUT, lv.ld, ncol,   \
-                                         c1, c2, cz, part, done));                               \
+        DISPATCH_RG(R, G_, launch(h, (k_cheb<T, B, S, OT, R, PREV, false, G>),                   \
+                                  grid2(lv.nN, ncol, R * G), blk, 0, st, s, b, (const S*)cur,    \
+                                  (const S*)oth, OUT, lv.ld, ncol, c1, c2, cz, part, done));     \
       }
       if (sep && last) {
         if (k == 0) {
@@ -859,8 +884,8 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
   };
   for (int gam = 0; gam < ngam; ++gam) {
     prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
-    DISPATCH_R(R, launch(h, (k_residual<T, B, S, R>), g, blk, 0, st, s, b, (const S*)cur,
-                                     res, lv.ld, ncol, done));
+    DISPATCH_RG(R, G_, launch(h, (k_residual<T, B, S, R, G>), grid2(lv.nN, ncol, R * G), blk,
+                              0, st, s, b, (const S*)cur, res, lv.ld, ncol, done));
     prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * (sB + 2 * sS));
     const int rtag = l == 0 ? SDC_PROF_RESTRICT : -1;
     if (h->prm.xfer_separable) {
@@ -995,6 +1020,7 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   Work<T>& W = work_of<T>(h);
   const Level& l0 = h->L[0];
   const int R = pick_r(h, ncol);
+  const int G_ = pick_g(h, ncol);
   const dim3 g = grid2(l0.nN, ncol, R);
   const dim3 blk(SDC_BLOCK);
   const uint32_t nbx = nblk(l0.nN);
@@ -1026,8 +1052,8 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
   // beta, p = z + beta p -- every launch through launch() so that it can be recorded
   auto iteration = [&](hipStream_t s2) -> int {
     prof_begin(h, SDC_PROF_APPLY, s2);
-    DISPATCH_R(R, launch(h, (k_apply<T, R, true>), g, blk, 0, s2, s, (const T*)W.p, W.q, l0.ld,
-                         ncol, W.partial, vdone));
+    DISPATCH_RG(R, G_, launch(h, (k_apply<T, R, true, G>), grid2(l0.nN, ncol, R * G), blk, 0,
+                              s2, s, (const T*)W.p, W.q, l0.ld, ncol, W.partial, vdone));
     prof_end(h, SDC_PROF_APPLY, s2,
              (double)l0.nN * 4 * sizeof(T) + (double)l0.nN * ncol * 2 * sizeof(T));
     launch(h, k_fin_alpha, dim3(ncol), blk, 0, s2, (const double*)W.partial, nbx,
@@ -1085,6 +1111,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   Work<float>& W = h->w32;
   const Level& l0 = h->L[0];
   const int R = pick_r(h, ncol);
+  const int G_ = pick_g(h, ncol);
   const dim3 g = grid2(l0.nN, ncol, R);
   const dim3 blk(SDC_BLOCK);
   const uint32_t nbx = nblk(l0.nN);
@@ -1173,13 +1200,15 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     prof_begin(h, atag, s);
 #define SDC_APPLY_PQ(ZT, ZP)                                                                   \
     if (first) {                                                                               \
-      DISPATCH_R(R, launch(h, (k_apply_pq<ZT, R, true>), g, blk, 0, s, s1, ZP, W.p, W.q, W.x,  \
-                           (const double*)W.beta, (const double*)W.alpha, l0.ld, ncol,         \
-                           W.partial, (const int*)W.ctl, K > 1 ? 1 : 0));                      \
+      DISPATCH_RG(R, G_, launch(h, (k_apply_pq<ZT, R, true, G>), grid2(l0.nN, ncol, R * G),    \
+                                blk, 0, s, s1, ZP, W.p, W.q, W.x, (const double*)W.beta,       \
+                                (const double*)W.alpha, l0.ld, ncol, W.partial,                \
+                                (const int*)W.ctl, K > 1 ? 1 : 0));                            \
     } else {                                                                                   \
-      DISPATCH_R(R, launch(h, (k_apply_pq<ZT, R, false>), g, blk, 0, s, s1, ZP, W.p, W.q, W.x, \
-                           (const double*)W.beta, (const double*)W.alpha, l0.ld, ncol,         \
-                           W.partial, (const int*)W.ctl, K > 1 ? 1 : 0));                      \
+      DISPATCH_RG(R, G_, launch(h, (k_apply_pq<ZT, R, false, G>), grid2(l0.nN, ncol, R * G),   \
+                                blk, 0, s, s1, ZP, W.p, W.q, W.x, (const double*)W.beta,       \
+                                (const double*)W.alpha, l0.ld, ncol, W.partial,                \
+                                (const int*)W.ctl, K > 1 ? 1 : 0));                            \
     }
     if (zb) {
       SDC_APPLY_PQ(__bf16, (const __bf16*)z16)
@@ -1215,15 +1244,17 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       fold_fresh = hc[SDC_CTL_UFRESH] != 0;
     }
     prof_begin(h, upd ? SDC_PROF_FOLD : -1, s);
-    launch(h, k_ir_fold_ctl, grid4((uint32_t)l0.ld, ncol), blk, 0, s,
+    launch(h, k_ir_fold_ctl,
+           dim3((nblk4(l0.ld) + SDC_FOLD_CHUNKS - 1) / SDC_FOLD_CHUNKS, (uint32_t)ncol), blk, 0, s,
            (const double* const*)W.qu, (const float*)W.x, (const float*)W.p,
            (const double*)W.alpha, l0.ld, l0.nN, (const int*)W.ctl);
     prof_end(h, upd ? SDC_PROF_FOLD : -1, s, (double)l0.nN * ncol * (fold_fresh ? 16 : 24));
     prof_begin(h, upd ? SDC_PROF_REFINE : -1, s);
-    DISPATCH_R(R64, launch(h, (k_ir_residual<float, R, false>), grid2(l0.nN, ncol, R64), blk, 0,
-                           s, s2, (const double*)nullptr, (const double*)nullptr, W.r, l0.ld,
-                           ncol, pf, W.partial, (unsigned*)W.qmask, r16, (const int*)W.ctl,
-                           (const double* const*)W.qu));
+    DISPATCH_RG(R64, G_, launch(h, (k_ir_residual<float, R, false, G>),
+                                grid2(l0.nN, ncol, R * G), blk, 0, s, s2,
+                                (const double*)nullptr, (const double*)nullptr, W.r, l0.ld,
+                                ncol, pf, W.partial, (unsigned*)W.qmask, r16, (const int*)W.ctl,
+                                (const double* const*)W.qu));
     // algorithmic bytes: fp64 coefficients, per column u64 read and r32 write; q is read on
     // source nodes only (not counted)
     prof_end(h, upd ? SDC_PROF_REFINE : -1, s, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
@@ -1365,6 +1396,7 @@ int sdc_default_params(sdc_params* p) {
   p->wcycle_smooth = 1;
   p->nu_coarse = 0;
   p->deriv_robin = 1;
+  p->col_loop = 0;
   p->check_every = 1;
   p->reliable_delta = 1e-4;
   return SDC_OK;
@@ -1535,6 +1567,8 @@ int sdc_set_params(sdc_handle h, const sdc_params* p) {
     return fail(SDC_ERR_ARG, "wcycle_smooth must be in [0, 15]");
   if (p->nu_coarse < 0 || p->nu_coarse > 15) return fail(SDC_ERR_ARG, "nu_coarse must be in [0, 15]");
   if (p->deriv_robin != 0 && p->deriv_robin != 1) return fail(SDC_ERR_ARG, "deriv_robin must be 0 or 1");
+  if (p->col_loop != 0 && p->col_loop != 1 && p->col_loop != 8 && p->col_loop != 16)
+    return fail(SDC_ERR_ARG, "col_loop must be 0, 1, 8 or 16");
   DeviceGuard dg(h->device);
   drop_graphs(h);
   h->prm = *p;

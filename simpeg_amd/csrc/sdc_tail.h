@@ -146,14 +146,10 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(TailDesc d, const S* __res
   extern __shared__ float lds[];
   const size_t col = blockIdx.x;
   float* y = lds + d.offY;
-  // the operator set of this column: every level's coefficients and weights, the dense inverse
-  if (d.cset) {
-    for (int l = 0; l < d.nlev; ++l) {
-      d.lv[l].s = sten_col(d.lv[l].s, (int)col);
-      if (l + 1 < d.nlev) d.lv[l].t = xfer_col(d.lv[l].t, (int)col);
-    }
-    d.ainv += (size_t)d.cset[col] * d.astride;
-  }
+  // the operator set of this column: each level's coefficients and weights are selected into
+  // local copies where they are used (the descriptor itself is not modified: a written kernel
+  // argument of this size is copied to scratch memory), and the dense inverse here
+  const float* ainv = d.ainv + (d.cset ? (size_t)d.cset[col] * d.astride : (size_t)0);
   {
     const TailLevel& L0 = d.lv[0];
     float* B = lds + L0.offB;
@@ -165,11 +161,12 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(TailDesc d, const S* __res
   // down: pre-smoothing, residual, restriction
   for (int l = 0; l < last; ++l) {
     const TailLevel& L = d.lv[l];
+    const Sten<float> s = sten_col(L.s, (int)col);
     float* B = lds + L.offB;
     float* X = lds + L.offX;
-    tail::smooth(d, L.s, B, X, y, d.nu_pre, true);
-    tail::residual(L.s, B, X, y);
-    tail::restrict_to(L.t, y, lds + d.lv[l + 1].offB);
+    tail::smooth(d, s, B, X, y, d.nu_pre, true);
+    tail::residual(s, B, X, y);
+    tail::restrict_to(xfer_col(L.t, (int)col), y, lds + d.lv[l + 1].offB);
   }
   // coarsest: dense inverse (symmetric: column m read as row m, consecutive threads consecutive)
   {
@@ -178,7 +175,7 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(TailDesc d, const S* __res
     float* X = lds + L.offX;
     for (int r = threadIdx.x; r < d.nn; r += blockDim.x) {
       float acc = 0.0f;
-      for (int m = 0; m < d.nn; ++m) acc += d.ainv[(size_t)m * d.nn + r] * B[m];
+      for (int m = 0; m < d.nn; ++m) acc += ainv[(size_t)m * d.nn + r] * B[m];
       X[r] = acc;
     }
     __syncthreads();
@@ -188,8 +185,8 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(TailDesc d, const S* __res
     const TailLevel& L = d.lv[l];
     float* B = lds + L.offB;
     float* X = lds + L.offX;
-    tail::prolong_add(L.t, lds + d.lv[l + 1].offX, X);
-    tail::smooth(d, L.s, B, X, y, d.nu_post, false);
+    tail::prolong_add(xfer_col(L.t, (int)col), lds + d.lv[l + 1].offX, X);
+    tail::smooth(d, sten_col(L.s, (int)col), B, X, y, d.nu_post, false);
   }
   {
     const TailLevel& L0 = d.lv[0];
