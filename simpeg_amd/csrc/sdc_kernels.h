@@ -548,76 +548,109 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_restrict_axis(AxisXfer t, const i
       st_once(out + (size_t)(c0 + r) * ldo + n, (X)(wl * v[r][0] + v[r][1] + wh * v[r][2]));
 }
 
-// xf += P xc.  Every fine node reads the 2x2x2 coarse nodes around it (weights as in
-// prolong_weights); a coincident axis has weight 0 on its second node, whose index still lies
-// inside the coarse grid.  OPD: operator-dependent weights (t.wx, t.wy, t.wz), else the
-// geometric tables.
-// Load order: the parent index tables, the node's own x weight and the R fine values first
-// (none depends on another load), then the weights and coarse values addressed through the
-// parents -- two memory round trips per wavefront.
+// xf += P xc, one work-item per coarse node (I, J, K): it updates the fine nodes it owns, the
+// fine nodes from its own fine position up to (not including) the next coarse node's along
+// every axis (1 or 2 per axis, up to 8), all of which interpolate from the 2x2x2 coarse nodes
+// (I..I+1, J..J+1, K..K+1): one coarse load per owned fine node and column, and the weight
+// loads are shared by the owned nodes and the R columns.  A fine node's value is
+//   xf + sum over the parent-cell corners m = a + 2b + 4c (in that order) of w_m xc_m,
+//   w_m = (a ? fx : 1 - fx) (b ? fy_a : 1 - fy_a) (c ? fz_ab : 1 - fz_ab)
+// with fx the weight of the right x parent at the fine node, fy_a the weight of the upper y
+// parent on the Py grid at x parent a, fz_ab that of the upper z parent on the Pz grid
+// (operator-dependent: t.wx, t.wy, t.wz; else the geometric tables pwx, pwy, pwz).  An owned
+// node coincident with the coarse grid along an axis has weight 0 on the next coarse node
+// there, whose index is clamped into the grid.
 template <typename T, typename X, int R, bool OPD>
-__global__ __launch_bounds__(SDC_BLOCK) void k_prolong_add(Xfer<T> ta, const X* __restrict__ xc,
+__global__ __launch_bounds__(SDC_BLOCK) void k_prolong_own(Xfer<T> ta, const X* __restrict__ xc,
                                                            X* __restrict__ xf, size_t ldf,
                                                            size_t ldc, int ncol,
                                                            const int* __restrict__ done) {
   if (done && ld_flag(done)) return;
-  SDC_NODE_PROLOGUE_R(ta.fN, ta.remap)
+  SDC_NODE_PROLOGUE_R(ta.cN, ta.remap)
   const Xfer<T> t = xfer_col(ta, c0);
   if (!live) return;
-  const uint32_t rr = fdiv(n, t.ffx);
-  const uint32_t i = n - rr * t.fn1x;
-  const uint32_t k = fdiv(rr, t.ffy);
-  const uint32_t j = rr - k * t.fn1y;
-  const uint32_t I0 = (uint32_t)t.pix[i], J0 = (uint32_t)t.piy[j], K0 = (uint32_t)t.piz[k];
-  const T fx = OPD ? t.wx[n] : t.pwx[i];
-  T fv[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) fv[r] = (T)col_ptr((const X*)xf, ldf, c0, r, ncol)[n];
-  T fy[2], fz[2][2];
-  if (OPD) {
-    const uint32_t py = I0 + t.cn1x * (j + t.fn1y * k);
-    fy[0] = t.wy[py];
-    fy[1] = t.wy[py + 1];
-    const uint32_t pz = I0 + t.cn1x * J0 + t.cplane * k;
-    fz[0][0] = t.wz[pz];
-    fz[1][0] = t.wz[pz + 1];
-    fz[0][1] = t.wz[pz + t.cn1x];
-    fz[1][1] = t.wz[pz + t.cn1x + 1];
-  } else {
-    fy[0] = fy[1] = t.pwy[j];
-    fz[0][0] = fz[1][0] = fz[0][1] = fz[1][1] = t.pwz[k];
-  }
-  const uint32_t c000 = I0 + J0 * t.cn1x + K0 * t.cplane;
-  const uint32_t c010 = c000 + t.cn1x, c001 = c000 + t.cplane, c011 = c001 + t.cn1x;
+  const uint32_t rr = fdiv(n, t.cfx);
+  const uint32_t I = n - rr * t.cn1x;
+  const uint32_t K = fdiv(rr, t.cfy);
+  const uint32_t J = rr - K * t.cn1y;
+  const bool hx = I + 1 < t.cn1x, hy = J + 1 < t.cn1y, hz = (K + 1) * t.cplane < t.cN;
+  const uint32_t Ip = hx ? I + 1 : I, Jp = hy ? J + 1 : J, Kp = hz ? K + 1 : K;
+  const uint32_t i0 = (uint32_t)t.rix[I], j0 = (uint32_t)t.riy[J], k0 = (uint32_t)t.riz[K];
+  // owned nodes per axis: 2 where the coarse cell spans 2 fine cells
+  const bool ox = hx && (uint32_t)t.rix[Ip] == i0 + 2, oy = hy && (uint32_t)t.riy[Jp] == j0 + 2,
+             oz = hz && (uint32_t)t.riz[Kp] == k0 + 2;
+  const uint32_t c000 = I + J * t.cn1x + K * t.cplane;
+  const uint32_t dx = Ip - I, dy = (Jp - J) * t.cn1x, dz = (Kp - K) * t.cplane;
   T cv[R][8];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const X* c = col_ptr(xc, ldc, c0, r, ncol);
     cv[r][0] = (T)c[c000];
-    cv[r][1] = (T)c[c000 + 1];
-    cv[r][2] = (T)c[c010];
-    cv[r][3] = (T)c[c010 + 1];
-    cv[r][4] = (T)c[c001];
-    cv[r][5] = (T)c[c001 + 1];
-    cv[r][6] = (T)c[c011];
-    cv[r][7] = (T)c[c011 + 1];
+    cv[r][1] = (T)c[c000 + dx];
+    cv[r][2] = (T)c[c000 + dy];
+    cv[r][3] = (T)c[c000 + dy + dx];
+    cv[r][4] = (T)c[c000 + dz];
+    cv[r][5] = (T)c[c000 + dz + dx];
+    cv[r][6] = (T)c[c000 + dz + dy];
+    cv[r][7] = (T)c[c000 + dz + dy + dx];
   }
-  T w[8];
 #pragma unroll
-  for (int c = 0; c < 2; ++c)
+  for (int cz = 0; cz < 2; ++cz) {
+    if (cz && !oz) break;
+    const uint32_t k = k0 + cz;
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int by = 0; by < 2; ++by) {
+      if (by && !oy) continue;
+      const uint32_t j = j0 + by;
+      // y weights of the two x parents at fine (j, k); z weights of the four (x, y) parents
+      T fy[2] = {T(0), T(0)}, fz[2][2] = {{T(0), T(0)}, {T(0), T(0)}};
+      if (by) {
+        if (OPD) {
+          const uint32_t py = I + t.cn1x * (j + t.fn1y * k);
+          fy[0] = t.wy[py];
+          fy[1] = t.wy[py + dx];
+        } else {
+          fy[0] = fy[1] = t.pwy[j];
+        }
+      }
+      if (cz) {
+        if (OPD) {
+          const uint32_t pz = I + t.cn1x * J + t.cplane * k;
+          fz[0][0] = t.wz[pz];
+          fz[1][0] = t.wz[pz + dx];
+          fz[0][1] = t.wz[pz + dy];
+          fz[1][1] = t.wz[pz + dy + dx];
+        } else {
+          fz[0][0] = fz[1][0] = fz[0][1] = fz[1][1] = t.pwz[k];
+        }
+      }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
-        w[a + 2 * b + 4 * c] = (a ? fx : T(1) - fx) * (b ? fy[a] : T(1) - fy[a]) *
-                               (c ? fz[a][b] : T(1) - fz[a][b]);
+      for (int ax = 0; ax < 2; ++ax) {
+        if (ax && !ox) continue;
+        const uint32_t f = i0 + ax + t.fn1x * (j + t.fn1y * k);
+        const T fx = ax ? (OPD ? t.wx[f] : t.pwx[i0 + 1]) : T(0);
+        T fv[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (c0 + r < ncol) {
-      T acc = fv[r];
+        for (int r = 0; r < R; ++r) fv[r] = (T)col_ptr((const X*)xf, ldf, c0, r, ncol)[f];
+        T w[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) acc += w[q] * cv[r][q];
-      xf[(size_t)(c0 + r) * ldf + n] = (X)acc;
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+              w[a + 2 * b + 4 * c] = (a ? fx : T(1) - fx) * (b ? fy[a] : T(1) - fy[a]) *
+                                     (c ? fz[a][b] : T(1) - fz[a][b]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (c0 + r < ncol) {
+            T acc = fv[r];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += w[q] * cv[r][q];
+            xf[(size_t)(c0 + r) * ldf + f] = (X)acc;
+          }
+        }
+      }
     }
   }
 }

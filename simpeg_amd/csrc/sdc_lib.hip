@@ -404,7 +404,7 @@ Xfer<T> make_xfer(const sdc_handle_s* h, int l) {
 // deriv: geometry for a derivative kernel (dA/dsigma products), whose Robin term follows
 // sdc_params.deriv_robin
 Geo make_geo(const sdc_handle_s* h, bool deriv = false) {
-  Geo g;
+  Geo g{};
   g.hx = h->hx; g.hy = h->hy; g.hz = h->hz;
   g.bxm = h->bxm; g.bxp = h->bxp; g.bym = h->bym; g.byp = h->byp; g.bzm = h->bzm; g.bzp = h->bzp;
   g.robin = h->bc == SDC_BC_ROBIN && (!deriv || h->prm.deriv_robin != 0);
@@ -943,10 +943,10 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
     CK((vcycle<T, S, S, S>(h, l + 1, cb, cx, ncol, false, done, st)));
     prof_begin(h, l == 0 ? SDC_PROF_PROLONG : -1, st);
     if (t.wx) {
-      DISPATCH_R(RX, launch(h, (k_prolong_add<T, S, R, true>), grid2(lv.nN, ncol, RX), blk, 0,
+      DISPATCH_R(RX, launch(h, (k_prolong_own<T, S, R, true>), grid2(cl.nN, ncol, RX), blk, 0,
                             st, t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
     } else {
-      DISPATCH_R(RX, launch(h, (k_prolong_add<T, S, R, false>), grid2(lv.nN, ncol, RX), blk, 0,
+      DISPATCH_R(RX, launch(h, (k_prolong_own<T, S, R, false>), grid2(cl.nN, ncol, RX), blk, 0,
                             st, t, (const S*)cx, cur, lv.ld, cl.ld, ncol, done));
     }
     prof_end(h, l == 0 ? SDC_PROF_PROLONG : -1, st, (2 * nodes + (double)cl.nN) * cols * sS);
