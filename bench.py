@@ -634,7 +634,8 @@ def main():
         log(f"[bench] layered step {out['layered_160x160x96_256src']}")
         b = batch_result(args)
         out["uneven_batch_48"] = {k: b[k] for k in ("value", "unit", "config", "fields_ms",
-                                                    "jtvec_ms", "concurrency_speedup")}
+                                                    "jtvec_ms", "same_mesh_groups",
+                                                    "concurrency_speedup")}
         log(f"[bench] uneven batch {out['uneven_batch_48']}")
 
     if not args.no_cpu:

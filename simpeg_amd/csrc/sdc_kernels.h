@@ -759,9 +759,10 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* const* xp, const T*
 // q = A z + beta q, so the stencil reads z (the preconditioner output, possibly bf16) instead of
 // p, and p, q are updated pointwise.  The x update of the previous step, x += alpha_prev p_old,
 // rides along (it needs p_old, which is read here anyway); partial sums of p.q per column.
-// FIRST: p = z, q = A z, no x update.  ctl[XFRESH]: x was folded away and counts as zero (not
-// read).  check_stop: return at once when ctl[STOP] is set (only needed when several iterations
-// run between host reads of the stop flag).
+// FIRST: p = z, q = A z, no x update.  ctl[XFRESH]: x was folded away and counts as zero (it
+// is still loaded, unconditionally, and masked afterwards, so that no operand load waits for
+// the flag's device-scope read).  check_stop: return at once when ctl[STOP] is set (only needed
+// when several iterations run between host reads of the stop flag).
 template <typename Z, int R, bool FIRST, int G = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z* __restrict__ z,
                                                         float* __restrict__ p,
@@ -774,7 +775,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z*
                                                         const int* __restrict__ ctl,
                                                         int check_stop) {
   if (check_stop && ld_flag(ctl + SDC_CTL_STOP)) return;
-  const int xfresh = ld_flag(ctl + SDC_CTL_XFRESH);
+  const int xfresh = FIRST ? 1 : ld_flag(ctl + SDC_CTL_XFRESH);
   SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
   double acc[R * G];
 #pragma unroll
@@ -793,10 +794,11 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z*
           const size_t o = (size_t)(cg + r < ncol ? cg + r : ncol - 1) * ld + n;
           po[r] = ld_once(p + o);
           qo[r] = ld_once(q + o);
-          xo[r] = xfresh ? 0.0f : ld_once(x + o);
+          xo[r] = ld_once(x + o);
         }
       }
       stencil_cols<float, R>(c, z, n, ld, cg, ncol, zc, az);
+      const uint32_t xkeep = xfresh ? 0u : 0xffffffffu;  // bit mask, not a select of the load
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (cg + r < ncol) {
@@ -806,7 +808,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z*
             const float be = (float)beta[cg + r];
             pn += be * po[r];
             qn += be * qo[r];
-            st_once(x + o, xo[r] + (float)alpha_prev[cg + r] * po[r]);
+            const float xv = __uint_as_float(__float_as_uint(xo[r]) & xkeep);
+            st_once(x + o, xv + (float)alpha_prev[cg + r] * po[r]);
           }
           st_once(p + o, pn);
           st_once(q + o, qn);
@@ -1209,8 +1212,10 @@ struct PinFix {
 // right-hand side of point sources touches a few nodes, so the fp64 residual then streams u.
 // ctl non-null: runs only when the control block asks for a reliable update, and q, u are read
 // from qu[0], qu[1] (device pointer table) instead of the arguments, so that a recorded
-// iteration serves every solve.
-template <typename TO, int R, bool U_ZERO, int G = 1>
+// iteration serves every solve.  CH: a work-group covers CH consecutive blocks of SDC_BLOCK
+// nodes (one partial per work-group and column), so that a launch that returns at once
+// dispatches fewer work-groups.
+template <typename TO, int R, bool U_ZERO, int G = 1, int CH = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> sa, const double* q,
                                                            const double* u,
                                                            TO* __restrict__ rout, size_t ld,
@@ -1225,7 +1230,12 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> sa, cons
     q = qu[0];
     u = qu[1];
   }
-  SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
+  const uint32_t nbx = gridDim.x;
+  const uint32_t lin0_ = blockIdx.x + nbx * blockIdx.y;
+  const uint32_t lin_ = sa.remap ? xcd_remap(lin0_, nbx * gridDim.y) : lin0_;
+  const uint32_t by_ = lin_ / nbx;
+  const uint32_t bx = lin_ - by_ * nbx;
+  const int c0 = (int)by_ * (R * G);
   // the operator (and the node-0 edges of the Neumann shift) of column cg's set
   auto select = [&](int cg, PinFix& pf) {
     pf = pf0;
@@ -1240,7 +1250,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> sa, cons
   double acc[R * G];
 #pragma unroll
   for (int r = 0; r < R * G; ++r) acc[r] = 0.0;
-  if (live) {
+  for (int ch = 0; ch < CH; ++ch) {
+    const uint32_t n = (bx * CH + ch) * SDC_BLOCK + threadIdx.x;
+    if (n >= sa.nN) break;
     PinFix pf;
     Sten<double> s = select(c0, pf);
     NodeC<double> c;
@@ -1276,7 +1288,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> sa, cons
           double rv = qv[r];
           anyq |= rv != 0.0;
           if (!U_ZERO) rv -= au[r];
-          acc[gq * R + r] = rv * rv;
+          acc[gq * R + r] += rv * rv;
           if (wfix != 0.0) {
             const double r0 = U_ZERO ? q[o] : q[o] - u[o];
             rv += wfix * r0;

@@ -1115,6 +1115,8 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const dim3 g = grid2(l0.nN, ncol, R);
   const dim3 blk(SDC_BLOCK);
   const uint32_t nbx = nblk(l0.nN);
+  constexpr int kIrChunks = 4;
+  const uint32_t nbx_ir = (nbx + kIrChunks - 1) / kIrChunks;
   const size_t len = l0.ld * (size_t)ncol;
   PinFix pf;
   pf.on = h->bc == SDC_BC_NEUMANN;
@@ -1250,15 +1252,16 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
            (const double*)W.alpha, l0.ld, l0.nN, (const int*)W.ctl);
     prof_end(h, upd ? SDC_PROF_FOLD : -1, s, (double)l0.nN * ncol * (fold_fresh ? 16 : 24));
     prof_begin(h, upd ? SDC_PROF_REFINE : -1, s);
-    DISPATCH_RG(R64, G_, launch(h, (k_ir_residual<float, R, false, G>),
-                                grid2(l0.nN, ncol, R * G), blk, 0, s, s2,
-                                (const double*)nullptr, (const double*)nullptr, W.r, l0.ld,
-                                ncol, pf, W.partial, (unsigned*)W.qmask, r16, (const int*)W.ctl,
-                                (const double* const*)W.qu));
+    // kIrChunks node blocks per work-group: most launches return at once (no update)
+    DISPATCH_RG(R64, G_, launch(h, (k_ir_residual<float, R, false, G, kIrChunks>),
+                                dim3(nbx_ir, (uint32_t)((ncol + R * G - 1) / (R * G))), blk, 0,
+                                s, s2, (const double*)nullptr, (const double*)nullptr, W.r,
+                                l0.ld, ncol, pf, W.partial, (unsigned*)W.qmask, r16,
+                                (const int*)W.ctl, (const double* const*)W.qu));
     // algorithmic bytes: fp64 coefficients, per column u64 read and r32 write; q is read on
     // source nodes only (not counted)
     prof_end(h, upd ? SDC_PROF_REFINE : -1, s, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
-    launch(h, k_fin_true, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx,
+    launch(h, k_fin_true, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx_ir,
            (const double*)W.aux, tol2, W.orr, W.rmax, W.active, (const int*)W.ctl);
     launch(h, k_after_update, dim3(1), blk, 0, s, W.ctl, (const double*)W.orr,
            (const double*)W.aux, (const int*)W.active, W.alpha, ncol);
