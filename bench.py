@@ -166,9 +166,9 @@ def kernel_names(params, ncol):
         "update": f"k_update_r4<{'true' if params.vcycle_bf16 in (3, 4) else 'false'}>",
         "smooth": f"k_cheb<float,{b},{v},{z},{r},1,true>",
         "apply": f"k_apply_pq<{z},{r},false>", "pair": f"k_smooth_pair<float,{b},{v},{r}>",
-        "residual": f"k_residual<float,{b},{v},{r}>",
+        "residual": f"k_residual_rx<float,{b},{v},{r},*>",
         "smooth0": f"k_cheb<float,{b},{v},{v},{r},0,false>",
-        "prolong": f"k_prolong_own<float,{v},{rx}>", "restrict": f"k_restrict_axis<float,{v},*,{rx}>",
+        "prolong": f"k_prolong_own<float,{v},{rx}>", "restrict": f"k_restrict_axis<float,{v},1|2,{rx}>",
         "update_p": "k_update_p4<float,false>", "refine": f"k_ir_residual<float,{rr},false>",
         "fold": "k_ir_fold_ctl", "jtvec": "k_jtvec<0>",
     }

@@ -497,6 +497,78 @@ struct AxisXfer {
   uint32_t wstride;
 };
 
+// r = b - A x and its restriction along x in one pass: out = Px^T r on the grid (coarse x, fine
+// y, fine z), the first pass of the separable P^T (k_restrict_axis<..., 0, ...>), without
+// storing r.  A work-group computes r at SDC_BLOCK consecutive fine nodes starting one node
+// before its tile of SDC_BLOCK - 2 nodes and keeps it in LDS, rounded to X as a stored residual
+// would be; every node of the tile that is a coarse x position then writes
+// wl r(n-1) + r(n) + wh r(n+1) with the x-restriction weights of k_restrict_axis (OPD: from
+// wop over the fine grid, masked by the geometric ones).  A node's x neighbours n-1, n+1 lie in
+// the same grid row wherever the weight is non-zero.  Results equal the two-pass form's (up to
+// the sign of zero terms).  ax: the x pass's AxisXfer; fi / pix: the x tables (coarse -> fine
+// index, fine -> left coarse node).
+template <typename T, typename B, typename X, int R, bool OPD>
+__global__ __launch_bounds__(SDC_BLOCK) void k_residual_rx(Sten<T> sa, const B* __restrict__ b,
+                                                           const X* __restrict__ x,
+                                                           X* __restrict__ out, size_t ld,
+                                                           size_t ldo, int ncol, AxisXfer ax,
+                                                           const int* __restrict__ fi,
+                                                           const int* __restrict__ pix,
+                                                           const T* __restrict__ wlo,
+                                                           const T* __restrict__ whi,
+                                                           const T* __restrict__ wop,
+                                                           const int* __restrict__ done) {
+  if (done && ld_flag(done)) return;
+  constexpr uint32_t TS = SDC_BLOCK - 2;
+  __shared__ T rl[R][SDC_BLOCK];
+  const uint32_t nbx = gridDim.x;
+  const uint32_t lin0 = blockIdx.x + nbx * blockIdx.y;
+  const uint32_t lin = sa.remap ? xcd_remap(lin0, nbx * gridDim.y) : lin0;
+  const uint32_t by = lin / nbx;
+  const uint32_t bx = lin - by * nbx;
+  const int c0 = (int)by * R;
+  const int64_t nn = (int64_t)bx * TS - 1 + threadIdx.x;
+  const bool live = nn >= 0 && nn < (int64_t)sa.nN;
+  const uint32_t n = live ? (uint32_t)nn : 0u;
+  T v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = T(0);
+  if (live) {
+    const NodeC<T> c = node_coef(sten_col(sa, c0), n);
+    T xc[R], av[R], bv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) bv[r] = (T)ld_once(col_ptr(b, ld, c0, r, ncol) + n);
+    stencil_cols<T, R>(c, x, n, ld, c0, ncol, xc, av);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = (T)(X)(bv[r] - av[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) rl[r][threadIdx.x] = v[r];
+  __syncthreads();
+  if (!live || threadIdx.x == 0 || threadIdx.x == SDC_BLOCK - 1) return;
+  // fine x index i of n and the rest of its index (j + n1y k); n restricts to coarse x node I
+  // when it is one's fine position
+  const uint32_t rest = fdiv(n, sa.fx);  // division by the fine row length (= ax.a0)
+  const uint32_t i = n - rest * sa.n1x;
+  const int I0 = pix[i];
+  const int I = fi[I0] == (int)i ? I0 : (I0 + 1 < (int)ax.b0 && fi[I0 + 1] == (int)i) ? I0 + 1 : -1;
+  if (I < 0) return;
+  T wl = wlo[I], wh = whi[I];
+  if (OPD) {
+    const T* w = ax.cset ? wop + (size_t)ax.cset[c0] * ax.wstride : wop;
+    const uint32_t il = i > 0 ? n - 1 : n, ih = i + 1 < ax.a0 ? n + 1 : n;
+    wl = w[il] * (T)(wl != T(0));
+    wh = (T(1) - w[ih]) * (T)(wh != T(0));
+  }
+  const size_t o = (size_t)I + (size_t)ax.b0 * rest;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (c0 + r < ncol)
+      st_once(out + (size_t)(c0 + r) * ldo + o,
+              (X)(wl * rl[r][t - 1] + rl[r][t] + wh * rl[r][t + 1]));
+}
+
 // OPD: operator-dependent weights wop (else the geometric wlo / whi alone).  The weight loads
 // are unconditional (clamped indices) and the missing neighbours are masked by multiplying, so
 // they go out with the value loads in the second round trip after the fi lookup.

@@ -2,6 +2,11 @@
 // refinement drivers and the C ABI declared in include/simpeg_dc.h.
 #include "sdc_kernels.h"
 #include "sdc_tail.h"
+
+// 1: the residual and the x pass of the separable restriction run as one kernel
+#ifndef SDC_FUSE_RX
+#define SDC_FUSE_RX 1
+#endif
 #include "simpeg_dc.h"
 
 #include <algorithm>
@@ -882,11 +887,17 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
       std::swap(cur, oth);
     }
   };
+  // the separable restriction starting along x takes the residual into its first pass
+  // (k_residual_rx, one work-item per node and R columns; no column loops there)
+  const bool fuse_rx = SDC_FUSE_RX && h->prm.xfer_separable && cl.n1x != lv.n1x && G_ == 1;
   for (int gam = 0; gam < ngam; ++gam) {
-    prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
-    DISPATCH_RG(R, G_, launch(h, (k_residual<T, B, S, R, G>), grid2(lv.nN, ncol, R * G), blk,
-                              0, st, s, b, (const S*)cur, res, lv.ld, ncol, done));
-    prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st, nodes * 4 * sT + nodes * cols * (sB + 2 * sS));
+    if (!fuse_rx) {
+      prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
+      DISPATCH_RG(R, G_, launch(h, (k_residual<T, B, S, R, G>), grid2(lv.nN, ncol, R * G), blk,
+                                0, st, s, b, (const S*)cur, res, lv.ld, ncol, done));
+      prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st,
+               nodes * 4 * sT + nodes * cols * (sB + 2 * sS));
+    }
     const int rtag = l == 0 ? SDC_PROF_RESTRICT : -1;
     if (h->prm.xfer_separable) {
       // P^T one axis at a time; the intermediates ping-pong between res and the smoother buffer
@@ -916,6 +927,24 @@ int vcycle(sdc_handle_s* h, int l, const B* b, O* x, int ncol, bool dot_out, con
         const T* wl = a == 0 ? t.rlx : a == 1 ? t.rly : t.rlz;
         const T* wh = a == 0 ? t.rhx : a == 1 ? t.rhy : t.rhz;
         const T* wop = a == 0 ? t.wx : a == 1 ? t.wy : t.wz;
+        if (p == 0 && fuse_rx) {
+          // residual + x pass: R columns per work-item, tiles of SDC_BLOCK - 2 nodes
+          const dim3 gr((lv.nN + SDC_BLOCK - 3) / (SDC_BLOCK - 2), (uint32_t)((ncol + R - 1) / R));
+          prof_begin(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st);
+          if (wop) {
+            DISPATCH_R(R, launch(h, (k_residual_rx<T, B, S, R, true>), gr, blk, 0, st, s, b,
+                                 (const S*)cur, dst, lv.ld, ldo, ncol, ax, fi, t.pix, wl, wh,
+                                 wop, done));
+          } else {
+            DISPATCH_R(R, launch(h, (k_residual_rx<T, B, S, R, false>), gr, blk, 0, st, s, b,
+                                 (const S*)cur, dst, lv.ld, ldo, ncol, ax, fi, t.pix, wl, wh,
+                                 wop, done));
+          }
+          prof_end(h, l == 0 ? SDC_PROF_RESIDUAL : -1, st,
+                   nodes * 4 * sT + nodes * cols * (sB + sS) + (double)ax.nout * cols * sS);
+          src = dst;
+          continue;
+        }
         const dim3 ga = grid2(ax.nout, ncol, RX);
         prof_begin(h, rtag, st);
 #define SDC_RESTRICT_AXIS(AXN, OPD)                                                          \
