@@ -80,6 +80,9 @@ class Simulation3DNodal(_DCNodal):
 
     @model.setter
     def model(self, value):
+        self._assign_model(value, copy=True)
+
+    def _assign_model(self, value, copy):
         if value is None:
             self._model = None
             return
@@ -91,7 +94,8 @@ class Simulation3DNodal(_DCNodal):
                 self._model, value):
             self._Jmatrix = None
             self._gtgdiag = None
-        self._model = value.copy()
+            self._model_version += 1
+        self._model = value.copy() if copy else value
 
     def _dc_fields(self):
         if self._f is None:

@@ -161,6 +161,10 @@ class Projection(IdentityMap):
         self.index = np.atleast_1d(np.arange(nP)[index] if isinstance(index, slice) else index)
         # each model entry selected at most once: P^T v is a scatter into a zero vector
         self.unique_index = bool(np.unique(self.index).size == self.index.size)
+        # a contiguous ascending range is copied as one block instead of gathered
+        i = self.index
+        self._range = (slice(int(i[0]), int(i[0]) + i.size)
+                       if i.size and np.array_equal(i, np.arange(i[0], i[0] + i.size)) else None)
         self._P = sp.csr_matrix((np.ones(self.index.size), (np.arange(self.index.size), self.index)),
                                 shape=(self.index.size, nP))
 
@@ -176,7 +180,8 @@ class Projection(IdentityMap):
         return True
 
     def _transform(self, m):
-        return np.asarray(m, dtype=float).ravel()[self.index]
+        m = np.asarray(m, dtype=float).ravel()
+        return m[self._range].copy() if self._range is not None else m[self.index]
 
     def deriv(self, m, v=None):
         return self._P if v is None else self._P @ v

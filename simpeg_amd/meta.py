@@ -29,7 +29,9 @@ def sum_adjoint(mappings, model, parts):
     adds its part into its own entries (no full-length temporary per problem)."""
     out = np.zeros(np.asarray(model).size)
     for mp, part in zip(mappings, parts):
-        if isinstance(mp, Projection) and mp.unique_index:
+        if isinstance(mp, Projection) and mp._range is not None:
+            out[mp._range] += part
+        elif isinstance(mp, Projection) and mp.unique_index:
             out[mp.index] += part
         else:
             out += mp.deriv(model).T @ part
@@ -91,13 +93,38 @@ class _SetGroup:
         self.cost = sum(s._nN * n for s, n in zip(self.sims, self.counts))
         self._q = None
         self._tuned = False
+        self._versions = None  # the members' model versions the engine's operators belong to
+        self._m_dev = None     # the members' models on the device (device-evaluated maps)
 
     def _views(self, block):
         return [block[o:o + n] for o, n in zip(self.offs, self.counts)]
 
+    def _device_maps(self):
+        fms = [s._device_map() for s in self.sims]
+        return fms if all(f is not None for f in fms) else None
+
     def _set_models(self):
+        """Operators of the members' current models (rebuilt only when a model changed).  With
+        elementwise maps the models go to the device and sigma is evaluated there; otherwise
+        sigma is evaluated on the host."""
         eng = self.engine
-        eng.set_sigma_sets(np.stack([np.asarray(s.sigma, dtype=float) for s in self.sims]))
+        versions = tuple(s._model_version for s in self.sims)
+        if versions != self._versions:
+            fms = self._device_maps()
+            if fms is not None:
+                m_dev = torch.empty((len(self.sims), eng.nC), dtype=torch.float64,
+                                    device=eng.device)
+                for k, s in enumerate(self.sims):
+                    m_dev[k].copy_(torch.from_numpy(np.ascontiguousarray(s.model, dtype=float)))
+                sig = torch.empty_like(m_dev)
+                for k, f in enumerate(fms):
+                    sig[k] = f[0](m_dev[k])
+                eng.set_sigma_sets(sig)
+                self._m_dev = m_dev
+            else:
+                eng.set_sigma_sets(np.stack([np.asarray(s.sigma, dtype=float) for s in self.sims]))
+                self._m_dev = None
+            self._versions = versions
         want = 1 if self.sims[0]._deriv_includes_bc else 0
         if eng.params.deriv_robin != want:
             eng.set_params(deriv_robin=want)
@@ -156,17 +183,27 @@ class _SetGroup:
         eng = self.engine
         self._set_models()
         rhs = eng.block(self.ncol)
-        for s, vk, out in zip(self.sims, v, self._views(rhs)):
+        for s, vk in zip(self.sims, v):
             if np.size(vk) != s.survey.nD:
                 raise ValueError(f"v must have {s.survey.nD} entries, got {np.size(vk)}")
-            eng.scatter(s._projection(eng)["scatter"], np.asarray(vk, dtype=float), out.shape[0],
-                        out=out)
+        # every member's data vector in one upload
+        v_dev = torch.as_tensor(np.concatenate([np.asarray(vk, dtype=float).ravel() for vk in v]))
+        v_dev = v_dev.to(eng.device)
+        o = 0
+        for s, out in zip(self.sims, self._views(rhs)):
+            n = s.survey.nD
+            eng.scatter(s._projection(eng)["scatter"], v_dev[o:o + n], out.shape[0], out=out)
+            o += n
         lam = self._solve(rhs)
-        parts = []
-        for s, fk, lk in zip(self.sims, f, self._views(lam)):
-            g = eng.contract(fk.u_block, lk, lk.shape[0]).cpu().numpy()
-            parts.append(-s._sigma_deriv_T(g))
-        return parts
+        gs = [eng.contract(fk.u_block, lk, lk.shape[0]) for fk, lk in zip(f, self._views(lam))]
+        fms = self._device_maps() if self._m_dev is not None else None
+        if fms is not None:
+            # -sigmaDeriv^T g on the device (diagonal derivative), one download for all members
+            out = torch.cat([-(fm[1](self._m_dev[k]) * g) for k, (fm, g) in enumerate(zip(fms, gs))])
+            flat = out.cpu().numpy()
+            offs = np.cumsum([0] + [g.numel() for g in gs])
+            return [flat[offs[k]:offs[k + 1]] for k in range(len(gs))]
+        return [-s._sigma_deriv_T(g.cpu().numpy()) for s, g in zip(self.sims, gs)]
 
 
 class MetaSimulation:
@@ -183,6 +220,7 @@ class MetaSimulation:
         self.mappings = list(mappings)
         self.n_streams = max(1, int(n_streams))
         self._model = None
+        self._model_bufs = []
         self._streams = None
         vnD = [sim.survey.nD for sim in self.simulations]
         self._data_offsets = np.cumsum(np.r_[0, vnD])
@@ -209,9 +247,35 @@ class MetaSimulation:
 
     @model.setter
     def model(self, value):
-        self._model = None if value is None else np.asarray(value, dtype=float).ravel()
+        prev = self._model
+        new = None if value is None else np.asarray(value, dtype=float).ravel()
+        if (prev is not None and new is not None and prev.shape == new.shape
+                and np.array_equal(prev[:4096], new[:4096]) and np.array_equal(prev, new)):
+            return  # the same model: every simulation keeps its own
+        # one private, read-only copy of the full model; a simulation behind a contiguous
+        # Projection keeps a view of its block of it instead of a copy of its own.  Two buffers
+        # alternate (the simulations compare the new model with the views they hold of the
+        # previous one), so that a new model is copied into memory already paged in.
+        if new is None:
+            self._model = None
+        else:
+            buf = next((b for b in self._model_bufs if b is not prev and b.shape == new.shape),
+                       None)
+            if buf is None:
+                buf = np.empty_like(new)
+                self._model_bufs = [b for b in self._model_bufs if b is prev][:1] + [buf]
+            buf.flags.writeable = True
+            np.copyto(buf, new)
+            buf.flags.writeable = False
+            self._model = buf
         for mp, sim in zip(self.mappings, self.simulations):
-            sim.model = None if value is None else mp * self._model
+            if value is None:
+                sim.model = None
+            elif (isinstance(mp, Projection) and mp._range is not None
+                    and hasattr(sim, "_assign_model")):
+                sim._assign_model(self._model[mp._range], copy=False)
+            else:
+                sim.model = mp * self._model
 
     def _map(self, fn):
         """fn(unit) for every work unit, concurrently on separate streams (longest first);

@@ -122,6 +122,7 @@ class Simulation3DNodal:
         self.exact_bc_deriv = bool(exact_bc_deriv)
         self._tuned = False
         self._model = None
+        self._model_version = 0  # counts model changes (operators built elsewhere check it)
         self._sigma_on_gpu = None
         self._Jmatrix = None
         self._gtgdiag = None
@@ -192,6 +193,11 @@ class Simulation3DNodal:
 
     @model.setter
     def model(self, value):
+        self._assign_model(value, copy=True)
+
+    def _assign_model(self, value, copy):
+        """The model setter; copy=False keeps ``value`` itself (MetaSimulation hands over
+        read-only views of its own private copy of the full model)."""
         if value is None:
             self._model = None
             return
@@ -202,12 +208,20 @@ class Simulation3DNodal:
             if nP != "*" and value.size != nP:
                 raise ValueError(
                     f"Model is the wrong size: {value.size}, expected {nP} parameters")
-        if self._model is None or self._model.shape != value.shape or not np.allclose(
-                self._model, value):
+        prev = self._model
+        changed = prev is None or prev.shape != value.shape
+        if not changed:
+            # a new model usually differs in its first entries already: the prefix settles it
+            # without a full pass (same outcome as np.allclose over all entries)
+            k = min(prev.size, 4096)
+            changed = not np.allclose(prev[:k], value[:k]) or not (
+                np.array_equal(prev, value) or np.allclose(prev, value))
+        if changed:
             self._sigma_on_gpu = None
             self._Jmatrix = None
             self._gtgdiag = None
-        self._model = value.copy()
+            self._model_version += 1
+        self._model = value.copy() if copy else value
 
     @property
     def sigma(self):
