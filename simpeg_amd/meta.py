@@ -16,6 +16,7 @@ releases the GIL), largest first.
 from __future__ import annotations
 
 import concurrent.futures as cf
+import warnings
 
 import numpy as np
 import torch
@@ -115,7 +116,11 @@ class _SetGroup:
                 m_dev = torch.empty((len(self.sims), eng.nC), dtype=torch.float64,
                                     device=eng.device)
                 for k, s in enumerate(self.sims):
-                    m_dev[k].copy_(torch.from_numpy(np.ascontiguousarray(s.model, dtype=float)))
+                    mk = np.ascontiguousarray(s.model, dtype=float)
+                    with warnings.catch_warnings():
+                        # a read-only view of the meta model: torch only reads it here
+                        warnings.simplefilter("ignore", UserWarning)
+                        m_dev[k].copy_(torch.from_numpy(mk))
                 sig = torch.empty_like(m_dev)
                 for k, f in enumerate(fms):
                     sig[k] = f[0](m_dev[k])
