@@ -170,7 +170,7 @@ def kernel_names(params, ncol):
         "smooth0": f"k_cheb<float,{b},{v},{v},{r},0,false>",
         "prolong": f"k_prolong_own<float,{v},{rx}>", "restrict": f"k_restrict_axis<float,{v},1|2,{rx}>",
         "update_p": "k_update_p4<float,false>", "refine": f"k_ir_residual<float,{rr},false>",
-        "fold": "k_ir_fold_ctl", "jtvec": "k_jtvec<0>",
+        "fold": "k_ir_fold_ring", "jtvec": "k_jtvec<0>",
     }
 
 

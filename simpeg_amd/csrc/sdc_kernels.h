@@ -40,15 +40,17 @@ __device__ __forceinline__ bool set_changes(const Sten<T>& sa, int c, int cprev)
 
 // Control block of the mixed PCG (device ints), read and written by the kernels so that the
 // iteration runs without a host decision: STOP ends the solve (every kernel of an iteration
-// returns at once), UPD asks for a reliable update in the current iteration, XFRESH / UFRESH
-// mark x as already folded into u / u as not yet written, IT counts iterations, LAST is the
-// iteration of the last update, REASON says why the solve stopped (1 converged, 2 iteration
-// limit, 3 stalled) and OUTER counts reliable updates; MAXIT and WINDOW are the iteration limit
-// and the stall window of the solve.  The solve's parameters live here, not in kernel
-// arguments, so that a recorded iteration serves solves with any tolerance or limit.
+// returns at once), UPD asks for a reliable update in the current iteration, PEND counts the
+// search directions in the ring not yet folded into u, FOLD asks for a fold without a
+// residual replacement (the ring is full), UFRESH marks u as not yet written, IT counts
+// iterations, LAST is the iteration of the last update, REASON says why the solve stopped
+// (1 converged, 2 iteration limit, 3 stalled) and OUTER counts reliable updates; MAXIT and
+// WINDOW are the iteration limit and the stall window of the solve.  The solve's parameters
+// live here, not in kernel arguments, so that a recorded iteration serves solves with any
+// tolerance or limit.
 #define SDC_CTL_STOP 0
 #define SDC_CTL_UPD 1
-#define SDC_CTL_XFRESH 2
+#define SDC_CTL_PEND 2
 #define SDC_CTL_UFRESH 3
 #define SDC_CTL_IT 4
 #define SDC_CTL_LAST 5
@@ -56,6 +58,7 @@ __device__ __forceinline__ bool set_changes(const Sten<T>& sa, int c, int cprev)
 #define SDC_CTL_OUTER 7
 #define SDC_CTL_MAXIT 8
 #define SDC_CTL_WINDOW 9
+#define SDC_CTL_FOLD 10
 #define SDC_CTL_N 16
 // doubles after the ints: the largest true relative residual of the previous update and of the
 // last one, tol^2 and reliable_delta^2
@@ -73,6 +76,13 @@ __device__ __forceinline__ const double* ctl_dbl(const int* ctl) {
 // a reliable update runs in this iteration
 __device__ __forceinline__ bool ctl_update(const int* ctl) {
   return ld_flag(ctl + SDC_CTL_STOP) == 0 && ld_flag(ctl + SDC_CTL_UPD) != 0;
+}
+
+// the pending search directions are folded into u in this iteration (a reliable update, or
+// a full ring)
+__device__ __forceinline__ bool ctl_fold(const int* ctl) {
+  return ld_flag(ctl + SDC_CTL_STOP) == 0 &&
+         (ld_flag(ctl + SDC_CTL_UPD) != 0 || ld_flag(ctl + SDC_CTL_FOLD) != 0);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -829,25 +839,23 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_xr4(T* const* xp, const T*
 
 // The mixed-precision PCG step that replaces "p = z + beta p; q = A p": by linearity
 // q = A z + beta q, so the stencil reads z (the preconditioner output, possibly bf16) instead of
-// p, and p, q are updated pointwise.  The x update of the previous step, x += alpha_prev p_old,
-// rides along (it needs p_old, which is read here anyway); partial sums of p.q per column.
-// FIRST: p = z, q = A z, no x update.  ctl[XFRESH]: x was folded away and counts as zero (it
-// is still loaded, unconditionally, and masked afterwards, so that no operand load waits for
-// the flag's device-scope read).  check_stop: return at once when ctl[STOP] is set (only needed
-// when several iterations run between host reads of the stop flag).
+// p, and p, q are updated pointwise; partial sums of p.q per column.  The search directions
+// live in a ring of slots: the new p goes to p_out, the previous one is read from p_prev, and
+// the solution update x += alpha p is not carried along: the pending directions are folded
+// into u (fp64) by k_ir_fold_ring, so no fp32 iterate is read and written every step.
+// FIRST: p = z, q = A z.  check_stop: return at once when ctl[STOP] is set (only needed when
+// several iterations run between host reads of the stop flag).
 template <typename Z, int R, bool FIRST, int G = 1>
 __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z* __restrict__ z,
-                                                        float* __restrict__ p,
+                                                        const float* __restrict__ p_prev,
+                                                        float* __restrict__ p_out,
                                                         float* __restrict__ q,
-                                                        float* __restrict__ x,
                                                         const double* __restrict__ beta,
-                                                        const double* __restrict__ alpha_prev,
                                                         size_t ld, int ncol,
                                                         double* __restrict__ partial,
                                                         const int* __restrict__ ctl,
                                                         int check_stop) {
   if (check_stop && ld_flag(ctl + SDC_CTL_STOP)) return;
-  const int xfresh = FIRST ? 1 : ld_flag(ctl + SDC_CTL_XFRESH);
   SDC_NODE_PROLOGUE_C(sa.nN, sa.remap, R * G)
   double acc[R * G];
 #pragma unroll
@@ -859,18 +867,16 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z*
       const int cg = c0 + gq * R;
       if (cg >= ncol) break;
       if (G > 1 && gq > 0 && set_changes(sa, cg, cg - R)) c = node_coef(sten_col(sa, cg), n);
-      float zc[R], az[R], po[R], qo[R], xo[R];
+      float zc[R], az[R], po[R], qo[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (!FIRST) {
           const size_t o = (size_t)(cg + r < ncol ? cg + r : ncol - 1) * ld + n;
-          po[r] = ld_once(p + o);
+          po[r] = ld_once(p_prev + o);
           qo[r] = ld_once(q + o);
-          xo[r] = ld_once(x + o);
         }
       }
       stencil_cols<float, R>(c, z, n, ld, cg, ncol, zc, az);
-      const uint32_t xkeep = xfresh ? 0u : 0xffffffffu;  // bit mask, not a select of the load
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (cg + r < ncol) {
@@ -880,10 +886,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z*
             const float be = (float)beta[cg + r];
             pn += be * po[r];
             qn += be * qo[r];
-            const float xv = __uint_as_float(__float_as_uint(xo[r]) & xkeep);
-            st_once(x + o, xv + (float)alpha_prev[cg + r] * po[r]);
           }
-          st_once(p + o, pn);
+          st_once(p_out + o, pn);
           st_once(q + o, qn);
           acc[gq * R + r] += (double)pn * (double)qn;
         }
@@ -938,65 +942,81 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
 
 #define SDC_FOLD_CHUNKS 4
 
-// u += x + alpha p: the fp32 partial solution, including the pending x update of the last step,
-// folded into the fp64 solution.  Runs only when the control block asks for a reliable update;
-// x is not cleared (XFRESH then tells the next k_apply_pq to treat it as zero, and XFRESH set
-// before the fold means x holds nothing yet and is not read).  UFRESH: u holds nothing yet (first
-// fold of a solve): u = x + alpha p without reading u, and the column padding [nN, ld) is zeroed,
-// so the solve needs no separate clearing pass.  u comes from the pointer table (u = qu[1]); the
-// grid covers ld entries per column, 4 consecutive nodes per work-item.
-__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_ctl(const double* const* qu,
-                                                           const float* __restrict__ x,
-                                                           const float* __restrict__ p,
-                                                           const double* __restrict__ alpha,
-                                                           size_t ld, uint32_t nN,
-                                                           const int* __restrict__ ctl) {
-  // most iterations make no update: this kernel then only reads the flag, so a work-group
-  // covers SDC_FOLD_CHUNKS consecutive chunks of SDC_NODES4 nodes (a smaller grid to dispatch)
-  if (!ctl_update(ctl)) return;
+// u += sum of alpha_j p_j over the pending search directions of the ring (oldest first, in
+// fp64), ending at slot phi: ctl[PEND] slots phi - PEND + 1 .. phi (mod nslot), slot k at
+// ptab[k] with its per-column alpha at ahist[k * ncol + c].  Runs only when the control block
+// asks for a fold (reliable update or full ring).  UFRESH: u holds nothing yet (first fold of a
+// solve): u = the sum without reading u, and the column padding [nN, ld) is zeroed, so the
+// solve needs no separate clearing pass.  u comes from the pointer table (u = qu[1]); a
+// work-group covers SDC_FOLD_CHUNKS chunks of SDC_NODES4 nodes (most launches only read the
+// flags and return).
+__global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_ring(const double* const* qu,
+                                                            const float* const* ptab,
+                                                            const double* __restrict__ ahist,
+                                                            int nslot, int phi, int ncol,
+                                                            size_t ld, uint32_t nN,
+                                                            const int* __restrict__ ctl) {
+  if (!ctl_fold(ctl)) return;
   double* __restrict__ u = const_cast<double*>(qu[1]);
-  const bool ufresh = ld_flag(ctl + SDC_CTL_UFRESH) != 0,
-             xfresh = ld_flag(ctl + SDC_CTL_XFRESH) != 0;
+  const bool ufresh = ld_flag(ctl + SDC_CTL_UFRESH) != 0;
+  const int pend = ld_flag(ctl + SDC_CTL_PEND);
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
-  const float al = (float)alpha[c];
   const uint32_t end = ufresh ? (uint32_t)ld : nN;
 #pragma unroll
   for (int ch = 0; ch < SDC_FOLD_CHUNKS; ++ch) {
     const uint32_t n0 =
         ((blockIdx.x * SDC_FOLD_CHUNKS + ch) * SDC_BLOCK + threadIdx.x) * 4;
     if (n0 + 3 < nN) {
-      const vec4_t<float> pv = *(const vec4_t<float>*)(p + o + n0);
-      const vec4_t<float> xv =
-          (xfresh ? vec4_t<float>(0.0f) : *(const vec4_t<float>*)(x + o + n0)) + al * pv;
       vec4_t<double> uv = ufresh ? vec4_t<double>(0.0) : *(const vec4_t<double>*)(u + o + n0);
-      uv.x += (double)xv.x;
-      uv.y += (double)xv.y;
-      uv.z += (double)xv.z;
-      uv.w += (double)xv.w;
+      for (int j = pend - 1; j >= 0; --j) {
+        const int k = (phi - j + nslot) % nslot;
+        const double al = ahist[(size_t)k * ncol + c];
+        const vec4_t<float> pv = *(const vec4_t<float>*)(ptab[k] + o + n0);
+        uv.x += al * (double)pv.x;
+        uv.y += al * (double)pv.y;
+        uv.z += al * (double)pv.z;
+        uv.w += al * (double)pv.w;
+      }
       *(vec4_t<double>*)(u + o + n0) = uv;
     } else {
       for (uint32_t n = n0; n < end && n < n0 + 4; ++n) {
-        if (n < nN)
-          u[o + n] = (ufresh ? 0.0 : u[o + n]) + (double)((xfresh ? 0.0f : x[o + n]) + al * p[o + n]);
-        else
-          u[o + n] = 0.0;
+        double v = 0.0;
+        if (n < nN) {
+          v = ufresh ? 0.0 : u[o + n];
+          for (int j = pend - 1; j >= 0; --j) {
+            const int k = (phi - j + nslot) % nslot;
+            v += ahist[(size_t)k * ncol + c] * (double)ptab[k][o + n];
+          }
+        }
+        u[o + n] = v;
       }
     }
   }
 }
 
+// After the fold (one work-item): the ring is empty and u is written.
+__global__ void k_after_fold(int* __restrict__ ctl) {
+  if (threadIdx.x == 0 && ctl_fold(ctl)) {
+    ctl[SDC_CTL_PEND] = 0;
+    ctl[SDC_CTL_FOLD] = 0;
+    ctl[SDC_CTL_UFRESH] = 0;
+  }
+}
+
 // Reliable-update decision after an iteration of the mixed PCG (one work-group): counts the
-// iteration and asks for an update (replace the fp32 residual by the fp64 true residual, fold x
-// into u) when every active column's fp32 residual |r|^2 passed tol^2 |q|^2, some active column's
-// dropped by delta^2 since its last replacement (rmax), the iteration limit is reached, or
-// `window` iterations passed without an update.  From the second iteration on, x holds the
-// lagged update of k_apply_pq, so it is no longer fresh.
+// iteration and its search direction (now pending in the ring) and asks for an update (replace
+// the fp32 residual by the fp64 true residual, fold the pending directions into u) when every
+// active column's fp32 residual |r|^2 passed tol^2 |q|^2, some active column's dropped by
+// delta^2 since its last replacement (rmax), the iteration limit is reached, or `window`
+// iterations passed without an update.  Otherwise, when all nslot slots of the ring are
+// pending, it asks for a fold alone (the next iteration writes the oldest slot).
 __global__ __launch_bounds__(SDC_BLOCK) void k_decide(int* __restrict__ ctl,
                                                       const double* __restrict__ rr,
                                                       const double* __restrict__ qn2,
                                                       const double* __restrict__ rmax,
-                                                      const int* __restrict__ active, int ncol) {
+                                                      const int* __restrict__ active, int ncol,
+                                                      int nslot) {
   if (ld_flag(ctl + SDC_CTL_STOP)) return;
   const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
   const double delta2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_DELTA2);
@@ -1012,21 +1032,23 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_decide(int* __restrict__ ctl,
   if (threadIdx.x == 0) {
     const int it = ld_flag(ctl + SDC_CTL_IT) + 1;
     ctl[SDC_CTL_IT] = it;
-    if (it > 1) ctl[SDC_CTL_XFRESH] = 0;
-    ctl[SDC_CTL_UPD] =
+    const int pend = ld_flag(ctl + SDC_CTL_PEND) + 1;
+    ctl[SDC_CTL_PEND] = pend;
+    const int upd =
         (cand || drop || it >= maxit || it - ld_flag(ctl + SDC_CTL_LAST) >= window) ? 1 : 0;
+    ctl[SDC_CTL_UPD] = upd;
+    ctl[SDC_CTL_FOLD] = (!upd && pend >= nslot) ? 1 : 0;
   }
 }
 
-// After a reliable update (one work-group): the lagged x update is cleared with alpha, the
-// solve stops when no column is open (converged), at the iteration limit, or when the largest
-// true relative residual failed to halve since the previous update (the fp32 recursion stalled;
-// the caller continues in fp64).  rmax and active were set by k_fin_true.
+// After a reliable update (one work-group): the solve stops when no column is open
+// (converged), at the iteration limit, or when the largest true relative residual failed to
+// halve since the previous update (the fp32 recursion stalled; the caller continues in fp64).
+// rmax and active were set by k_fin_true.
 __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ctl,
                                                             const double* __restrict__ orr,
                                                             const double* __restrict__ qn2,
                                                             const int* __restrict__ active,
-                                                            double* __restrict__ alpha,
                                                             int ncol) {
   if (!ctl_update(ctl)) return;
   const int maxit = ld_flag(ctl + SDC_CTL_MAXIT);
@@ -1035,7 +1057,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ct
   double m = 0.0;
   int nact = 0;
   for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
-    alpha[c] = 0.0;
     nact += active[c] != 0;
     if (qn2[c] > 0.0) m = fmax(m, sqrt(orr[c] / qn2[c]));
   }
@@ -1058,8 +1079,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ct
     const int it = ld_flag(ctl + SDC_CTL_IT);
     ctl[SDC_CTL_OUTER] = ld_flag(ctl + SDC_CTL_OUTER) + 1;
     ctl[SDC_CTL_UPD] = 0;
-    ctl[SDC_CTL_XFRESH] = 1;
-    ctl[SDC_CTL_UFRESH] = 0;
     dctl[SDC_DCTL_MREL] = m;
     int reason = 0;
     if (nact == 0)
@@ -1085,7 +1104,6 @@ __global__ void k_ctl_init(int* __restrict__ ctl, const double** __restrict__ qu
                            int window) {
   if (threadIdx.x == 0) {
     for (int i = 0; i < SDC_CTL_N; ++i) ctl[i] = 0;
-    ctl[SDC_CTL_XFRESH] = 1;
     ctl[SDC_CTL_UFRESH] = 1;
     ctl[SDC_CTL_MAXIT] = maxit;
     ctl[SDC_CTL_WINDOW] = window;
@@ -1180,11 +1198,16 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_alpha(const double* __restric
                                                          const double* __restrict__ rz,
                                                          double* __restrict__ alpha,
                                                          const int* __restrict__ active,
-                                                         const int* __restrict__ done) {
+                                                         const int* __restrict__ done,
+                                                         double* __restrict__ ahist) {
   if (done && ld_flag(done)) return;
   const int c = blockIdx.x;
   const double pq = block_sum(partial + (size_t)c * nbx, nbx);
-  if (threadIdx.x == 0) alpha[c] = (active[c] && pq > 0.0) ? rz[c] / pq : 0.0;
+  if (threadIdx.x == 0) {
+    const double a = (active[c] && pq > 0.0) ? rz[c] / pq : 0.0;
+    alpha[c] = a;
+    if (ahist) ahist[c] = a;  // the ring slot's alpha (mixed PCG)
+  }
 }
 
 // convergence test |r| <= tol |b| per column (tol^2 from the control block ctl); the last

@@ -7,6 +7,10 @@
 #ifndef SDC_FUSE_RX
 #define SDC_FUSE_RX 1
 #endif
+
+// search directions held by the mixed PCG before they are folded into the solution (a full
+// ring forces a fold)
+constexpr int kRingSlots = 6;
 #include "simpeg_dc.h"
 
 #include <algorithm>
@@ -174,9 +178,17 @@ struct Work {
   const double** qu = nullptr;
   // plain PCG: the solution buffer, read by k_update_xr4 through this device pointer
   T** xtab = nullptr;
+  // mixed PCG: ring of nslot search-direction blocks (slot k at pr + k * ld0 * ncap, device
+  // pointer table ptab) and their per-column alphas (ahist, nslot x ncap)
+  T* pr = nullptr;
+  T** ptab = nullptr;
+  double* ahist = nullptr;
+  int nslot = 0;
   void release() {
     dfree(x); dfree(r); dfree(z); dfree(p); dfree(q); dfree(r16);
     dfree(ctlmem); dfree(rmax); dfree(qu); dfree(xtab);
+    dfree(pr); dfree(ptab); dfree(ahist);
+    nslot = 0;
     ctl = nullptr;
     dctl = nullptr;
     for (auto* v : b) dfree(v);
@@ -592,6 +604,14 @@ int ensure_work(sdc_handle_s* h, int ncol) {
   CK(dmalloc(&w.rmax, cap));
   CK(dmalloc(&w.qu, 2));
   CK(dmalloc(&w.xtab, 1));
+  if (std::is_same<T, float>::value) {
+    w.nslot = kRingSlots;
+    CK(dmalloc(&w.pr, n0 * (size_t)w.nslot));
+    CK(dmalloc(&w.ahist, (size_t)cap * w.nslot));
+    std::vector<T*> tab((size_t)w.nslot);
+    for (int k = 0; k < w.nslot; ++k) tab[(size_t)k] = w.pr + (size_t)k * n0;
+    CK(upload(&w.ptab, tab));
+  }
   w.ncap = cap;
   return SDC_OK;
 }
@@ -1086,7 +1106,7 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
     prof_end(h, SDC_PROF_APPLY, s2,
              (double)l0.nN * 4 * sizeof(T) + (double)l0.nN * ncol * 2 * sizeof(T));
     launch(h, k_fin_alpha, dim3(ncol), blk, 0, s2, (const double*)W.partial, nbx,
-           (const double*)W.rz, W.alpha, (const int*)W.active, vdone);
+           (const double*)W.rz, W.alpha, (const int*)W.active, vdone, (double*)nullptr);
     prof_begin(h, SDC_PROF_UPDATE, s2);
     launch(h, k_update_xr4<T>, grid4(l0.nN, ncol), blk, 0, s2, (T* const*)W.xtab, (const T*)W.p,
            W.r, (const T*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial,
@@ -1209,11 +1229,19 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
                   : ncol >= 8                      ? 8
                                                    : R;
   // Iteration: z = M r (z kept in the V-cycle storage type), beta, p = z + beta p and
-  // q = A z + beta q in one pass (plus the lagged x += alpha p), alpha, r -= alpha q, the update
-  // decision, and the predicated reliable update (fold x + alpha p into u, fp64 true residual).
+  // q = A z + beta q in one pass (p into the next ring slot), alpha, r -= alpha q, the update
+  // decision, and the predicated fold of the pending directions into u (reliable update or
+  // full ring) and fp64 true residual (reliable update).
   // Every launch goes through launch() so that the whole iteration can be recorded.
   bool fold_fresh = true;  // profiled runs only: the next fold writes u without reading it
-  auto iteration = [&](bool first, hipStream_t s) -> int {
+  int fold_pend = 0;       // profiled runs only: directions the next fold reads
+  const int S_ = W.nslot;
+  const size_t slot_len = l0.ld * (size_t)W.ncap;
+  // iteration `it` (0: the first) writes its search direction to ring slot it % S_
+  auto iteration = [&](bool first, hipStream_t s, int it) -> int {
+    const int phi = it % S_;
+    float* const p_out = W.pr + (size_t)phi * slot_len;
+    const float* const p_prev = W.pr + (size_t)((it + S_ - 1) % S_) * slot_len;
     int rc = SDC_OK;
     if (zb && r16)
       rc = vcycle<float, __bf16, __bf16, __bf16>(h, 0, r16, z16, ncol, true, stop, s);
@@ -1232,14 +1260,12 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
 #define SDC_APPLY_PQ(ZT, ZP)                                                                   \
     if (first) {                                                                               \
       DISPATCH_RG(R, G_, launch(h, (k_apply_pq<ZT, R, true, G>), grid2(l0.nN, ncol, R * G),    \
-                                blk, 0, s, s1, ZP, W.p, W.q, W.x, (const double*)W.beta,       \
-                                (const double*)W.alpha, l0.ld, ncol, W.partial,                \
-                                (const int*)W.ctl, K > 1 ? 1 : 0));                            \
+                                blk, 0, s, s1, ZP, p_prev, p_out, W.q, (const double*)W.beta,  \
+                                l0.ld, ncol, W.partial, (const int*)W.ctl, K > 1 ? 1 : 0));    \
     } else {                                                                                   \
       DISPATCH_RG(R, G_, launch(h, (k_apply_pq<ZT, R, false, G>), grid2(l0.nN, ncol, R * G),   \
-                                blk, 0, s, s1, ZP, W.p, W.q, W.x, (const double*)W.beta,       \
-                                (const double*)W.alpha, l0.ld, ncol, W.partial,                \
-                                (const int*)W.ctl, K > 1 ? 1 : 0));                            \
+                                blk, 0, s, s1, ZP, p_prev, p_out, W.q, (const double*)W.beta,  \
+                                l0.ld, ncol, W.partial, (const int*)W.ctl, K > 1 ? 1 : 0));    \
     }
     if (zb) {
       SDC_APPLY_PQ(__bf16, (const __bf16*)z16)
@@ -1247,10 +1273,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       SDC_APPLY_PQ(float, (const float*)W.z)
     }
 #undef SDC_APPLY_PQ
-    // algorithmic bytes of the steady state (x read and written)
-    prof_end(h, atag, s, (double)l0.nN * 16 + (double)l0.nN * ncol * (bz + 24.0));
+    // algorithmic bytes of the steady state: z (stencil), p_prev and q read, p and q written
+    prof_end(h, atag, s, (double)l0.nN * 16 + (double)l0.nN * ncol * (bz + 16.0));
     launch(h, k_fin_alpha, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx,
-           (const double*)W.rz, W.alpha, (const int*)W.active, stop);
+           (const double*)W.rz, W.alpha, (const int*)W.active, stop,
+           W.ahist + (size_t)phi * W.ncap);
     prof_begin(h, SDC_PROF_UPDATE, s);
     if (r16)
       launch(h, k_update_r4<true>, grid4(l0.nN, ncol), blk, 0, s, W.r, (const float*)W.q,
@@ -1262,7 +1289,7 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     launch(h, k_fin_norm, dim3(ncol), blk, 0, s, (const double*)W.partial, nblk4(l0.nN), W.rr,
            W.active, 0, stop);
     launch(h, k_decide, dim3(1), blk, 0, s, W.ctl, (const double*)W.rr, (const double*)W.aux,
-           (const double*)W.rmax, (const int*)W.active, ncol);
+           (const double*)W.rmax, (const int*)W.active, ncol, S_);
     // reliable update (every kernel below returns at once unless k_decide asked for it).  A
     // profiled run brackets the fold and the fp64 residual only when they do work, so it reads
     // the decision on the host (graphs are off while profiling).
@@ -1273,13 +1300,17 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       const int* hc = (const int*)h->h_ctl;
       upd = hc[SDC_CTL_STOP] == 0 && hc[SDC_CTL_UPD] != 0;
       fold_fresh = hc[SDC_CTL_UFRESH] != 0;
+      fold_pend = hc[SDC_CTL_PEND];
     }
     prof_begin(h, upd ? SDC_PROF_FOLD : -1, s);
-    launch(h, k_ir_fold_ctl,
+    launch(h, k_ir_fold_ring,
            dim3((nblk4(l0.ld) + SDC_FOLD_CHUNKS - 1) / SDC_FOLD_CHUNKS, (uint32_t)ncol), blk, 0, s,
-           (const double* const*)W.qu, (const float*)W.x, (const float*)W.p,
-           (const double*)W.alpha, l0.ld, l0.nN, (const int*)W.ctl);
-    prof_end(h, upd ? SDC_PROF_FOLD : -1, s, (double)l0.nN * ncol * (fold_fresh ? 16 : 24));
+           (const double* const*)W.qu, (const float* const*)W.ptab, (const double*)W.ahist, S_,
+           phi, (int)W.ncap, l0.ld, l0.nN, (const int*)W.ctl);
+    // algorithmic bytes: the pending directions read, u read (unless fresh) and written
+    prof_end(h, upd ? SDC_PROF_FOLD : -1, s,
+             (double)l0.nN * ncol * (4.0 * fold_pend + (fold_fresh ? 8 : 16)));
+    launch(h, k_after_fold, dim3(1), dim3(64), 0, s, W.ctl);
     prof_begin(h, upd ? SDC_PROF_REFINE : -1, s);
     // kIrChunks node blocks per work-group: most launches return at once (no update)
     DISPATCH_RG(R64, G_, launch(h, (k_ir_residual<float, R, false, G, kIrChunks>),
@@ -1293,23 +1324,25 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     launch(h, k_fin_true, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx_ir,
            (const double*)W.aux, tol2, W.orr, W.rmax, W.active, (const int*)W.ctl);
     launch(h, k_after_update, dim3(1), blk, 0, s, W.ctl, (const double*)W.orr,
-           (const double*)W.aux, (const int*)W.active, W.alpha, ncol);
+           (const double*)W.aux, (const int*)W.active, ncol);
     return SDC_OK;
   };
   // the first iteration (FIRST variants) is launched; later ones run check_every at a time,
   // then the host reads the control block
-  CK(iteration(true, st));
+  CK(iteration(true, st, 0));
   // profiled kernel classes are bracketed by events, which cannot sit inside a graph
   const bool rec = h->prm.graph && h->prof_mask == 0;
   const int variant = 1000 + 16 * K + (zb ? 1 : 0) + (r16 ? 2 : 0) + 4 * (R64 == 8);
   const int* h_ctl = (const int*)h->h_ctl;
   for (int done_it = 1;; done_it += K) {
     auto block = [&](hipStream_t s) -> int {
-      for (int k = 0; k < K; ++k) CK(iteration(false, s));
+      for (int k = 0; k < K; ++k) CK(iteration(false, s, done_it + k));
       return SDC_OK;
     };
+    // a recorded block fixes its ring slots: one graph per starting slot
     if (rec)
-      CK(graph_run(h, W.ctl, (void*)W.partial, ncol, variant, st, block));
+      CK(graph_run(h, W.ctl, (void*)W.partial, ncol, variant + ((done_it % S_) << 16), st,
+                   block));
     else
       CK(block(st));
     HIPCK(hipMemcpyAsync(h->h_ctl, W.ctlmem, (kCtlInts + SDC_DCTL_N) * sizeof(double),
