@@ -58,7 +58,7 @@ struct Sten {
   uint32_t n1x, n1y, n1z, nN, plane;
   FastDiv fx, fy;
   int pin0;
-  int remap;  // 1: XCD-contiguous work-group order
+  int remap;  // work-group order (block_of): 0 plain, 1 XCD-contiguous, B > 1 also column bands
   // Operator sets (several operators on one mesh, one per column group): column c uses the
   // arrays at offset cset[c] * sstride.  cset null: one operator.
   const int32_t* cset;

@@ -15,12 +15,32 @@
 
 // The linear work-group id (x fastest, as the dispatcher numbers them) is remapped so that
 // every XCD works through one contiguous range of (column group, node block) pairs.
+// Work-group -> (node block bx, column group by) of a (node blocks, column groups) grid.
+// remap 0: blockIdx order; 1: each XCD takes one contiguous run of the linear order (node
+// blocks fastest), so neighbouring planes of a sweep share that XCD's L2; B > 1: in addition
+// the column groups come in bands of B whose work-groups for one node block are consecutive,
+// so that block's coefficients are fetched once per band and then read from L2.
+__device__ __forceinline__ void block_of(int remap, uint32_t& bx, uint32_t& by) {
+  const uint32_t nbx = gridDim.x, ng = gridDim.y;
+  const uint32_t lin0 = blockIdx.x + nbx * blockIdx.y;
+  const uint32_t lin = remap ? xcd_remap(lin0, nbx * ng) : lin0;
+  if (remap <= 1) {
+    by = lin / nbx;
+    bx = lin - by * nbx;
+    return;
+  }
+  const uint32_t B = (uint32_t)remap;
+  const uint32_t band = lin / (nbx * B);
+  const uint32_t w = lin - band * nbx * B;
+  const uint32_t cb = min(B, ng - band * B);
+  bx = w / cb;
+  by = band * B + (w - bx * cb);
+}
+
 #define SDC_NODE_PROLOGUE_C(NN, REMAP, CPB)                                     \
   const uint32_t nbx = gridDim.x;                                               \
-  const uint32_t lin0_ = blockIdx.x + nbx * blockIdx.y;                         \
-  const uint32_t lin_ = (REMAP) ? xcd_remap(lin0_, nbx * gridDim.y) : lin0_;    \
-  const uint32_t by_ = lin_ / nbx;                                              \
-  const uint32_t bx = lin_ - by_ * nbx;                                         \
+  uint32_t bx, by_;                                                             \
+  block_of((REMAP), bx, by_);                                                   \
   const uint32_t n = bx * SDC_BLOCK + threadIdx.x;                              \
   const int c0 = (int)by_ * (CPB);                                              \
   const bool live = n < (NN);
@@ -531,11 +551,8 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_residual_rx(Sten<T> sa, const B* 
   if (done && ld_flag(done)) return;
   constexpr uint32_t TS = SDC_BLOCK - 2;
   __shared__ T rl[R][SDC_BLOCK];
-  const uint32_t nbx = gridDim.x;
-  const uint32_t lin0 = blockIdx.x + nbx * blockIdx.y;
-  const uint32_t lin = sa.remap ? xcd_remap(lin0, nbx * gridDim.y) : lin0;
-  const uint32_t by = lin / nbx;
-  const uint32_t bx = lin - by * nbx;
+  uint32_t bx, by;
+  block_of(sa.remap, bx, by);
   const int c0 = (int)by * R;
   const int64_t nn = (int64_t)bx * TS - 1 + threadIdx.x;
   const bool live = nn >= 0 && nn < (int64_t)sa.nN;

@@ -8,6 +8,12 @@
 #define SDC_FUSE_RX 1
 #endif
 
+// k_apply_pq: column groups per band whose work-groups for one node block run back to back
+// (its coefficients fetched once per band; measured on the 932k-node / 128-column solve: 461 ->
+// 438 us per launch with 4, while the smoothing kernels, which read b at 7 points, gained
+// nothing and keep the plain XCD order)
+constexpr int kApplyBand = 4;
+
 // search directions held by the mixed PCG before they are folded into the solution (a full
 // ring forces a fold)
 constexpr int kRingSlots = 6;
@@ -1175,7 +1181,8 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   pf.cy = l0.cy64;
   pf.cz = l0.cz64;
   const Sten<double> s2 = make_sten<double>(h, 0, 2);
-  const Sten<float> s1 = make_sten<float>(h, 0, 1);
+  Sten<float> s1 = make_sten<float>(h, 0, 1);
+  if (s1.remap) s1.remap = kApplyBand;
   const double tol2 = o->tol * o->tol;
   const double delta2 = h->prm.reliable_delta * h->prm.reliable_delta;
   const int maxit = o->max_iter;
