@@ -1012,15 +1012,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_ring(const double* const*
   }
 }
 
-// After the fold (one work-item): the ring is empty and u is written.
-__global__ void k_after_fold(int* __restrict__ ctl) {
-  if (threadIdx.x == 0 && ctl_fold(ctl)) {
-    ctl[SDC_CTL_PEND] = 0;
-    ctl[SDC_CTL_FOLD] = 0;
-    ctl[SDC_CTL_UFRESH] = 0;
-  }
-}
-
 // Reliable-update decision after an iteration of the mixed PCG (one work-group): counts the
 // iteration and its search direction (now pending in the ring) and asks for an update (replace
 // the fp32 residual by the fp64 true residual, fold the pending directions into u) when every
@@ -1058,16 +1049,24 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_decide(int* __restrict__ ctl,
   }
 }
 
-// After a reliable update (one work-group): the solve stops when no column is open
-// (converged), at the iteration limit, or when the largest true relative residual failed to
-// halve since the previous update (the fp32 recursion stalled; the caller continues in fp64).
-// rmax and active were set by k_fin_true.
+// After a fold (one work-group): the ring is empty and u is written.  After a reliable update
+// also: the solve stops when no column is open (converged), at the iteration limit, or when the
+// largest true relative residual failed to halve since the previous update (the fp32 recursion
+// stalled; the caller continues in fp64).  rmax and active were set by k_fin_true.
 __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ctl,
                                                             const double* __restrict__ orr,
                                                             const double* __restrict__ qn2,
                                                             const int* __restrict__ active,
                                                             int ncol) {
-  if (!ctl_update(ctl)) return;
+  if (!ctl_fold(ctl)) return;
+  if (ld_flag(ctl + SDC_CTL_UPD) == 0) {  // a fold alone (full ring)
+    if (threadIdx.x == 0) {
+      ctl[SDC_CTL_PEND] = 0;
+      ctl[SDC_CTL_FOLD] = 0;
+      ctl[SDC_CTL_UFRESH] = 0;
+    }
+    return;
+  }
   const int maxit = ld_flag(ctl + SDC_CTL_MAXIT);
   __shared__ double sm[SDC_BLOCK / 64];
   __shared__ int sn[SDC_BLOCK / 64];
@@ -1096,6 +1095,9 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ct
     const int it = ld_flag(ctl + SDC_CTL_IT);
     ctl[SDC_CTL_OUTER] = ld_flag(ctl + SDC_CTL_OUTER) + 1;
     ctl[SDC_CTL_UPD] = 0;
+    ctl[SDC_CTL_PEND] = 0;
+    ctl[SDC_CTL_FOLD] = 0;
+    ctl[SDC_CTL_UFRESH] = 0;
     dctl[SDC_DCTL_MREL] = m;
     int reason = 0;
     if (nact == 0)
@@ -1574,7 +1576,17 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coef_from_sigma(Geo g, const doub
   bd[n] = scale * rb;
 }
 
-// full diagonal and its inverse; pin = 1 makes node 0 an identity row
+// fp32 copies of a level's coefficients (null: none)
+struct Coef32 {
+  float* cx;
+  float* cy;
+  float* cz;
+  float* dg;
+  float* idg;
+};
+
+// full diagonal and its inverse; pin = 1 makes node 0 an identity row; with c32.cx set, also
+// the fp32 copies of cx, cy, cz, dg, idg (the fp32 solver's operator) in the same pass
 __global__ __launch_bounds__(SDC_BLOCK) void k_coef_diag(uint32_t n1x, uint32_t n1y, uint32_t nN,
                                                          FastDiv fx, FastDiv fy,
                                                          const double* __restrict__ cx,
@@ -1583,7 +1595,7 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coef_diag(uint32_t n1x, uint32_t 
                                                          const double* __restrict__ bd,
                                                          double* __restrict__ dg,
                                                          double* __restrict__ idg, int pin,
-                                                         size_t stride) {
+                                                         size_t stride, Coef32 c32) {
   const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
   if (n >= nN) return;
   const size_t zo = (size_t)blockIdx.z * stride;  // operator set
@@ -1605,6 +1617,13 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coef_diag(uint32_t n1x, uint32_t 
   if (pin && n == 0) d = 1.0;
   dg[n] = d;
   idg[n] = 1.0 / d;
+  if (c32.cx) {
+    c32.cx[zo + n] = (float)cx[n];
+    c32.cy[zo + n] = (float)cy[n];
+    c32.cz[zo + n] = (float)cz[n];
+    c32.dg[zo + n] = (float)d;
+    c32.idg[zo + n] = (float)(1.0 / d);
+  }
 }
 
 // Coarse-level conductances from fine ones: along the edge direction the (one or two)
@@ -1704,15 +1723,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coarsen_coef(Xfer<double> t,
   ccy[n] = sy;
   ccz[n] = sz;
   cbd[n] = sb;
-}
-
-template <typename TO>
-// b = a in type TO, len entries of operator set blockIdx.z (arrays + z * stride)
-__global__ __launch_bounds__(SDC_BLOCK) void k_cast(const double* __restrict__ a, TO* __restrict__ b,
-                                                    size_t len, size_t stride) {
-  const size_t i = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
-  const size_t z = (size_t)blockIdx.z * stride;
-  if (i < len) b[z + i] = (TO)a[z + i];
 }
 
 // dense operator of the coarsest level (row-major nn x nn), pin = 1: identity row/column 0;

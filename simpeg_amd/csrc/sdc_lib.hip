@@ -1317,7 +1317,6 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     // algorithmic bytes: the pending directions read, u read (unless fresh) and written
     prof_end(h, upd ? SDC_PROF_FOLD : -1, s,
              (double)l0.nN * ncol * (4.0 * fold_pend + (fold_fresh ? 8 : 16)));
-    launch(h, k_after_fold, dim3(1), dim3(64), 0, s, W.ctl);
     prof_begin(h, upd ? SDC_PROF_REFINE : -1, s);
     // kIrChunks node blocks per work-group: most launches return at once (no update)
     DISPATCH_RG(R64, G_, launch(h, (k_ir_residual<float, R, false, G, kIrChunks>),
@@ -1720,17 +1719,11 @@ int build_levels(sdc_handle_s* h, hipStream_t st, int nset) {
   }
   const int pin = h->bc == SDC_BC_NEUMANN ? 1 : 0;
   for (auto& v : h->L) {
+    const Coef32 c32{v.cx32, v.cy32, v.cz32, v.dg32, v.idg32};
     hipLaunchKernelGGL(k_coef_diag, dim3(nblk(v.nN), 1, z), dim3(SDC_BLOCK), 0, st, v.n1x, v.n1y,
                        v.nN, make_fastdiv(v.n1x), make_fastdiv(v.n1y), (const double*)v.cx64,
                        (const double*)v.cy64, (const double*)v.cz64, (const double*)v.bd64,
-                       v.dg64, v.idg64, pin, v.sstride);
-    const dim3 gg(nblk(v.nN), 1, z);
-    const size_t n = v.nN, ss = v.sstride;
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cx64, v.cx32, n, ss);
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cy64, v.cy32, n, ss);
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.cz64, v.cz32, n, ss);
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.dg64, v.dg32, n, ss);
-    hipLaunchKernelGGL(k_cast<float>, gg, dim3(SDC_BLOCK), 0, st, (const double*)v.idg64, v.idg32, n, ss);
+                       v.dg64, v.idg64, pin, v.sstride, c32);
   }
   // coarsest level: dense inverse
   const int lc = (int)h->L.size() - 1;
@@ -1997,7 +1990,7 @@ int sdc_jvec_rhs(sdc_handle h, const double* dsig, const double* u, double* out,
   hipLaunchKernelGGL(k_coef_diag, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, l0.n1x, l0.n1y,
                      l0.nN, make_fastdiv(l0.n1x), make_fastdiv(l0.n1y), (const double*)h->tcx,
                      (const double*)h->tcy, (const double*)h->tcz, (const double*)h->tbd, h->tdg,
-                     h->tidg, 0, (size_t)0);
+                     h->tidg, 0, (size_t)0, Coef32{});
   Sten<double> s{};
   s.cx = h->tcx; s.cy = h->tcy; s.cz = h->tcz; s.dg = h->tdg; s.idg = h->tidg;
   s.n1x = l0.n1x; s.n1y = l0.n1y; s.n1z = l0.n1z; s.nN = l0.nN; s.plane = l0.plane;

@@ -286,13 +286,17 @@ class NodalEngine:
             raise ValueError(f"ucol entries must lie in [0, {u.shape[0]})")
         return torch.as_tensor(uc[:ncol].astype(np.int32)).to(self.device).contiguous()
 
-    def contract(self, u, lam, ncol=None, ucol=None, per_column=False):
+    def contract(self, u, lam, ncol=None, ucol=None, per_column=False, out=None):
+        """sum over columns (or, per_column, one row per column) of the adjoint contraction;
+        ``out``: a contiguous float64 device tensor of that shape to write into."""
         ncol = lam.shape[0] if ncol is None else ncol
         ucol = self._check_pairing(u, lam, ncol, ucol)
-        if per_column:
-            out = torch.empty((ncol, self.nC), dtype=torch.float64, device=self.device)
-        else:
-            out = torch.empty((self.nC,), dtype=torch.float64, device=self.device)
+        shape = (ncol, self.nC) if per_column else (self.nC,)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float64, device=self.device)
+        elif (tuple(out.shape) != shape or out.dtype != torch.float64 or not out.is_contiguous()
+              or out.device != torch.device(self.device)):
+            raise ValueError(f"out must be a contiguous float64 {shape} tensor on {self.device}")
         uc = ucol
         _lib.check(self.lib.sdc_jtvec_contract(self.h, _lib.ptr(u), _lib.ptr(lam), ncol,
                                                _lib.ptr(uc) if uc is not None else None,

@@ -16,13 +16,15 @@ releases the GIL), largest first.
 from __future__ import annotations
 
 import concurrent.futures as cf
-import warnings
 
 import numpy as np
 import torch
 
 from .engine import NodalEngine
-from .maps import Projection
+from .maps import ExpMap, IdentityMap, Projection, ReciprocalMap
+
+# elementwise maps whose device evaluation has no per-instance state
+_STATELESS_MAPS = (IdentityMap, ExpMap, ReciprocalMap)
 
 
 def sum_adjoint(mappings, model, parts):
@@ -96,6 +98,7 @@ class _SetGroup:
         self._tuned = False
         self._versions = None  # the members' model versions the engine's operators belong to
         self._m_dev = None     # the members' models on the device (device-evaluated maps)
+        self._scat = None      # P^T of all members into the group's block (one scatter table)
 
     def _views(self, block):
         return [block[o:o + n] for o, n in zip(self.offs, self.counts)]
@@ -104,6 +107,18 @@ class _SetGroup:
         fms = [s._device_map() for s in self.sims]
         return fms if all(f is not None for f in fms) else None
 
+    def _shared_device_map(self):
+        """One device map for all members, evaluated on the (members, cells) block at once: every
+        member has the same elementwise map class without per-instance state (Identity, Exp,
+        Reciprocal) as its sigmaMap, or every member as its rhoMap.  None otherwise."""
+        kind = (type(self.sims[0].sigmaMap), type(self.sims[0].rhoMap))
+        if any((type(s.sigmaMap), type(s.rhoMap)) != kind for s in self.sims):
+            return None
+        if not all(t in _STATELESS_MAPS or t is type(None) for t in kind):
+            return None
+        fms = self._device_maps()
+        return None if fms is None else fms[0]
+
     def _set_models(self):
         """Operators of the members' current models (rebuilt only when a model changed).  With
         elementwise maps the models go to the device and sigma is evaluated there; otherwise
@@ -111,19 +126,18 @@ class _SetGroup:
         eng = self.engine
         versions = tuple(s._model_version for s in self.sims)
         if versions != self._versions:
-            fms = self._device_maps()
-            if fms is not None:
-                m_dev = torch.empty((len(self.sims), eng.nC), dtype=torch.float64,
-                                    device=eng.device)
-                for k, s in enumerate(self.sims):
-                    mk = np.ascontiguousarray(s.model, dtype=float)
-                    with warnings.catch_warnings():
-                        # a read-only view of the meta model: torch only reads it here
-                        warnings.simplefilter("ignore", UserWarning)
-                        m_dev[k].copy_(torch.from_numpy(mk))
-                sig = torch.empty_like(m_dev)
-                for k, f in enumerate(fms):
-                    sig[k] = f[0](m_dev[k])
+            fm = self._shared_device_map()
+            fms = self._device_maps() if fm is None else None
+            if fm is not None or fms is not None:
+                # the members' models in one host block and one upload
+                m_host = np.stack([np.asarray(s.model, dtype=float) for s in self.sims])
+                m_dev = torch.from_numpy(m_host).to(eng.device)
+                if fm is not None:
+                    sig = fm[0](m_dev)
+                else:
+                    sig = torch.empty_like(m_dev)
+                    for k, f in enumerate(fms):
+                        sig[k] = f[0](m_dev[k])
                 eng.set_sigma_sets(sig)
                 self._m_dev = m_dev
             else:
@@ -133,6 +147,31 @@ class _SetGroup:
         want = 1 if self.sims[0]._deriv_includes_bc else 0
         if eng.params.deriv_robin != want:
             eng.set_params(deriv_robin=want)
+
+    def _group_scatter(self):
+        """The members' P^T tables joined into one for the group's block: member k's
+        destinations shifted by its first column, its data indices by its data offset."""
+        if self._scat is None:
+            from .engine import ScatterCSR
+
+            eng = self.engine
+            dst, rowptr, src, w = [], [], [], []
+            ne = 0
+            nd = 0
+            nrow = 0
+            for s, off in zip(self.sims, self.offs):
+                sc = s._projection(eng)["scatter"]
+                dst.append(sc.dst + off * eng.ld)
+                rowptr.append(sc.rowptr[:-1] + ne)
+                src.append(sc.src + nd)
+                w.append(sc.w)
+                ne += int(sc.rowptr[-1].item())
+                nd += s.survey.nD
+                nrow += sc.nrow
+            rowptr.append(torch.tensor([ne], dtype=torch.int32, device=eng.device))
+            self._scat = ScatterCSR(nrow, torch.cat(dst), torch.cat(rowptr).to(torch.int32),
+                                    torch.cat(src).to(torch.int32), torch.cat(w))
+        return self._scat
 
     def _source(self):
         if self._q is None:
@@ -187,28 +226,28 @@ class _SetGroup:
     def jtvec(self, v, f):
         eng = self.engine
         self._set_models()
-        rhs = eng.block(self.ncol)
         for s, vk in zip(self.sims, v):
             if np.size(vk) != s.survey.nD:
                 raise ValueError(f"v must have {s.survey.nD} entries, got {np.size(vk)}")
-        # every member's data vector in one upload
+        # every member's data vector in one upload, P^T of all members in one scatter
         v_dev = torch.as_tensor(np.concatenate([np.asarray(vk, dtype=float).ravel() for vk in v]))
-        v_dev = v_dev.to(eng.device)
-        o = 0
-        for s, out in zip(self.sims, self._views(rhs)):
-            n = s.survey.nD
-            eng.scatter(s._projection(eng)["scatter"], v_dev[o:o + n], out.shape[0], out=out)
-            o += n
+        rhs = eng.scatter(self._group_scatter(), v_dev.to(eng.device), self.ncol)
         lam = self._solve(rhs)
-        gs = [eng.contract(fk.u_block, lk, lk.shape[0]) for fk, lk in zip(f, self._views(lam))]
-        fms = self._device_maps() if self._m_dev is not None else None
-        if fms is not None:
+        G = torch.empty((len(self.sims), eng.nC), dtype=torch.float64, device=eng.device)
+        for k, (fk, lk) in enumerate(zip(f, self._views(lam))):
+            eng.contract(fk.u_block, lk, lk.shape[0], out=G[k])
+        if self._m_dev is not None:
             # -sigmaDeriv^T g on the device (diagonal derivative), one download for all members
-            out = torch.cat([-(fm[1](self._m_dev[k]) * g) for k, (fm, g) in enumerate(zip(fms, gs))])
-            flat = out.cpu().numpy()
-            offs = np.cumsum([0] + [g.numel() for g in gs])
-            return [flat[offs[k]:offs[k + 1]] for k in range(len(gs))]
-        return [-s._sigma_deriv_T(g.cpu().numpy()) for s, g in zip(self.sims, gs)]
+            fm = self._shared_device_map()
+            if fm is not None:
+                flat = (-(fm[1](self._m_dev) * G)).cpu().numpy()
+            else:
+                fms = self._device_maps()
+                flat = torch.stack([-(fms[k][1](self._m_dev[k]) * G[k])
+                                    for k in range(len(self.sims))]).cpu().numpy()
+            return [flat[k] for k in range(len(self.sims))]
+        Gh = G.cpu().numpy()
+        return [-s._sigma_deriv_T(Gh[k]) for k, s in enumerate(self.sims)]
 
 
 class MetaSimulation:
