@@ -350,14 +350,15 @@ def batch_result(args):
         v = np.random.default_rng(5).standard_normal(meta.nD)
         f = meta.fields(m)  # warm-up (workspace allocation)
         meta.Jtvec(m, v, f=f)
+        m1 = m + 1e-3  # a new model: every problem rebuilds its operator
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _range(f"batch fields streams={ns}")
-        f = meta.fields(m + 1e-3)  # a new model: every problem rebuilds its operator
+        f = meta.fields(m1)
         _range(None)
         t1 = time.perf_counter()
         _range(f"batch jtvec streams={ns}")
-        meta.Jtvec(m + 1e-3, v, f=f)
+        meta.Jtvec(m1, v, f=f)
         torch.cuda.synchronize()
         _range(None)
         t2 = time.perf_counter()

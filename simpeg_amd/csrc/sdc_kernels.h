@@ -79,6 +79,7 @@ __device__ __forceinline__ bool set_changes(const Sten<T>& sa, int c, int cprev)
 #define SDC_CTL_MAXIT 8
 #define SDC_CTL_WINDOW 9
 #define SDC_CTL_FOLD 10
+#define SDC_CTL_TICKET 11  // last-work-group counter of the finalisers (zero between launches)
 #define SDC_CTL_N 16
 // doubles after the ints: the largest true relative residual of the previous update and of the
 // last one, tol^2 and reliable_delta^2
@@ -103,6 +104,24 @@ __device__ __forceinline__ bool ctl_update(const int* ctl) {
 __device__ __forceinline__ bool ctl_fold(const int* ctl) {
   return ld_flag(ctl + SDC_CTL_STOP) == 0 &&
          (ld_flag(ctl + SDC_CTL_UPD) != 0 || ld_flag(ctl + SDC_CTL_FOLD) != 0);
+}
+
+// The work-group that finishes last among `count` work-groups sharing a ticket counter: each
+// publishes its results from thread 0, which then draws a ticket (the release orders its
+// stores before the ticket); the one that draws the last sees the others' results (read them
+// with device-scope loads) and resets the counter for the next launch.  A launch whose
+// work-groups all return before drawing leaves the counter at zero.  Each ticket costs a
+// write-back and invalidation of the XCD's L2: only for launches of few work-groups.
+__device__ __forceinline__ bool last_of(int* ticket, uint32_t count) {
+  __shared__ int is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = t == (int)count - 1;
+    if (is_last) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  return is_last != 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -426,20 +445,12 @@ __device__ __forceinline__ void restrict_weights(const Xfer<T>& t, uint32_t I, u
 // its two edges along AX on the fine line through it: the exact interpolation of the 1D
 // operator, equal to the geometric weight in a homogeneous medium.  Coincident nodes keep their
 // 0 / 1 and nodes of longer gaps the geometric weight.  t: the geometric tables (fp64);
-// c: the fine level's conductances along AX.
+// n: the point of the factor's grid; c: the fine level's conductances along AX (of one operator
+// set, as w64 and w32).  Run by k_coarsen_coef.
 template <int AX>
-__global__ __launch_bounds__(SDC_BLOCK) void k_interp_weights(Xfer<double> t, uint32_t g0,
-                                                              uint32_t g1, uint32_t g2,
-                                                              const double* __restrict__ c,
-                                                              double* __restrict__ w64,
-                                                              float* __restrict__ w32,
-                                                              size_t cstride, size_t wstride) {
-  const size_t n = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
-  if (n >= (size_t)g0 * g1 * g2) return;
-  // operator set blockIdx.z: its conductances and its weights
-  c += (size_t)blockIdx.z * cstride;
-  w64 += (size_t)blockIdx.z * wstride;
-  w32 += (size_t)blockIdx.z * wstride;
+__device__ __forceinline__ void interp_weight(const Xfer<double>& t, size_t n, uint32_t g0,
+                                              uint32_t g1, const double* __restrict__ c,
+                                              double* __restrict__ w64, float* __restrict__ w32) {
   const uint32_t p0 = (uint32_t)(n % g0);
   const size_t q = n / g0;
   const uint32_t p1 = (uint32_t)(q % g1);
@@ -1012,110 +1023,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_ring(const double* const*
   }
 }
 
-// Reliable-update decision after an iteration of the mixed PCG (one work-group): counts the
-// iteration and its search direction (now pending in the ring) and asks for an update (replace
-// the fp32 residual by the fp64 true residual, fold the pending directions into u) when every
-// active column's fp32 residual |r|^2 passed tol^2 |q|^2, some active column's dropped by
-// delta^2 since its last replacement (rmax), the iteration limit is reached, or `window`
-// iterations passed without an update.  Otherwise, when all nslot slots of the ring are
-// pending, it asks for a fold alone (the next iteration writes the oldest slot).
-__global__ __launch_bounds__(SDC_BLOCK) void k_decide(int* __restrict__ ctl,
-                                                      const double* __restrict__ rr,
-                                                      const double* __restrict__ qn2,
-                                                      const double* __restrict__ rmax,
-                                                      const int* __restrict__ active, int ncol,
-                                                      int nslot) {
-  if (ld_flag(ctl + SDC_CTL_STOP)) return;
-  const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
-  const double delta2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_DELTA2);
-  const int maxit = ld_flag(ctl + SDC_CTL_MAXIT), window = ld_flag(ctl + SDC_CTL_WINDOW);
-  int cand = 1, drop = 0;
-  for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
-    if (!active[c]) continue;
-    if (rr[c] > tol2 * qn2[c]) cand = 0;
-    if (rr[c] <= delta2 * rmax[c]) drop = 1;
-  }
-  cand = __syncthreads_and(cand);
-  drop = __syncthreads_or(drop);
-  if (threadIdx.x == 0) {
-    const int it = ld_flag(ctl + SDC_CTL_IT) + 1;
-    ctl[SDC_CTL_IT] = it;
-    const int pend = ld_flag(ctl + SDC_CTL_PEND) + 1;
-    ctl[SDC_CTL_PEND] = pend;
-    const int upd =
-        (cand || drop || it >= maxit || it - ld_flag(ctl + SDC_CTL_LAST) >= window) ? 1 : 0;
-    ctl[SDC_CTL_UPD] = upd;
-    ctl[SDC_CTL_FOLD] = (!upd && pend >= nslot) ? 1 : 0;
-  }
-}
-
-// After a fold (one work-group): the ring is empty and u is written.  After a reliable update
-// also: the solve stops when no column is open (converged), at the iteration limit, or when the
-// largest true relative residual failed to halve since the previous update (the fp32 recursion
-// stalled; the caller continues in fp64).  rmax and active were set by k_fin_true.
-__global__ __launch_bounds__(SDC_BLOCK) void k_after_update(int* __restrict__ ctl,
-                                                            const double* __restrict__ orr,
-                                                            const double* __restrict__ qn2,
-                                                            const int* __restrict__ active,
-                                                            int ncol) {
-  if (!ctl_fold(ctl)) return;
-  if (ld_flag(ctl + SDC_CTL_UPD) == 0) {  // a fold alone (full ring)
-    if (threadIdx.x == 0) {
-      ctl[SDC_CTL_PEND] = 0;
-      ctl[SDC_CTL_FOLD] = 0;
-      ctl[SDC_CTL_UFRESH] = 0;
-    }
-    return;
-  }
-  const int maxit = ld_flag(ctl + SDC_CTL_MAXIT);
-  __shared__ double sm[SDC_BLOCK / 64];
-  __shared__ int sn[SDC_BLOCK / 64];
-  double m = 0.0;
-  int nact = 0;
-  for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
-    nact += active[c] != 0;
-    if (qn2[c] > 0.0) m = fmax(m, sqrt(orr[c] / qn2[c]));
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    m = fmax(m, __shfl_xor(m, off, 64));
-    nact += __shfl_xor(nact, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    sm[threadIdx.x >> 6] = m;
-    sn[threadIdx.x >> 6] = nact;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < SDC_BLOCK / 64; ++w) {
-      m = fmax(m, sm[w]);
-      nact += sn[w];
-    }
-    double* dctl = ctl_dbl(ctl);
-    const int it = ld_flag(ctl + SDC_CTL_IT);
-    ctl[SDC_CTL_OUTER] = ld_flag(ctl + SDC_CTL_OUTER) + 1;
-    ctl[SDC_CTL_UPD] = 0;
-    ctl[SDC_CTL_PEND] = 0;
-    ctl[SDC_CTL_FOLD] = 0;
-    ctl[SDC_CTL_UFRESH] = 0;
-    dctl[SDC_DCTL_MREL] = m;
-    int reason = 0;
-    if (nact == 0)
-      reason = 1;
-    else if (it >= maxit)
-      reason = 2;
-    else if (m > 0.5 * ld_flag(dctl + SDC_DCTL_MPREV))
-      reason = 3;
-    if (reason) {
-      ctl[SDC_CTL_REASON] = reason;
-      ctl[SDC_CTL_STOP] = 1;
-    } else {
-      dctl[SDC_DCTL_MPREV] = m;
-      ctl[SDC_CTL_LAST] = it;
-    }
-  }
-}
-
 // Start of a device-driven solve: the control block with the solve's parameters, and q, u in
 // the pointer table.
 __global__ void k_ctl_init(int* __restrict__ ctl, const double** __restrict__ qu,
@@ -1208,6 +1115,153 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_norm(const double* __restrict
   if (threadIdx.x == 0) {
     out[c] = v;
     if (init) active[c] = (v > 0.0) ? 1 : 0;
+  }
+}
+
+// Reliable-update decision after an iteration of the mixed PCG, in the last work-group of
+// k_fin_decide: counts the iteration and its search direction (now pending in the ring) and
+// asks for an update (replace the fp32 residual by the fp64 true residual, fold the pending
+// directions into u) when every active column's fp32 residual |r|^2 passed tol^2 |q|^2, some
+// active column's dropped by delta^2 since its last replacement (rmax), the iteration limit is
+// reached, or `window` iterations passed without an update.  Otherwise, when all nslot slots of
+// the ring are pending, it asks for a fold alone (the next iteration writes the oldest slot).
+__device__ void decide(int* __restrict__ ctl, const double* __restrict__ rr,
+                       const double* __restrict__ qn2, const double* __restrict__ rmax,
+                       const int* __restrict__ active, int ncol, int nslot) {
+  if (ld_flag(ctl + SDC_CTL_STOP)) return;
+  const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
+  const double delta2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_DELTA2);
+  const int maxit = ld_flag(ctl + SDC_CTL_MAXIT), window = ld_flag(ctl + SDC_CTL_WINDOW);
+  int cand = 1, drop = 0;
+  for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
+    if (!active[c]) continue;
+    const double r = ld_flag(rr + c);
+    if (r > tol2 * qn2[c]) cand = 0;
+    if (r <= delta2 * rmax[c]) drop = 1;
+  }
+  cand = __syncthreads_and(cand);
+  drop = __syncthreads_or(drop);
+  if (threadIdx.x == 0) {
+    const int it = ld_flag(ctl + SDC_CTL_IT) + 1;
+    ctl[SDC_CTL_IT] = it;
+    const int pend = ld_flag(ctl + SDC_CTL_PEND) + 1;
+    ctl[SDC_CTL_PEND] = pend;
+    const int upd =
+        (cand || drop || it >= maxit || it - ld_flag(ctl + SDC_CTL_LAST) >= window) ? 1 : 0;
+    ctl[SDC_CTL_UPD] = upd;
+    ctl[SDC_CTL_FOLD] = (!upd && pend >= nslot) ? 1 : 0;
+  }
+}
+
+// rr[c] = |r_c|^2 from the update's per-block partials (one work-group per column), then the
+// reliable-update decision in the work-group that finishes last.
+__global__ __launch_bounds__(SDC_BLOCK) void k_fin_decide(const double* __restrict__ partial,
+                                                          uint32_t nbx, double* __restrict__ rr,
+                                                          int* __restrict__ ctl,
+                                                          const double* __restrict__ qn2,
+                                                          const double* __restrict__ rmax,
+                                                          const int* __restrict__ active,
+                                                          int nslot,
+                                                          const int* __restrict__ done) {
+  if (done && ld_flag(done)) return;
+  const int c = blockIdx.x;
+  const double v = block_sum(partial + (size_t)c * nbx, nbx);
+  if (threadIdx.x == 0) rr[c] = v;
+  if (!last_of(ctl + SDC_CTL_TICKET, gridDim.x)) return;
+  decide(ctl, rr, qn2, rmax, active, (int)gridDim.x, nslot);
+}
+
+// After a fold: the ring is empty and u is written.  After a reliable update (one work-group
+// per column) also: the true residual |q - A u|^2 per column from the fp64 residual's
+// partials (rr, rmax, and active = rr > tol^2 |q|^2), and, in the work-group that finishes
+// last, the stop rules: the solve stops when no column is open (converged), at the iteration
+// limit, or when the largest true relative residual failed to halve since the previous update
+// (the fp32 recursion stalled; the caller continues in fp64).  At the stop the control block
+// is also written to hctl (pinned host memory, may be null) for the host to read.
+__global__ __launch_bounds__(SDC_BLOCK) void k_fin_true_update(const double* __restrict__ partial,
+                                                               uint32_t nbx,
+                                                               const double* __restrict__ qn2,
+                                                               double* __restrict__ rr,
+                                                               double* __restrict__ rmax,
+                                                               int* __restrict__ active,
+                                                               int* __restrict__ ctl,
+                                                               int* __restrict__ hctl) {
+  if (!ctl_fold(ctl)) return;
+  if (ld_flag(ctl + SDC_CTL_UPD) == 0) {  // a fold alone (full ring)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl[SDC_CTL_PEND] = 0;
+      ctl[SDC_CTL_FOLD] = 0;
+      ctl[SDC_CTL_UFRESH] = 0;
+    }
+    return;
+  }
+  const int ncol = (int)gridDim.x;
+  {
+    const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
+    const int c = blockIdx.x;
+    const double v = block_sum(partial + (size_t)c * nbx, nbx);
+    if (threadIdx.x == 0) {
+      rr[c] = v;
+      rmax[c] = v;
+      active[c] = (qn2[c] > 0.0 && v > tol2 * qn2[c]) ? 1 : 0;
+    }
+  }
+  if (!last_of(ctl + SDC_CTL_TICKET, gridDim.x)) return;
+  const int maxit = ld_flag(ctl + SDC_CTL_MAXIT);
+  __shared__ double sm[SDC_BLOCK / 64];
+  __shared__ int sn[SDC_BLOCK / 64];
+  double m = 0.0;
+  int nact = 0;
+  for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
+    nact += ld_flag(active + c) != 0;
+    if (qn2[c] > 0.0) m = fmax(m, sqrt(ld_flag(rr + c) / qn2[c]));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    m = fmax(m, __shfl_xor(m, off, 64));
+    nact += __shfl_xor(nact, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sm[threadIdx.x >> 6] = m;
+    sn[threadIdx.x >> 6] = nact;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < SDC_BLOCK / 64; ++w) {
+      m = fmax(m, sm[w]);
+      nact += sn[w];
+    }
+    double* dctl = ctl_dbl(ctl);
+    const int it = ld_flag(ctl + SDC_CTL_IT);
+    ctl[SDC_CTL_OUTER] = ld_flag(ctl + SDC_CTL_OUTER) + 1;
+    ctl[SDC_CTL_UPD] = 0;
+    ctl[SDC_CTL_PEND] = 0;
+    ctl[SDC_CTL_FOLD] = 0;
+    ctl[SDC_CTL_UFRESH] = 0;
+    dctl[SDC_DCTL_MREL] = m;
+    int reason = 0;
+    if (nact == 0)
+      reason = 1;
+    else if (it >= maxit)
+      reason = 2;
+    else if (m > 0.5 * ld_flag(dctl + SDC_DCTL_MPREV))
+      reason = 3;
+    if (reason) {
+      ctl[SDC_CTL_REASON] = reason;
+      ctl[SDC_CTL_STOP] = 1;
+      if (hctl) {
+        // the stop flag goes last, after a system-scope fence
+        for (int i = 0; i < SDC_CTL_N; ++i)
+          if (i != SDC_CTL_STOP) hctl[i] = ld_flag(ctl + i);
+        double* hd = (double*)(hctl + SDC_CTL_N);
+        for (int i = 0; i < SDC_DCTL_N; ++i) hd[i] = ld_flag(dctl + i);
+        __threadfence_system();
+        __hip_atomic_store(hctl + SDC_CTL_STOP, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    } else {
+      dctl[SDC_DCTL_MPREV] = m;
+      ctl[SDC_CTL_LAST] = it;
+    }
   }
 }
 
@@ -1417,26 +1471,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_residual(Sten<double> sa, cons
   block_partials<R * G>(acc, partial, c0, ncol, nbx, bx);
 }
 
-// true-residual check after a reliable update: rr = |q - A u|^2, rmax = rr,
-// active = rr > tol^2 |q|^2 (ctl non-null: only when a reliable update runs)
-__global__ __launch_bounds__(SDC_BLOCK) void k_fin_true(const double* __restrict__ partial,
-                                                        uint32_t nbx,
-                                                        const double* __restrict__ qn2,
-                                                        double tol2, double* __restrict__ rr,
-                                                        double* __restrict__ rmax,
-                                                        int* __restrict__ active,
-                                                        const int* __restrict__ ctl) {
-  if (ctl && !ctl_update(ctl)) return;
-  if (ctl) tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
-  const int c = blockIdx.x;
-  const double v = block_sum(partial + (size_t)c * nbx, nbx);
-  if (threadIdx.x == 0) {
-    rr[c] = v;
-    rmax[c] = v;
-    active[c] = (qn2[c] > 0.0 && v > tol2 * qn2[c]) ? 1 : 0;
-  }
-}
-
 // u += e
 template <typename TE, int R>
 __global__ __launch_bounds__(SDC_BLOCK) void k_ir_update(double* __restrict__ u,
@@ -1585,51 +1619,73 @@ struct Coef32 {
   float* idg;
 };
 
-// full diagonal and its inverse; pin = 1 makes node 0 an identity row; with c32.cx set, also
-// the fp32 copies of cx, cy, cz, dg, idg (the fp32 solver's operator) in the same pass
-__global__ __launch_bounds__(SDC_BLOCK) void k_coef_diag(uint32_t n1x, uint32_t n1y, uint32_t nN,
-                                                         FastDiv fx, FastDiv fy,
-                                                         const double* __restrict__ cx,
-                                                         const double* __restrict__ cy,
-                                                         const double* __restrict__ cz,
-                                                         const double* __restrict__ bd,
-                                                         double* __restrict__ dg,
-                                                         double* __restrict__ idg, int pin,
-                                                         size_t stride, Coef32 c32) {
-  const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
-  if (n >= nN) return;
-  const size_t zo = (size_t)blockIdx.z * stride;  // operator set
-  cx += zo;
-  cy += zo;
-  cz += zo;
-  bd += zo;
-  dg += zo;
-  idg += zo;
-  const uint32_t r = fdiv(n, fx);
-  const uint32_t i = n - r * n1x;
-  const uint32_t k = fdiv(r, fy);
-  const uint32_t j = r - k * n1y;
-  const uint32_t plane = n1x * n1y;
-  double d = cx[n] + cy[n] + cz[n] + bd[n];
+// One level of k_coef_diag: its shape, coefficient arrays (one operator per set, stride
+// entries apart), its first work-group in the launch, and pin = 1 to make node 0 an identity row.
+struct DiagLevel {
+  uint32_t n1x, n1y, nN, blk0;
+  FastDiv fx, fy;
+  const double *cx, *cy, *cz, *bd;
+  double *dg, *idg;
+  size_t stride;
+  Coef32 c32;
+  int pin;
+};
+#define SDC_DIAG_LEVELS 12
+struct DiagLevels {
+  DiagLevel lv[SDC_DIAG_LEVELS];
+  int n;
+};
+
+// Full diagonal and its inverse of up to SDC_DIAG_LEVELS levels in one launch (work-groups
+// [lv[l].blk0, lv[l+1].blk0) take level l, blockIdx.z the operator set); with c32.cx set, also
+// the fp32 copies of cx, cy, cz, dg, idg (the fp32 solver's operator) in the same pass.
+__global__ __launch_bounds__(SDC_BLOCK) void k_coef_diag(DiagLevels tab) {
+  int l = 0;
+  while (l + 1 < tab.n && blockIdx.x >= tab.lv[l + 1].blk0) ++l;
+  const DiagLevel& v = tab.lv[l];
+  const uint32_t n = (blockIdx.x - v.blk0) * SDC_BLOCK + threadIdx.x;
+  if (n >= v.nN) return;
+  const size_t zo = (size_t)blockIdx.z * v.stride;  // operator set
+  const double* cx = v.cx + zo;
+  const double* cy = v.cy + zo;
+  const double* cz = v.cz + zo;
+  const uint32_t r = fdiv(n, v.fx);
+  const uint32_t i = n - r * v.n1x;
+  const uint32_t k = fdiv(r, v.fy);
+  const uint32_t j = r - k * v.n1y;
+  const uint32_t plane = v.n1x * v.n1y;
+  double d = cx[n] + cy[n] + cz[n] + v.bd[zo + n];
   if (i > 0) d += cx[n - 1];
-  if (j > 0) d += cy[n - n1x];
+  if (j > 0) d += cy[n - v.n1x];
   if (k > 0) d += cz[n - plane];
-  if (pin && n == 0) d = 1.0;
-  dg[n] = d;
-  idg[n] = 1.0 / d;
-  if (c32.cx) {
-    c32.cx[zo + n] = (float)cx[n];
-    c32.cy[zo + n] = (float)cy[n];
-    c32.cz[zo + n] = (float)cz[n];
-    c32.dg[zo + n] = (float)d;
-    c32.idg[zo + n] = (float)(1.0 / d);
+  if (v.pin && n == 0) d = 1.0;
+  v.dg[zo + n] = d;
+  v.idg[zo + n] = 1.0 / d;
+  if (v.c32.cx) {
+    v.c32.cx[zo + n] = (float)cx[n];
+    v.c32.cy[zo + n] = (float)cy[n];
+    v.c32.cz[zo + n] = (float)cz[n];
+    v.c32.dg[zo + n] = (float)d;
+    v.c32.idg[zo + n] = (float)(1.0 / d);
   }
 }
 
-// Coarse-level conductances from fine ones: along the edge direction the (one or two)
-// fine edges between the two coarse nodes act in series; across it the fine edge lines
+// Grids and outputs of the operator-dependent interpolation weights of a transfer's three
+// factors (axis a: a g0[a] x g1[a] x g2[a] grid, weights of operator set z at w64[a] + z *
+// wstride[a] and their fp32 copies at w32[a] + z * wstride[a]).
+struct InterpGrids {
+  double* w64[3];
+  float* w32[3];
+  size_t wstride[3];
+  uint32_t g0[3], g1[3], g2[3];
+};
+
+// Coarse-level conductances from fine ones (blockIdx.y = 0): along the edge direction the (one
+// or two) fine edges between the two coarse nodes act in series; across it the fine edge lines
 // are summed with the interpolation weights of the transverse axes.  The Robin diagonal is
-// restricted like a vector (lumped Galerkin P^T b), so A_c 1 = P^T A_f 1.
+// restricted like a vector (lumped Galerkin P^T b), so A_c 1 = P^T A_f 1.  In the same launch
+// (blockIdx.y = 1 + axis) the operator-dependent interpolation weights of the transfer's
+// factors, which read the same fine conductances.
 __global__ __launch_bounds__(SDC_BLOCK) void k_coarsen_coef(Xfer<double> t,
                                                             const double* __restrict__ fcx,
                                                             const double* __restrict__ fcy,
@@ -1640,7 +1696,20 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coarsen_coef(Xfer<double> t,
                                                             double* __restrict__ ccz,
                                                             double* __restrict__ cbd,
                                                             uint32_t cn1z, size_t fstride,
-                                                            size_t cstride) {
+                                                            size_t cstride, InterpGrids iw) {
+  if (blockIdx.y > 0) {
+    const int ax = (int)blockIdx.y - 1;
+    const size_t m = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
+    if (m >= (size_t)iw.g0[ax] * iw.g1[ax] * iw.g2[ax]) return;
+    const size_t zf = (size_t)blockIdx.z * fstride, zw = (size_t)blockIdx.z * iw.wstride[ax];
+    if (ax == 0)
+      interp_weight<0>(t, m, iw.g0[0], iw.g1[0], fcx + zf, iw.w64[0] + zw, iw.w32[0] + zw);
+    else if (ax == 1)
+      interp_weight<1>(t, m, iw.g0[1], iw.g1[1], fcy + zf, iw.w64[1] + zw, iw.w32[1] + zw);
+    else
+      interp_weight<2>(t, m, iw.g0[2], iw.g1[2], fcz + zf, iw.w64[2] + zw, iw.w32[2] + zw);
+    return;
+  }
   const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
   if (n >= t.cN) return;
   // operator set blockIdx.z: fine arrays + z * fstride, coarse arrays + z * cstride

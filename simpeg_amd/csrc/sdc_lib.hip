@@ -251,6 +251,7 @@ struct sdc_handle_s {
   Work<double> w64;
   int* h_flag = nullptr;  // pinned host copy of the convergence flag
   double* h_ctl = nullptr;  // pinned host copy of the mixed PCG control block
+  int* d_ctl = nullptr;     // its device address: the mixed PCG writes it when it stops
   // recorded finest-level V-cycles (sdc_params.graph): one executable graph per input, output,
   // column count and storage variant, launched on the caller's stream
   struct GraphRec {
@@ -1156,10 +1157,10 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
 // and the fp32 residual is replaced by the fp64 true residual q - A u whenever every active
 // column's fp32 residual passed the tolerance, or some column's residual dropped by
 // reliable_delta since its last replacement.  Columns stop on the true residual only.
-// Every decision is taken on the device (k_decide, k_after_update on the control block W.ctl):
-// an iteration is a fixed sequence of launches whose kernels return at once when the control
-// block says the solve stopped or no update is due, so check_every iterations at a time run
-// without a host synchronisation -- as one recorded graph when sdc_params.graph is set.
+// Every decision is taken on the device (k_fin_decide, k_fin_true_update on the control block
+// W.ctl): an iteration is a fixed sequence of launches whose kernels return at once when the
+// control block says the solve stopped or no update is due, so check_every iterations at a time
+// run without a host synchronisation -- as one recorded graph when sdc_params.graph is set.
 int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sdc_solve_opts* o,
                  sdc_solve_stats& S, hipStream_t st) {
   CK(ensure_work<float>(h, ncol));
@@ -1217,6 +1218,9 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   // the largest true relative residual of the previous one.  This happens when the fp32
   // recursion no longer tracks the true residual (conductivity contrasts of 1e4 and more).
   constexpr int kStallWindow = 25;
+  // the device writes the control block to h_ctl when it stops (d_ctl), and the host reads the
+  // stop flag there after each block of iterations: cleared before the first launch
+  ((int*)h->h_ctl)[SDC_CTL_STOP] = 0;
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, W.ctl, W.qu, q, u, tol2, delta2, maxit,
                      kStallWindow);
   // vcycle_bf16 = 1, 4: the preconditioned residual z is bfloat16
@@ -1293,11 +1297,10 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
       launch(h, k_update_r4<false>, grid4(l0.nN, ncol), blk, 0, s, W.r, (const float*)W.q,
              (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
     prof_end(h, SDC_PROF_UPDATE, s, (double)l0.nN * ncol * (r16 ? 14 : 12));
-    launch(h, k_fin_norm, dim3(ncol), blk, 0, s, (const double*)W.partial, nblk4(l0.nN), W.rr,
-           W.active, 0, stop);
-    launch(h, k_decide, dim3(1), blk, 0, s, W.ctl, (const double*)W.rr, (const double*)W.aux,
-           (const double*)W.rmax, (const int*)W.active, ncol, S_);
-    // reliable update (every kernel below returns at once unless k_decide asked for it).  A
+    // |r|^2 per column, then the reliable-update decision in the last work-group
+    launch(h, k_fin_decide, dim3(ncol), blk, 0, s, (const double*)W.partial, nblk4(l0.nN), W.rr,
+           W.ctl, (const double*)W.aux, (const double*)W.rmax, (const int*)W.active, S_, stop);
+    // reliable update (every kernel below returns at once unless the decision asked for it).  A
     // profiled run brackets the fold and the fp64 residual only when they do work, so it reads
     // the decision on the host (graphs are off while profiling).
     bool upd = true;
@@ -1327,10 +1330,9 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     // algorithmic bytes: fp64 coefficients, per column u64 read and r32 write; q is read on
     // source nodes only (not counted)
     prof_end(h, upd ? SDC_PROF_REFINE : -1, s, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
-    launch(h, k_fin_true, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx_ir,
-           (const double*)W.aux, tol2, W.orr, W.rmax, W.active, (const int*)W.ctl);
-    launch(h, k_after_update, dim3(1), blk, 0, s, W.ctl, (const double*)W.orr,
-           (const double*)W.aux, (const int*)W.active, ncol);
+    // true |q - A u|^2 per column, then the stop rules in the last work-group
+    launch(h, k_fin_true_update, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx_ir,
+           (const double*)W.aux, W.orr, W.rmax, W.active, W.ctl, h->d_ctl);
     return SDC_OK;
   };
   // the first iteration (FIRST variants) is launched; later ones run check_every at a time,
@@ -1351,10 +1353,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
                    block));
     else
       CK(block(st));
-    HIPCK(hipMemcpyAsync(h->h_ctl, W.ctlmem, (kCtlInts + SDC_DCTL_N) * sizeof(double),
-                         hipMemcpyDeviceToHost, st));
+    if (!h->d_ctl)
+      HIPCK(hipMemcpyAsync(h->h_ctl, W.ctlmem, (kCtlInts + SDC_DCTL_N) * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
-    if (h_ctl[SDC_CTL_STOP]) break;
+    if (__atomic_load_n(h_ctl + SDC_CTL_STOP, __ATOMIC_ACQUIRE)) break;
     // the device stops at maxit by itself; this bound only guards the host loop
     if (done_it > maxit + K) return fail(SDC_ERR_STATE, "internal: mixed PCG did not stop");
   }
@@ -1570,6 +1573,12 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
     h->h_ctl = nullptr;
     return bail(fail(SDC_ERR_ALLOC, "hipHostMalloc failed"));
   }
+  // pinned host memory is mapped for the device; without a device address the mixed PCG copies
+  // its control block to the host after every block of iterations instead
+  if (hipHostGetDevicePointer((void**)&h->d_ctl, h->h_ctl, 0) != hipSuccess) {
+    h->d_ctl = nullptr;
+    (void)hipGetLastError();
+  }
   // the fused coarse tail may use all 160 KB of LDS (set once, outside any graph capture)
   if (hipFuncSetAttribute((const void*)k_tail_vcycle<__bf16>,
                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
@@ -1691,39 +1700,46 @@ namespace {
 int build_levels(sdc_handle_s* h, hipStream_t st, int nset) {
   // every launch covers all nset operator sets (grid z, or one work-group per set)
   const uint32_t z = (uint32_t)nset;
+  // coarse conductances and, in the same launch, the operator-dependent interpolation weights
+  // of every transfer (used when sdc_params.interp = 1), which read the fine conductances
   for (size_t l = 0; l + 1 < h->L.size(); ++l) {
     Level& f = h->L[l];
     Level& c = h->L[l + 1];
     const Xfer<double> t = make_xfer<double>(h, (int)l);
-    hipLaunchKernelGGL(k_coarsen_coef, dim3(nblk(c.nN), 1, z), dim3(SDC_BLOCK), 0, st, t,
+    InterpGrids iw{};
+    const uint32_t g[3][3] = {{f.n1x, f.n1y, f.n1z}, {c.n1x, f.n1y, f.n1z}, {c.n1x, c.n1y, f.n1z}};
+    double* const w64[3] = {f.x64.wx, f.x64.wy, f.x64.wz};
+    float* const w32[3] = {f.x32.wx, f.x32.wy, f.x32.wz};
+    const size_t ws[3] = {f.x64.wxs, f.x64.wys, f.x64.wzs};
+    size_t nb = nblk(c.nN);
+    for (int a = 0; a < 3; ++a) {
+      iw.g0[a] = g[a][0];
+      iw.g1[a] = g[a][1];
+      iw.g2[a] = g[a][2];
+      iw.w64[a] = w64[a];
+      iw.w32[a] = w32[a];
+      iw.wstride[a] = ws[a];
+      nb = std::max(nb, (size_t)nblk((size_t)g[a][0] * g[a][1] * g[a][2]));
+    }
+    hipLaunchKernelGGL(k_coarsen_coef, dim3((uint32_t)nb, 4, z), dim3(SDC_BLOCK), 0, st, t,
                        (const double*)f.cx64, (const double*)f.cy64, (const double*)f.cz64,
                        (const double*)f.bd64, c.cx64, c.cy64, c.cz64, c.bd64, c.n1z, f.sstride,
-                       c.sstride);
+                       c.sstride, iw);
   }
-  // operator-dependent interpolation weights of every transfer (used when sdc_params.interp = 1)
-  for (size_t l = 0; l + 1 < h->L.size(); ++l) {
-    Level& f = h->L[l];
-    const Level& c = h->L[l + 1];
-    const Xfer<double> t = make_xfer<double>(h, (int)l);
-    const size_t nwx = f.nN, nwy = (size_t)c.n1x * f.n1y * f.n1z,
-                 nwz = (size_t)c.n1x * c.n1y * f.n1z;
-    hipLaunchKernelGGL(k_interp_weights<0>, dim3(nblk(nwx), 1, z), dim3(SDC_BLOCK), 0, st, t,
-                       f.n1x, f.n1y, f.n1z, (const double*)f.cx64, f.x64.wx, f.x32.wx, f.sstride,
-                       f.x64.wxs);
-    hipLaunchKernelGGL(k_interp_weights<1>, dim3(nblk(nwy), 1, z), dim3(SDC_BLOCK), 0, st, t,
-                       c.n1x, f.n1y, f.n1z, (const double*)f.cy64, f.x64.wy, f.x32.wy, f.sstride,
-                       f.x64.wys);
-    hipLaunchKernelGGL(k_interp_weights<2>, dim3(nblk(nwz), 1, z), dim3(SDC_BLOCK), 0, st, t,
-                       c.n1x, c.n1y, f.n1z, (const double*)f.cz64, f.x64.wz, f.x32.wz, f.sstride,
-                       f.x64.wzs);
-  }
+  // diagonals and fp32 copies of all levels, SDC_DIAG_LEVELS levels per launch
   const int pin = h->bc == SDC_BC_NEUMANN ? 1 : 0;
-  for (auto& v : h->L) {
-    const Coef32 c32{v.cx32, v.cy32, v.cz32, v.dg32, v.idg32};
-    hipLaunchKernelGGL(k_coef_diag, dim3(nblk(v.nN), 1, z), dim3(SDC_BLOCK), 0, st, v.n1x, v.n1y,
-                       v.nN, make_fastdiv(v.n1x), make_fastdiv(v.n1y), (const double*)v.cx64,
-                       (const double*)v.cy64, (const double*)v.cz64, (const double*)v.bd64,
-                       v.dg64, v.idg64, pin, v.sstride, c32);
+  for (size_t l0 = 0; l0 < h->L.size(); l0 += SDC_DIAG_LEVELS) {
+    DiagLevels tab{};
+    uint32_t nb = 0;
+    for (size_t l = l0; l < h->L.size() && tab.n < SDC_DIAG_LEVELS; ++l) {
+      Level& v = h->L[l];
+      tab.lv[tab.n++] = DiagLevel{v.n1x, v.n1y, v.nN, nb, make_fastdiv(v.n1x),
+                                  make_fastdiv(v.n1y), v.cx64, v.cy64, v.cz64, v.bd64, v.dg64,
+                                  v.idg64, v.sstride,
+                                  Coef32{v.cx32, v.cy32, v.cz32, v.dg32, v.idg32}, pin};
+      nb += (uint32_t)nblk(v.nN);
+    }
+    hipLaunchKernelGGL(k_coef_diag, dim3(nb, 1, z), dim3(SDC_BLOCK), 0, st, tab);
   }
   // coarsest level: dense inverse
   const int lc = (int)h->L.size() - 1;
@@ -1987,10 +2003,11 @@ int sdc_jvec_rhs(sdc_handle h, const double* dsig, const double* u, double* out,
   const Geo geo = make_geo(h, true);
   hipLaunchKernelGGL(k_coef_from_sigma, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, geo, dsig,
                      h->tcx, h->tcy, h->tcz, h->tbd, -1.0, (size_t)0, (size_t)0);
-  hipLaunchKernelGGL(k_coef_diag, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, l0.n1x, l0.n1y,
-                     l0.nN, make_fastdiv(l0.n1x), make_fastdiv(l0.n1y), (const double*)h->tcx,
-                     (const double*)h->tcy, (const double*)h->tcz, (const double*)h->tbd, h->tdg,
-                     h->tidg, 0, (size_t)0, Coef32{});
+  DiagLevels tab{};
+  tab.n = 1;
+  tab.lv[0] = DiagLevel{l0.n1x, l0.n1y, l0.nN, 0, make_fastdiv(l0.n1x), make_fastdiv(l0.n1y),
+                        h->tcx, h->tcy, h->tcz, h->tbd, h->tdg, h->tidg, 0, Coef32{}, 0};
+  hipLaunchKernelGGL(k_coef_diag, dim3(nblk(l0.nN)), dim3(SDC_BLOCK), 0, st, tab);
   Sten<double> s{};
   s.cx = h->tcx; s.cy = h->tcy; s.cz = h->tcz; s.dg = h->tdg; s.idg = h->tidg;
   s.n1x = l0.n1x; s.n1y = l0.n1y; s.n1z = l0.n1z; s.nN = l0.nN; s.plane = l0.plane;
