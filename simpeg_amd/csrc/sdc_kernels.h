@@ -79,7 +79,6 @@ __device__ __forceinline__ bool set_changes(const Sten<T>& sa, int c, int cprev)
 #define SDC_CTL_MAXIT 8
 #define SDC_CTL_WINDOW 9
 #define SDC_CTL_FOLD 10
-#define SDC_CTL_TICKET 11  // last-work-group counter of the finalisers (zero between launches)
 #define SDC_CTL_N 16
 // doubles after the ints: the largest true relative residual of the previous update and of the
 // last one, tol^2 and reliable_delta^2
@@ -104,24 +103,6 @@ __device__ __forceinline__ bool ctl_update(const int* ctl) {
 __device__ __forceinline__ bool ctl_fold(const int* ctl) {
   return ld_flag(ctl + SDC_CTL_STOP) == 0 &&
          (ld_flag(ctl + SDC_CTL_UPD) != 0 || ld_flag(ctl + SDC_CTL_FOLD) != 0);
-}
-
-// The work-group that finishes last among `count` work-groups sharing a ticket counter: each
-// publishes its results from thread 0, which then draws a ticket (the release orders its
-// stores before the ticket); the one that draws the last sees the others' results (read them
-// with device-scope loads) and resets the counter for the next launch.  A launch whose
-// work-groups all return before drawing leaves the counter at zero.  Each ticket costs a
-// write-back and invalidation of the XCD's L2: only for launches of few work-groups.
-__device__ __forceinline__ bool last_of(int* ticket, uint32_t count) {
-  __shared__ int is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = t == (int)count - 1;
-    if (is_last) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  return is_last != 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -925,20 +906,20 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z*
   block_partials<R * G>(acc, partial, c0, ncol, nbx, bx);
 }
 
-// r -= alpha q and per-column |r|^2 partials (4 nodes per work-item)
+// r -= alpha q and per-column |r|^2 partials (4 nodes per work-item, NT work-items per
+// work-group: large work-groups, so that the single-work-group decision sums few partials)
 // (R16: also write a bfloat16 copy of r, the multigrid input)
-template <bool R16>
-__global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
-                                                         const float* __restrict__ q,
-                                                         const double* __restrict__ alpha,
-                                                         size_t ld, uint32_t nN,
-                                                         double* __restrict__ partial,
-                                                         __bf16* __restrict__ r16,
-                                                         const int* __restrict__ done) {
+template <bool R16, int NT>
+__global__ __launch_bounds__(NT) void k_update_r4(float* __restrict__ r,
+                                                  const float* __restrict__ q,
+                                                  const double* __restrict__ alpha, size_t ld,
+                                                  uint32_t nN, double* __restrict__ partial,
+                                                  __bf16* __restrict__ r16,
+                                                  const int* __restrict__ done) {
   if (done && ld_flag(done)) return;
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
-  const uint32_t n0 = (blockIdx.x * SDC_BLOCK + threadIdx.x) * 4;
+  const uint32_t n0 = (blockIdx.x * NT + threadIdx.x) * 4;
   const float al = (float)alpha[c];
   double acc = 0.0;
   if (n0 + 3 < nN) {
@@ -964,8 +945,17 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
       acc += (double)rn * rn;
     }
   }
-  const double t = block_reduce1(acc);
-  if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
+  __shared__ double red[NT / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) t += red[w];
+    partial[(size_t)c * gridDim.x + blockIdx.x] = t;
+  }
 }
 
 #define SDC_FOLD_CHUNKS 4
@@ -1020,26 +1010,6 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_ir_fold_ring(const double* const*
         u[o + n] = v;
       }
     }
-  }
-}
-
-// Start of a device-driven solve: the control block with the solve's parameters, and q, u in
-// the pointer table.
-__global__ void k_ctl_init(int* __restrict__ ctl, const double** __restrict__ qu,
-                           const double* q, double* u, double tol2, double delta2, int maxit,
-                           int window) {
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < SDC_CTL_N; ++i) ctl[i] = 0;
-    ctl[SDC_CTL_UFRESH] = 1;
-    ctl[SDC_CTL_MAXIT] = maxit;
-    ctl[SDC_CTL_WINDOW] = window;
-    double* d = ctl_dbl(ctl);
-    d[SDC_DCTL_MPREV] = 1.0;
-    d[SDC_DCTL_MREL] = 0.0;
-    d[SDC_DCTL_TOL2] = tol2;
-    d[SDC_DCTL_DELTA2] = delta2;
-    qu[0] = q;
-    qu[1] = u;
   }
 }
 
@@ -1118,26 +1088,45 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_norm(const double* __restrict
   }
 }
 
-// Reliable-update decision after an iteration of the mixed PCG, in the last work-group of
-// k_fin_decide: counts the iteration and its search direction (now pending in the ring) and
-// asks for an update (replace the fp32 residual by the fp64 true residual, fold the pending
-// directions into u) when every active column's fp32 residual |r|^2 passed tol^2 |q|^2, some
-// active column's dropped by delta^2 since its last replacement (rmax), the iteration limit is
-// reached, or `window` iterations passed without an update.  Otherwise, when all nslot slots of
-// the ring are pending, it asks for a fold alone (the next iteration writes the oldest slot).
-__device__ void decide(int* __restrict__ ctl, const double* __restrict__ rr,
-                       const double* __restrict__ qn2, const double* __restrict__ rmax,
-                       const int* __restrict__ active, int ncol, int nslot) {
+// The mixed PCG's per-iteration finalisers run as one work-group of SDC_FIN_THREADS: a wave
+// sums a column's block partials (lane-strided, then a butterfly; a fixed order), so the
+// partials come from kernels that write few per column.
+#define SDC_FIN_THREADS 1024
+
+__device__ __forceinline__ double wave_sum(const double* __restrict__ p, uint32_t n) {
+  double v = 0.0;
+  for (uint32_t b = threadIdx.x & 63; b < n; b += 64) v += p[b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// |r|^2 per column (rr) from the update's partials, then the reliable-update decision: counts
+// the iteration and its search direction (now pending in the ring) and asks for an update
+// (replace the fp32 residual by the fp64 true residual, fold the pending directions into u)
+// when every active column's |r|^2 passed tol^2 |q|^2, some active column's dropped by
+// delta^2 since its last replacement (rmax), the iteration limit is reached, or `window`
+// iterations passed without an update.  Otherwise, when all nslot slots of the ring are
+// pending, it asks for a fold alone (the next iteration writes the oldest slot).
+__global__ __launch_bounds__(SDC_FIN_THREADS) void k_norm_decide(const double* __restrict__ partial,
+                                                                 uint32_t nbx,
+                                                                 double* __restrict__ rr,
+                                                                 int* __restrict__ ctl,
+                                                                 const double* __restrict__ qn2,
+                                                                 const double* __restrict__ rmax,
+                                                                 const int* __restrict__ active,
+                                                                 int ncol, int nslot) {
   if (ld_flag(ctl + SDC_CTL_STOP)) return;
   const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
   const double delta2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_DELTA2);
   const int maxit = ld_flag(ctl + SDC_CTL_MAXIT), window = ld_flag(ctl + SDC_CTL_WINDOW);
   int cand = 1, drop = 0;
-  for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
+  for (int c = (int)threadIdx.x >> 6; c < ncol; c += SDC_FIN_THREADS / 64) {
+    const double v = wave_sum(partial + (size_t)c * nbx, nbx);
+    if ((threadIdx.x & 63) == 0) rr[c] = v;
     if (!active[c]) continue;
-    const double r = ld_flag(rr + c);
-    if (r > tol2 * qn2[c]) cand = 0;
-    if (r <= delta2 * rmax[c]) drop = 1;
+    if (v > tol2 * qn2[c]) cand = 0;
+    if (v <= delta2 * rmax[c]) drop = 1;
   }
   cand = __syncthreads_and(cand);
   drop = __syncthreads_or(drop);
@@ -1153,81 +1142,61 @@ __device__ void decide(int* __restrict__ ctl, const double* __restrict__ rr,
   }
 }
 
-// rr[c] = |r_c|^2 from the update's per-block partials (one work-group per column), then the
-// reliable-update decision in the work-group that finishes last.
-__global__ __launch_bounds__(SDC_BLOCK) void k_fin_decide(const double* __restrict__ partial,
-                                                          uint32_t nbx, double* __restrict__ rr,
-                                                          int* __restrict__ ctl,
-                                                          const double* __restrict__ qn2,
-                                                          const double* __restrict__ rmax,
-                                                          const int* __restrict__ active,
-                                                          int nslot,
-                                                          const int* __restrict__ done) {
-  if (done && ld_flag(done)) return;
-  const int c = blockIdx.x;
-  const double v = block_sum(partial + (size_t)c * nbx, nbx);
-  if (threadIdx.x == 0) rr[c] = v;
-  if (!last_of(ctl + SDC_CTL_TICKET, gridDim.x)) return;
-  decide(ctl, rr, qn2, rmax, active, (int)gridDim.x, nslot);
-}
-
-// After a fold: the ring is empty and u is written.  After a reliable update (one work-group
-// per column) also: the true residual |q - A u|^2 per column from the fp64 residual's
-// partials (rr, rmax, and active = rr > tol^2 |q|^2), and, in the work-group that finishes
-// last, the stop rules: the solve stops when no column is open (converged), at the iteration
-// limit, or when the largest true relative residual failed to halve since the previous update
-// (the fp32 recursion stalled; the caller continues in fp64).  At the stop the control block
-// is also written to hctl (pinned host memory, may be null) for the host to read.
-__global__ __launch_bounds__(SDC_BLOCK) void k_fin_true_update(const double* __restrict__ partial,
-                                                               uint32_t nbx,
-                                                               const double* __restrict__ qn2,
-                                                               double* __restrict__ rr,
-                                                               double* __restrict__ rmax,
-                                                               int* __restrict__ active,
-                                                               int* __restrict__ ctl,
-                                                               int* __restrict__ hctl) {
+// After a fold: the ring is empty and u is written.  After a reliable update also: the true
+// residual per column from the fp64 residual's partials (orr = rmax = |q - A u|^2, active =
+// orr > tol^2 |q|^2), then the stop rules: the solve stops when no column is open (converged),
+// at the iteration limit, or when the largest true relative residual failed to halve since the
+// previous update (the fp32 recursion stalled; the caller continues in fp64).  At the stop the
+// control block is also written to hctl (pinned host memory, may be null) for the host to read.
+__global__ __launch_bounds__(SDC_FIN_THREADS) void k_true_after(const double* __restrict__ partial,
+                                                                uint32_t nbx,
+                                                                const double* __restrict__ qn2,
+                                                                double* __restrict__ orr,
+                                                                double* __restrict__ rmax,
+                                                                int* __restrict__ active,
+                                                                int* __restrict__ ctl, int ncol,
+                                                                int* __restrict__ hctl) {
   if (!ctl_fold(ctl)) return;
   if (ld_flag(ctl + SDC_CTL_UPD) == 0) {  // a fold alone (full ring)
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (threadIdx.x == 0) {
       ctl[SDC_CTL_PEND] = 0;
       ctl[SDC_CTL_FOLD] = 0;
       ctl[SDC_CTL_UFRESH] = 0;
     }
     return;
   }
-  const int ncol = (int)gridDim.x;
-  {
-    const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
-    const int c = blockIdx.x;
-    const double v = block_sum(partial + (size_t)c * nbx, nbx);
-    if (threadIdx.x == 0) {
-      rr[c] = v;
-      rmax[c] = v;
-      active[c] = (qn2[c] > 0.0 && v > tol2 * qn2[c]) ? 1 : 0;
-    }
-  }
-  if (!last_of(ctl + SDC_CTL_TICKET, gridDim.x)) return;
+  const double tol2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_TOL2);
   const int maxit = ld_flag(ctl + SDC_CTL_MAXIT);
-  __shared__ double sm[SDC_BLOCK / 64];
-  __shared__ int sn[SDC_BLOCK / 64];
+  constexpr int NW = SDC_FIN_THREADS / 64;
+  __shared__ double sm[NW];
+  __shared__ int sn[NW];
+  const int lane = threadIdx.x & 63;
   double m = 0.0;
   int nact = 0;
-  for (int c = threadIdx.x; c < ncol; c += SDC_BLOCK) {
-    nact += ld_flag(active + c) != 0;
-    if (qn2[c] > 0.0) m = fmax(m, sqrt(ld_flag(rr + c) / qn2[c]));
+  for (int c = (int)threadIdx.x >> 6; c < ncol; c += NW) {
+    const double v = wave_sum(partial + (size_t)c * nbx, nbx);
+    const double q2 = qn2[c];
+    const int a = (q2 > 0.0 && v > tol2 * q2) ? 1 : 0;
+    if (lane == 0) {
+      orr[c] = v;
+      rmax[c] = v;
+      active[c] = a;
+      nact += a;
+    }
+    if (q2 > 0.0) m = fmax(m, sqrt(v / q2));
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     m = fmax(m, __shfl_xor(m, off, 64));
     nact += __shfl_xor(nact, off, 64);
   }
-  if ((threadIdx.x & 63) == 0) {
+  if (lane == 0) {
     sm[threadIdx.x >> 6] = m;
     sn[threadIdx.x >> 6] = nact;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < SDC_BLOCK / 64; ++w) {
+    for (int w = 1; w < NW; ++w) {
       m = fmax(m, sm[w]);
       nact += sn[w];
     }
@@ -1262,6 +1231,47 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_true_update(const double* __r
       dctl[SDC_DCTL_MPREV] = m;
       ctl[SDC_CTL_LAST] = it;
     }
+  }
+}
+
+// Start of the mixed PCG: the node mask of non-zero right-hand sides cleared (work-groups over
+// the nodes), and in work-group 0 the control block with the solve's parameters and q, u in the
+// pointer table.
+__global__ __launch_bounds__(SDC_BLOCK) void k_mixed_init(unsigned* __restrict__ qmask,
+                                                          uint32_t nN, int* __restrict__ ctl,
+                                                          const double** __restrict__ qu,
+                                                          const double* q, double* u,
+                                                          double tol2, double delta2, int maxit,
+                                                          int window) {
+  const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
+  if (n < nN) qmask[n] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int i = 0; i < SDC_CTL_N; ++i) ctl[i] = 0;
+    ctl[SDC_CTL_UFRESH] = 1;
+    ctl[SDC_CTL_MAXIT] = maxit;
+    ctl[SDC_CTL_WINDOW] = window;
+    double* d = ctl_dbl(ctl);
+    d[SDC_DCTL_MPREV] = 1.0;
+    d[SDC_DCTL_MREL] = 0.0;
+    d[SDC_DCTL_TOL2] = tol2;
+    d[SDC_DCTL_DELTA2] = delta2;
+    qu[0] = q;
+    qu[1] = u;
+  }
+}
+
+// |q|^2 per column into qn2 and rmax (the residual at the last replacement: q, u = 0) and
+// active = |q| > 0, from the first residual's partials (one work-group per column)
+__global__ __launch_bounds__(SDC_BLOCK) void k_fin_qnorm(const double* __restrict__ partial,
+                                                         uint32_t nbx, double* __restrict__ qn2,
+                                                         double* __restrict__ rmax,
+                                                         int* __restrict__ active) {
+  const int c = blockIdx.x;
+  const double v = block_sum(partial + (size_t)c * nbx, nbx);
+  if (threadIdx.x == 0) {
+    qn2[c] = v;
+    rmax[c] = v;
+    active[c] = (v > 0.0) ? 1 : 0;
   }
 }
 
@@ -1794,63 +1804,43 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coarsen_coef(Xfer<double> t,
   cbd[n] = sb;
 }
 
-// dense operator of the coarsest level (row-major nn x nn), pin = 1: identity row/column 0;
-// operator set blockIdx.z: coefficients + z * s.sstride, matrix + z * astride
-__global__ __launch_bounds__(SDC_BLOCK) void k_dense_build(Sten<double> s, double* __restrict__ a,
-                                                           size_t astride) {
-  const uint32_t n = blockIdx.x * SDC_BLOCK + threadIdx.x;
-  if (n >= s.nN) return;
-  const size_t zo = (size_t)blockIdx.z * s.sstride;
+// Inverse of the coarsest level's operator, one work-group of 1024 per operator set (blockIdx.x:
+// coefficients + set * s.sstride, matrices + set * astride, scratch + set * nn):
+//   1. the dense operator a (row-major nn x nn; pin = 1: identity row / column 0), one row per
+//      work-item;
+//   2. in-place Gauss-Jordan elimination without pivoting (the operator is SPD), every pivot
+//      step updating all nn*nn entries of a and inv in parallel (f: the elimination factors);
+//   3. inv <- (inv + inv^T) / 2 and its fp32 copy inv32.
+__global__ __launch_bounds__(1024) void k_coarse_inverse(Sten<double> s, double* __restrict__ a,
+                                                         double* __restrict__ inv,
+                                                         float* __restrict__ inv32,
+                                                         double* __restrict__ f,
+                                                         size_t astride) {
+  const size_t zo = (size_t)blockIdx.x * s.sstride;
   s.cx += zo;
   s.cy += zo;
   s.cz += zo;
   s.dg += zo;
   s.idg += zo;
-  a += (size_t)blockIdx.z * astride;
-  const size_t N = s.nN;
-  double* row = a + (size_t)n * N;
-  for (size_t m = 0; m < N; ++m) row[m] = 0.0;
-  const NodeC<double> c = node_coef(s, n);
-  row[n] = c.d;
-  if (c.ixm != n) row[c.ixm] = -c.wxm;
-  if (c.ixp != n) row[c.ixp] = -c.wxp;
-  if (c.iym != n) row[c.iym] = -c.wym;
-  if (c.iyp != n) row[c.iyp] = -c.wyp;
-  if (c.izm != n) row[c.izm] = -c.wzm;
-  if (c.izp != n) row[c.izp] = -c.wzp;
-}
-
-// ainv64 <- (ainv64 + ainv64^T)/2 and its fp32 copy
-__global__ __launch_bounds__(SDC_BLOCK) void k_symmetrize(double* __restrict__ a,
-                                                          float* __restrict__ a32, int nn,
-                                                          size_t astride) {
-  const size_t e = (size_t)blockIdx.x * SDC_BLOCK + threadIdx.x;
-  const size_t N = nn;
-  if (e >= N * N) return;
-  a += (size_t)blockIdx.y * astride;  // operator set blockIdx.y
-  a32 += (size_t)blockIdx.y * astride;
-  const size_t r = e / N, c = e % N;
-  if (c < r) return;
-  const double v = 0.5 * (a[r * N + c] + a[c * N + r]);
-  a[r * N + c] = v;
-  a[c * N + r] = v;
-  a32[r * N + c] = (float)v;
-  a32[c * N + r] = (float)v;
-}
-
-// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one work-group of 1024 per
-// operator set (blockIdx.x; matrices + set * astride, scratch + set * nn).
-// a: nn x nn (destroyed), inv: nn x nn output, f: nn scratch for the elimination factors.
-// Each pivot step updates all nn*nn entries of both matrices in parallel.
-__global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ a, double* __restrict__ inv,
-                                                       double* __restrict__ f, int nn,
-                                                       size_t astride) {
   a += (size_t)blockIdx.x * astride;
   inv += (size_t)blockIdx.x * astride;
-  f += (size_t)blockIdx.x * nn;
+  inv32 += (size_t)blockIdx.x * astride;
+  const int N = (int)s.nN;
+  f += (size_t)blockIdx.x * N;
   const int tid = threadIdx.x;
-  const int N = nn;
   const int NN = N * N;
+  for (int n = tid; n < N; n += 1024) {
+    double* row = a + (size_t)n * N;
+    for (int m = 0; m < N; ++m) row[m] = 0.0;
+    const NodeC<double> c = node_coef(s, (uint32_t)n);
+    row[n] = c.d;
+    if (c.ixm != (uint32_t)n) row[c.ixm] = -c.wxm;
+    if (c.ixp != (uint32_t)n) row[c.ixp] = -c.wxp;
+    if (c.iym != (uint32_t)n) row[c.iym] = -c.wym;
+    if (c.iyp != (uint32_t)n) row[c.iyp] = -c.wyp;
+    if (c.izm != (uint32_t)n) row[c.izm] = -c.wzm;
+    if (c.izp != (uint32_t)n) row[c.izp] = -c.wzp;
+  }
   for (int e = tid; e < NN; e += 1024) inv[e] = ((e / N) == (e % N)) ? 1.0 : 0.0;
   __syncthreads();
   for (int p = 0; p < N; ++p) {
@@ -1873,6 +1863,16 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ a, d
       }
     }
     __syncthreads();
+  }
+  // each unordered pair (r, c) is read and written by the one work-item with c >= r
+  for (int e = tid; e < NN; e += 1024) {
+    const int r = e / N, c = e % N;
+    if (c < r) continue;
+    const double v = 0.5 * (inv[r * N + c] + inv[c * N + r]);
+    inv[r * N + c] = v;
+    inv[c * N + r] = v;
+    inv32[r * N + c] = (float)v;
+    inv32[c * N + r] = (float)v;
   }
 }
 

@@ -1157,7 +1157,7 @@ int pcg(sdc_handle_s* h, T* x, int ncol, double tol, int maxit, const int* col_m
 // and the fp32 residual is replaced by the fp64 true residual q - A u whenever every active
 // column's fp32 residual passed the tolerance, or some column's residual dropped by
 // reliable_delta since its last replacement.  Columns stop on the true residual only.
-// Every decision is taken on the device (k_fin_decide, k_fin_true_update on the control block
+// Every decision is taken on the device (k_norm_decide, k_true_after on the control block
 // W.ctl): an iteration is a fixed sequence of launches whose kernels return at once when the
 // control block says the solve stopped or no update is due, so check_every iterations at a time
 // run without a host synchronisation -- as one recorded graph when sdc_params.graph is set.
@@ -1173,6 +1173,10 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const uint32_t nbx = nblk(l0.nN);
   constexpr int kIrChunks = 4;
   const uint32_t nbx_ir = (nbx + kIrChunks - 1) / kIrChunks;
+  // r update: work-groups of kUpdThreads work-items (4 nodes each), so that k_norm_decide sums
+  // few partials per column
+  constexpr int kUpdThreads = 1024;
+  const dim3 gupd((uint32_t)((l0.nN + 4 * kUpdThreads - 1) / (4 * kUpdThreads)), (uint32_t)ncol);
   const size_t len = l0.ld * (size_t)ncol;
   PinFix pf;
   pf.on = h->bc == SDC_BC_NEUMANN;
@@ -1187,8 +1191,18 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const double tol2 = o->tol * o->tol;
   const double delta2 = h->prm.reliable_delta * h->prm.reliable_delta;
   const int maxit = o->max_iter;
-  // u is not cleared: the first fold writes it (and its column padding) without reading it
-  hipLaunchKernelGGL(k_zero<unsigned>, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, (size_t)l0.nN);
+  // Stall guard: a reliable update is forced after kStallWindow iterations without one, and the
+  // solve gives up (converged = 0, the caller continues in fp64) when an update fails to halve
+  // the largest true relative residual of the previous one.  This happens when the fp32
+  // recursion no longer tracks the true residual (conductivity contrasts of 1e4 and more).
+  constexpr int kStallWindow = 25;
+  // the device writes the control block to h_ctl when it stops (d_ctl), and the host reads the
+  // stop flag there after each block of iterations: cleared before the first launch
+  ((int*)h->h_ctl)[SDC_CTL_STOP] = 0;
+  // the right-hand-side node mask cleared and the control block set up in one launch; u is not
+  // cleared: the first fold writes it (and its column padding) without reading it
+  hipLaunchKernelGGL(k_mixed_init, dim3(nblk(l0.nN)), blk, 0, st, W.qmask, l0.nN, W.ctl, W.qu, q,
+                     u, tol2, delta2, maxit, kStallWindow);
   // vcycle_bf16 = 3, 4: the multigrid reads a bfloat16 copy of r, written with r
   __bf16* const r16 =
       ((h->prm.vcycle_bf16 == 3 || h->prm.vcycle_bf16 == 4) && h->L.size() > 1) ? W.r16 : nullptr;
@@ -1196,10 +1210,8 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
                                    (const double*)u, W.r, l0.ld, ncol, pf, W.partial, W.qmask, r16,
                                    (const int*)nullptr, (const double* const*)nullptr));
   // aux = |q|^2 and active = |q| > 0; rmax = |q|^2 (the residual at the last replacement)
-  hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
-                     W.active, 1, (const int*)nullptr);
-  hipLaunchKernelGGL(k_fin_norm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.rmax,
-                     W.active, 0, (const int*)nullptr);
+  hipLaunchKernelGGL(k_fin_qnorm, dim3(ncol), blk, 0, st, (const double*)W.partial, nbx, W.aux,
+                     W.rmax, W.active);
   std::vector<double> h_qn(ncol);
   HIPCK(hipMemcpyAsync(h_qn.data(), W.aux, ncol * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCK(hipStreamSynchronize(st));
@@ -1213,16 +1225,6 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(len)), blk, 0, st, u, len);
     return SDC_OK;
   }
-  // Stall guard: a reliable update is forced after kStallWindow iterations without one, and the
-  // solve gives up (converged = 0, the caller continues in fp64) when an update fails to halve
-  // the largest true relative residual of the previous one.  This happens when the fp32
-  // recursion no longer tracks the true residual (conductivity contrasts of 1e4 and more).
-  constexpr int kStallWindow = 25;
-  // the device writes the control block to h_ctl when it stops (d_ctl), and the host reads the
-  // stop flag there after each block of iterations: cleared before the first launch
-  ((int*)h->h_ctl)[SDC_CTL_STOP] = 0;
-  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, W.ctl, W.qu, q, u, tol2, delta2, maxit,
-                     kStallWindow);
   // vcycle_bf16 = 1, 4: the preconditioned residual z is bfloat16
   const bool zb = (h->prm.vcycle_bf16 == 1 || h->prm.vcycle_bf16 == 4) && h->L.size() > 1;
   __bf16* const z16 = (__bf16*)W.xb0;
@@ -1291,16 +1293,16 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
            W.ahist + (size_t)phi * W.ncap);
     prof_begin(h, SDC_PROF_UPDATE, s);
     if (r16)
-      launch(h, k_update_r4<true>, grid4(l0.nN, ncol), blk, 0, s, W.r, (const float*)W.q,
-             (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
+      launch(h, k_update_r4<true, kUpdThreads>, gupd, dim3(kUpdThreads), 0, s, W.r,
+             (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
     else
-      launch(h, k_update_r4<false>, grid4(l0.nN, ncol), blk, 0, s, W.r, (const float*)W.q,
-             (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
+      launch(h, k_update_r4<false, kUpdThreads>, gupd, dim3(kUpdThreads), 0, s, W.r,
+             (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
     prof_end(h, SDC_PROF_UPDATE, s, (double)l0.nN * ncol * (r16 ? 14 : 12));
-    // |r|^2 per column, then the reliable-update decision in the last work-group
-    launch(h, k_fin_decide, dim3(ncol), blk, 0, s, (const double*)W.partial, nblk4(l0.nN), W.rr,
-           W.ctl, (const double*)W.aux, (const double*)W.rmax, (const int*)W.active, S_, stop);
-    // reliable update (every kernel below returns at once unless the decision asked for it).  A
+    launch(h, k_norm_decide, dim3(1), dim3(SDC_FIN_THREADS), 0, s, (const double*)W.partial,
+           gupd.x, W.rr, W.ctl, (const double*)W.aux, (const double*)W.rmax,
+           (const int*)W.active, ncol, S_);
+    // reliable update (every kernel below returns at once unless k_norm_decide asked for it).  A
     // profiled run brackets the fold and the fp64 residual only when they do work, so it reads
     // the decision on the host (graphs are off while profiling).
     bool upd = true;
@@ -1330,9 +1332,8 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
     // algorithmic bytes: fp64 coefficients, per column u64 read and r32 write; q is read on
     // source nodes only (not counted)
     prof_end(h, upd ? SDC_PROF_REFINE : -1, s, (double)l0.nN * 32 + (double)l0.nN * ncol * 12);
-    // true |q - A u|^2 per column, then the stop rules in the last work-group
-    launch(h, k_fin_true_update, dim3(ncol), blk, 0, s, (const double*)W.partial, nbx_ir,
-           (const double*)W.aux, W.orr, W.rmax, W.active, W.ctl, h->d_ctl);
+    launch(h, k_true_after, dim3(1), dim3(SDC_FIN_THREADS), 0, s, (const double*)W.partial,
+           nbx_ir, (const double*)W.aux, W.orr, W.rmax, W.active, W.ctl, ncol, h->d_ctl);
     return SDC_OK;
   };
   // the first iteration (FIRST variants) is launched; later ones run check_every at a time,
@@ -1745,12 +1746,9 @@ int build_levels(sdc_handle_s* h, hipStream_t st, int nset) {
   const int lc = (int)h->L.size() - 1;
   const Sten<double> sc = make_sten<double>(h, lc, 1);
   const size_t nc = h->ncoarse;
-  hipLaunchKernelGGL(k_dense_build, dim3(nblk(nc), 1, z), dim3(SDC_BLOCK), 0, st, sc, h->gj_a,
-                     h->astride);
-  hipLaunchKernelGGL(k_gauss_jordan, dim3(z), dim3(1024), 0, st, h->gj_a, h->ainv64, h->gj_f,
-                     (int)nc, h->astride);
-  hipLaunchKernelGGL(k_symmetrize, dim3(nblk(nc * nc), z), dim3(SDC_BLOCK), 0, st, h->ainv64,
-                     h->ainv32, (int)nc, h->astride);
+  if (sc.nN != nc) return fail(SDC_ERR_STATE, "internal: coarsest level size");
+  hipLaunchKernelGGL(k_coarse_inverse, dim3(z), dim3(1024), 0, st, sc, h->gj_a, h->ainv64,
+                     h->ainv32, h->gj_f, h->astride);
   HIPCK(hipGetLastError());
   h->nset = nset;
   h->sigma_set = true;

@@ -1,10 +1,9 @@
-"""The mixed PCG's per-iteration finalisers with their decisions in the last work-group: the
-column norms then the reliable-update decision (k_fin_decide), the true residuals then the stop
-rules (k_fin_true_update), both drawing tickets from a counter in the control block, and the
-control block written to pinned host memory at the stop.  Repeated solves on one workspace
-(the counter must be back at zero after each launch), with another column count in between,
-must give the same bits and match the oracle's direct solve; launched one by one and recorded
-as graphs, one and several iterations per host read."""
+"""The mixed PCG's device-side control block as the host sees it: the device writes it into
+pinned host memory when the solve stops (the host reads the stop flag there after each block of
+iterations, no copy per read).  Repeated solves on one workspace, with another column count in
+between, must stop at the same iteration with the same statistics and bits, and match the
+oracle's direct solve; launched one by one and recorded as graphs, one and several iterations
+per host read."""
 
 import numpy as np
 import pytest
@@ -17,7 +16,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("graph", [1, 0])
 @pytest.mark.parametrize("check_every", [1, 3])
 @pytest.mark.parametrize("bc", ["Robin", "Neumann"])
-def test_repeated_solves_with_last_group_finalisers(gpu, bc, check_every, graph):
+def test_repeated_solves_read_the_stop_block_from_host_memory(gpu, bc, check_every, graph):
     from simpeg_amd import maps
     from simpeg_amd.resistivity import Simulation3DNodal
 
@@ -29,7 +28,7 @@ def test_repeated_solves_with_last_group_finalisers(gpu, bc, check_every, graph)
     ns = survey.nSrc
     eng.set_params(check_every=check_every, graph=graph)
     u1, r1 = eng.solve(q, ns, mode="mixed", tol=1e-10)
-    # fewer columns: fewer work-groups draw tickets
+    # fewer columns in between
     u3, r3 = eng.solve(q[:3].clone(), 3, mode="mixed", tol=1e-10)
     u2, r2 = eng.solve(q, ns, mode="mixed", tol=1e-10)
     assert r1.converged and r2.converged and r3.converged

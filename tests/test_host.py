@@ -251,6 +251,14 @@ def test_meta_adjoint_sum_matches_sparse_transpose():
     parts = [rng.standard_normal(mp.shape[0]) for mp in mps]
     ref = sum(np.asarray(mp.deriv(model).T @ p).ravel() for mp, p in zip(mps, parts))
     assert np.allclose(sum_adjoint(mps, model, parts), ref, rtol=1e-14, atol=1e-14)
+    # contiguous projections that tile the model (in any order) are written, not added; a gap
+    # or an overlap takes the accumulating path
+    for tiles in ([slice(20, 50), slice(0, 20)], [slice(0, 20), slice(21, 50)],
+                  [slice(0, 25), slice(20, 50)]):
+        mps = [maps.Projection(n, t) for t in tiles]
+        parts = [rng.standard_normal(mp.shape[0]) for mp in mps]
+        ref = sum(np.asarray(mp.deriv(model).T @ p).ravel() for mp, p in zip(mps, parts))
+        assert np.array_equal(sum_adjoint(mps, model, parts), ref)
 
 
 def test_edge_cell_weights_and_mesigma_deriv_match_oracle():
