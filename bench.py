@@ -163,7 +163,7 @@ def kernel_names(params, ncol):
     b = "bf16" if params.vcycle_bf16 in (3, 4) else "float"
     v = "float" if params.vcycle_bf16 == 0 else "bf16"
     return {
-        "update": f"k_update_r4<{'true' if params.vcycle_bf16 in (3, 4) else 'false'},1024>",
+        "update": f"k_update_r4<{'true' if params.vcycle_bf16 in (3, 4) else 'false'},2>",
         "smooth": f"k_cheb<float,{b},{v},{z},{r},1,true>",
         "apply": f"k_apply_pq<{z},{r},false>", "pair": f"k_smooth_pair<float,{b},{v},{r}>",
         "residual": f"k_residual_rx<float,{b},{v},{r},*>",

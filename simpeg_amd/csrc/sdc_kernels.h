@@ -906,56 +906,61 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_apply_pq(Sten<float> sa, const Z*
   block_partials<R * G>(acc, partial, c0, ncol, nbx, bx);
 }
 
-// r -= alpha q and per-column |r|^2 partials (4 nodes per work-item, NT work-items per
-// work-group: large work-groups, so that the single-work-group decision sums few partials)
+// r -= alpha q and per-column |r|^2 partials: a work-group takes CH chunks of SDC_NODES4 nodes
+// (4 per work-item and chunk, all loads issued before the arithmetic), one partial per
+// work-group, so that the single-work-group decision sums few partials per column
 // (R16: also write a bfloat16 copy of r, the multigrid input)
-template <bool R16, int NT>
-__global__ __launch_bounds__(NT) void k_update_r4(float* __restrict__ r,
-                                                  const float* __restrict__ q,
-                                                  const double* __restrict__ alpha, size_t ld,
-                                                  uint32_t nN, double* __restrict__ partial,
-                                                  __bf16* __restrict__ r16,
-                                                  const int* __restrict__ done) {
+template <bool R16, int CH>
+__global__ __launch_bounds__(SDC_BLOCK) void k_update_r4(float* __restrict__ r,
+                                                         const float* __restrict__ q,
+                                                         const double* __restrict__ alpha,
+                                                         size_t ld, uint32_t nN,
+                                                         double* __restrict__ partial,
+                                                         __bf16* __restrict__ r16,
+                                                         const int* __restrict__ done) {
   if (done && ld_flag(done)) return;
   const int c = blockIdx.y;
   const size_t o = (size_t)c * ld;
-  const uint32_t n0 = (blockIdx.x * NT + threadIdx.x) * 4;
   const float al = (float)alpha[c];
+  const uint32_t base = (blockIdx.x * CH * SDC_BLOCK + threadIdx.x) * 4;
+  vec4_t<float> rv[CH], qv[CH];
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    const uint32_t n0 = base + k * SDC_NODES4;
+    if (n0 + 3 < nN) {
+      rv[k] = *(const vec4_t<float>*)(r + o + n0);
+      qv[k] = *(const vec4_t<float>*)(q + o + n0);
+    }
+  }
   double acc = 0.0;
-  if (n0 + 3 < nN) {
-    vec4_t<float> rv = *(const vec4_t<float>*)(r + o + n0);
-    const vec4_t<float> qv = *(const vec4_t<float>*)(q + o + n0);
-    rv -= al * qv;
-    *(vec4_t<float>*)(r + o + n0) = rv;
-    if (R16) {
-      typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
-      bf4 h;
-      h.x = (__bf16)rv.x;
-      h.y = (__bf16)rv.y;
-      h.z = (__bf16)rv.z;
-      h.w = (__bf16)rv.w;
-      *(bf4*)(r16 + o + n0) = h;
-    }
-    acc = (double)rv.x * rv.x + (double)rv.y * rv.y + (double)rv.z * rv.z + (double)rv.w * rv.w;
-  } else {
-    for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
-      const float rn = r[o + n] - al * q[o + n];
-      r[o + n] = rn;
-      if (R16) r16[o + n] = (__bf16)rn;
-      acc += (double)rn * rn;
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    const uint32_t n0 = base + k * SDC_NODES4;
+    if (n0 + 3 < nN) {
+      const vec4_t<float> rn = rv[k] - al * qv[k];
+      *(vec4_t<float>*)(r + o + n0) = rn;
+      if (R16) {
+        typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+        bf4 h;
+        h.x = (__bf16)rn.x;
+        h.y = (__bf16)rn.y;
+        h.z = (__bf16)rn.z;
+        h.w = (__bf16)rn.w;
+        *(bf4*)(r16 + o + n0) = h;
+      }
+      acc += (double)rn.x * rn.x + (double)rn.y * rn.y + (double)rn.z * rn.z +
+             (double)rn.w * rn.w;
+    } else {
+      for (uint32_t n = n0; n < nN && n < n0 + 4; ++n) {
+        const float rn = r[o + n] - al * q[o + n];
+        r[o + n] = rn;
+        if (R16) r16[o + n] = (__bf16)rn;
+        acc += (double)rn * rn;
+      }
     }
   }
-  __shared__ double red[NT / 64];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < NT / 64; ++w) t += red[w];
-    partial[(size_t)c * gridDim.x + blockIdx.x] = t;
-  }
+  const double t = block_reduce1(acc);
+  if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
 }
 
 #define SDC_FOLD_CHUNKS 4

@@ -1173,10 +1173,10 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const uint32_t nbx = nblk(l0.nN);
   constexpr int kIrChunks = 4;
   const uint32_t nbx_ir = (nbx + kIrChunks - 1) / kIrChunks;
-  // r update: work-groups of kUpdThreads work-items (4 nodes each), so that k_norm_decide sums
+  // r update: kUpdChunks chunks of SDC_NODES4 nodes per work-group, so that k_norm_decide sums
   // few partials per column
-  constexpr int kUpdThreads = 1024;
-  const dim3 gupd((uint32_t)((l0.nN + 4 * kUpdThreads - 1) / (4 * kUpdThreads)), (uint32_t)ncol);
+  constexpr int kUpdChunks = 2;
+  const dim3 gupd((nblk4(l0.nN) + kUpdChunks - 1) / kUpdChunks, (uint32_t)ncol);
   const size_t len = l0.ld * (size_t)ncol;
   PinFix pf;
   pf.on = h->bc == SDC_BC_NEUMANN;
@@ -1293,11 +1293,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
            W.ahist + (size_t)phi * W.ncap);
     prof_begin(h, SDC_PROF_UPDATE, s);
     if (r16)
-      launch(h, k_update_r4<true, kUpdThreads>, gupd, dim3(kUpdThreads), 0, s, W.r,
-             (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
+      launch(h, k_update_r4<true, kUpdChunks>, gupd, blk, 0, s, W.r, (const float*)W.q,
+             (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
     else
-      launch(h, k_update_r4<false, kUpdThreads>, gupd, dim3(kUpdThreads), 0, s, W.r,
-             (const float*)W.q, (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
+      launch(h, k_update_r4<false, kUpdChunks>, gupd, blk, 0, s, W.r, (const float*)W.q,
+             (const double*)W.alpha, l0.ld, l0.nN, W.partial, r16, stop);
     prof_end(h, SDC_PROF_UPDATE, s, (double)l0.nN * ncol * (r16 ? 14 : 12));
     launch(h, k_norm_decide, dim3(1), dim3(SDC_FIN_THREADS), 0, s, (const double*)W.partial,
            gupd.x, W.rr, W.ctl, (const double*)W.aux, (const double*)W.rmax,

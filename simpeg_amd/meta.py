@@ -97,6 +97,17 @@ class _Single:
         return [self.sim.Jtvec(self.sim.model, v[0], f=f[0])]
 
 
+def _download(x):
+    """A device tensor as a host array: through page-locked memory (a DMA copy on the current
+    stream; torch's host allocator keeps the block for the next call) when x is on a GPU."""
+    if not x.is_cuda:
+        return x.numpy()
+    host = torch.empty(x.shape, dtype=x.dtype, pin_memory=True)
+    host.copy_(x, non_blocking=True)
+    torch.cuda.current_stream(x.device).synchronize()
+    return host.numpy()
+
+
 def _host_buffer(shape):
     """Host float64 storage for the private model copy: a page-locked torch tensor when a GPU is
     present (the groups upload their members' blocks of it asynchronously), else an array."""
@@ -282,13 +293,13 @@ class _SetGroup:
             # -sigmaDeriv^T g on the device (diagonal derivative), one download for all members
             fm = self._shared_device_map()
             if fm is not None:
-                flat = (-(fm[1](self._m_dev) * G)).cpu().numpy()
+                flat = _download(-(fm[1](self._m_dev) * G))
             else:
                 fms = self._device_maps()
-                flat = torch.stack([-(fms[k][1](self._m_dev[k]) * G[k])
-                                    for k in range(len(self.sims))]).cpu().numpy()
+                flat = _download(torch.stack([-(fms[k][1](self._m_dev[k]) * G[k])
+                                              for k in range(len(self.sims))]))
             return [flat[k] for k in range(len(self.sims))]
-        Gh = G.cpu().numpy()
+        Gh = _download(G)
         return [-s._sigma_deriv_T(Gh[k]) for k, s in enumerate(self.sims)]
 
 
