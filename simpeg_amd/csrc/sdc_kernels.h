@@ -1094,16 +1094,38 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_fin_norm(const double* __restrict
 }
 
 // The mixed PCG's per-iteration finalisers run as one work-group of SDC_FIN_THREADS: a wave
-// sums a column's block partials (lane-strided, then a butterfly; a fixed order), so the
-// partials come from kernels that write few per column.
+// sums the block partials of SDC_FIN_COLS columns at a time (lane-strided, then a butterfly; a
+// fixed order), so the partials come from kernels that write few per column.
 #define SDC_FIN_THREADS 1024
+#define SDC_FIN_COLS 8
 
-__device__ __forceinline__ double wave_sum(const double* __restrict__ p, uint32_t n) {
-  double v = 0.0;
-  for (uint32_t b = threadIdx.x & 63; b < n; b += 64) v += p[b];
+// The sums of the n partials of columns c + k step, k < C (those below ncol; column j's
+// partials at p + j * n) by one wave: each lane issues the loads of all C columns at 4 strides
+// together (guarded), then one butterfly per column.  Every lane returns all sums.
+template <int C>
+__device__ __forceinline__ void wave_sums(const double* __restrict__ p, uint32_t n, int c,
+                                          int step, int ncol, double (&v)[C]) {
+  const uint32_t lane = threadIdx.x & 63;
+  const double* q[C];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  for (int k = 0; k < C; ++k) {
+    v[k] = 0.0;
+    const int ck = c + k * step < ncol ? c + k * step : c;
+    q[k] = p + (size_t)ck * n;
+  }
+  for (uint32_t b = lane; b < n; b += 256) {
+    double a[C][4];
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[k][u] = b + 64 * u < n ? q[k][b + 64 * u] : 0.0;
+#pragma unroll
+    for (int k = 0; k < C; ++k) v[k] += (a[k][0] + a[k][1]) + (a[k][2] + a[k][3]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < C; ++k) v[k] += __shfl_xor(v[k], off, 64);
 }
 
 // |r|^2 per column (rr) from the update's partials, then the reliable-update decision: counts
@@ -1126,12 +1148,20 @@ __global__ __launch_bounds__(SDC_FIN_THREADS) void k_norm_decide(const double* _
   const double delta2 = ld_flag(ctl_dbl(ctl) + SDC_DCTL_DELTA2);
   const int maxit = ld_flag(ctl + SDC_CTL_MAXIT), window = ld_flag(ctl + SDC_CTL_WINDOW);
   int cand = 1, drop = 0;
-  for (int c = (int)threadIdx.x >> 6; c < ncol; c += SDC_FIN_THREADS / 64) {
-    const double v = wave_sum(partial + (size_t)c * nbx, nbx);
-    if ((threadIdx.x & 63) == 0) rr[c] = v;
-    if (!active[c]) continue;
-    if (v > tol2 * qn2[c]) cand = 0;
-    if (v <= delta2 * rmax[c]) drop = 1;
+  constexpr int NW = SDC_FIN_THREADS / 64;
+  for (int c0 = (int)threadIdx.x >> 6; c0 < ncol; c0 += SDC_FIN_COLS * NW) {
+    double vs[SDC_FIN_COLS];
+    wave_sums<SDC_FIN_COLS>(partial, nbx, c0, NW, ncol, vs);
+#pragma unroll
+    for (int k = 0; k < SDC_FIN_COLS; ++k) {
+      const int c = c0 + k * NW;
+      if (c >= ncol) break;
+      const double v = vs[k];
+      if ((threadIdx.x & 63) == 0) rr[c] = v;
+      if (!active[c]) continue;
+      if (v > tol2 * qn2[c]) cand = 0;
+      if (v <= delta2 * rmax[c]) drop = 1;
+    }
   }
   cand = __syncthreads_and(cand);
   drop = __syncthreads_or(drop);
@@ -1178,17 +1208,25 @@ __global__ __launch_bounds__(SDC_FIN_THREADS) void k_true_after(const double* __
   const int lane = threadIdx.x & 63;
   double m = 0.0;
   int nact = 0;
-  for (int c = (int)threadIdx.x >> 6; c < ncol; c += NW) {
-    const double v = wave_sum(partial + (size_t)c * nbx, nbx);
-    const double q2 = qn2[c];
-    const int a = (q2 > 0.0 && v > tol2 * q2) ? 1 : 0;
-    if (lane == 0) {
-      orr[c] = v;
-      rmax[c] = v;
-      active[c] = a;
-      nact += a;
+  // 4 columns per wave at a time: this kernel holds more state (runs at reliable updates only)
+  for (int c0 = (int)threadIdx.x >> 6; c0 < ncol; c0 += 4 * NW) {
+    double vs[4];
+    wave_sums<4>(partial, nbx, c0, NW, ncol, vs);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int c = c0 + k * NW;
+      if (c >= ncol) break;
+      const double v = vs[k];
+      const double q2 = qn2[c];
+      const int a = (q2 > 0.0 && v > tol2 * q2) ? 1 : 0;
+      if (lane == 0) {
+        orr[c] = v;
+        rmax[c] = v;
+        active[c] = a;
+        nact += a;
+      }
+      if (q2 > 0.0) m = fmax(m, sqrt(v / q2));
     }
-    if (q2 > 0.0) m = fmax(m, sqrt(v / q2));
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
