@@ -1171,7 +1171,9 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const dim3 g = grid2(l0.nN, ncol, R);
   const dim3 blk(SDC_BLOCK);
   const uint32_t nbx = nblk(l0.nN);
-  constexpr int kIrChunks = 4;
+  // 8 node blocks per work-group of the fp64 residual: its empty launches (no reliable update)
+  // are cheaper, and its full pass was not slower (cfg 2: 686 against 705 us with 4)
+  constexpr int kIrChunks = 8;
   const uint32_t nbx_ir = (nbx + kIrChunks - 1) / kIrChunks;
   // r update: kUpdChunks chunks of SDC_NODES4 nodes per work-group, so that k_norm_decide sums
   // few partials per column
