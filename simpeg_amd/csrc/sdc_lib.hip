@@ -1171,9 +1171,11 @@ int pcg_reliable(sdc_handle_s* h, const double* q, double* u, int ncol, const sd
   const dim3 g = grid2(l0.nN, ncol, R);
   const dim3 blk(SDC_BLOCK);
   const uint32_t nbx = nblk(l0.nN);
-  // 8 node blocks per work-group of the fp64 residual: its empty launches (no reliable update)
-  // are cheaper, and its full pass was not slower (cfg 2: 686 against 705 us with 4)
-  constexpr int kIrChunks = 8;
+  // 4 node blocks per work-group of the fp64 residual: its empty launches (no reliable update)
+  // are cheap (cfg 2: 9.7 us).  With 8 its full pass took 686-702 against 705-733 us but
+  // fetched 3.84 against 2.77 GB from HBM (5.5 TB/s), too close to the bandwidth that
+  // concurrent streams share.
+  constexpr int kIrChunks = 4;
   const uint32_t nbx_ir = (nbx + kIrChunks - 1) / kIrChunks;
   // r update: kUpdChunks chunks of SDC_NODES4 nodes per work-group, so that k_norm_decide sums
   // few partials per column
