@@ -1853,25 +1853,30 @@ __global__ __launch_bounds__(SDC_BLOCK) void k_coarsen_coef(Xfer<double> t,
 //      work-item;
 //   2. in-place Gauss-Jordan elimination without pivoting (the operator is SPD), every pivot
 //      step updating all nn*nn entries of a and inv in parallel (f: the elimination factors);
-//   3. inv <- (inv + inv^T) / 2 and its fp32 copy inv32.
-__global__ __launch_bounds__(1024) void k_coarse_inverse(Sten<double> s, double* __restrict__ a,
-                                                         double* __restrict__ inv,
+//   3. inv <- (inv + inv^T) / 2 into inv64 and its fp32 copy inv32.
+// SH: a, inv and f live in LDS ((2 nn^2 + nn) doubles of dynamic shared memory), so the pivot
+// steps do not go through L2; otherwise in the global scratch ga, gf and in inv64.
+template <bool SH>
+__global__ __launch_bounds__(1024) void k_coarse_inverse(Sten<double> s, double* __restrict__ ga,
+                                                         double* __restrict__ inv64,
                                                          float* __restrict__ inv32,
-                                                         double* __restrict__ f,
+                                                         double* __restrict__ gf,
                                                          size_t astride) {
+  extern __shared__ __align__(16) double cinv_smem[];
   const size_t zo = (size_t)blockIdx.x * s.sstride;
   s.cx += zo;
   s.cy += zo;
   s.cz += zo;
   s.dg += zo;
   s.idg += zo;
-  a += (size_t)blockIdx.x * astride;
-  inv += (size_t)blockIdx.x * astride;
+  inv64 += (size_t)blockIdx.x * astride;
   inv32 += (size_t)blockIdx.x * astride;
   const int N = (int)s.nN;
-  f += (size_t)blockIdx.x * N;
-  const int tid = threadIdx.x;
   const int NN = N * N;
+  double* a = SH ? cinv_smem : ga + (size_t)blockIdx.x * astride;
+  double* inv = SH ? cinv_smem + NN : inv64;
+  double* f = SH ? cinv_smem + 2 * NN : gf + (size_t)blockIdx.x * N;
+  const int tid = threadIdx.x;
   for (int n = tid; n < N; n += 1024) {
     double* row = a + (size_t)n * N;
     for (int m = 0; m < N; ++m) row[m] = 0.0;
@@ -1912,8 +1917,8 @@ __global__ __launch_bounds__(1024) void k_coarse_inverse(Sten<double> s, double*
     const int r = e / N, c = e % N;
     if (c < r) continue;
     const double v = 0.5 * (inv[r * N + c] + inv[c * N + r]);
-    inv[r * N + c] = v;
-    inv[c * N + r] = v;
+    inv64[r * N + c] = v;
+    inv64[c * N + r] = v;
     inv32[r * N + c] = (float)v;
     inv32[c * N + r] = (float)v;
   }

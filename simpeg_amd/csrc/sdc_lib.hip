@@ -150,6 +150,9 @@ struct Level {
 // doubles taken by the int part of a control block
 constexpr int kCtlInts = SDC_CTL_N * (int)sizeof(int) / (int)sizeof(double);
 
+// dynamic LDS of the coarsest inverse when it runs in LDS (the part's 160 KB per work-group)
+constexpr size_t kCoarseInvLds = 160 * 1024;
+
 template <typename T>
 struct Work {
   int ncap = 0;
@@ -1584,11 +1587,15 @@ int sdc_create(const sdc_mesh_desc* d, sdc_handle* out) {
     h->d_ctl = nullptr;
     (void)hipGetLastError();
   }
-  // the fused coarse tail may use all 160 KB of LDS (set once, outside any graph capture)
+  // the fused coarse tail and the coarsest inverse may use all 160 KB of LDS (set once,
+  // outside any graph capture)
   if (hipFuncSetAttribute((const void*)k_tail_vcycle<__bf16>,
                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_tail_vcycle<float>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_coarse_inverse<true>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCoarseInvLds) !=
+          hipSuccess)
     return bail(fail(SDC_ERR_HIP, "hipFuncSetAttribute failed"));
   *out = h;
   return SDC_OK;
@@ -1751,8 +1758,14 @@ int build_levels(sdc_handle_s* h, hipStream_t st, int nset) {
   const Sten<double> sc = make_sten<double>(h, lc, 1);
   const size_t nc = h->ncoarse;
   if (sc.nN != nc) return fail(SDC_ERR_STATE, "internal: coarsest level size");
-  hipLaunchKernelGGL(k_coarse_inverse, dim3(z), dim3(1024), 0, st, sc, h->gj_a, h->ainv64,
-                     h->ainv32, h->gj_f, h->astride);
+  // the elimination in LDS when a, inv and the factors fit (nn <= 99 with 160 KB)
+  const size_t cinv_lds = (2 * nc * nc + nc) * sizeof(double);
+  if (cinv_lds <= kCoarseInvLds)
+    hipLaunchKernelGGL(k_coarse_inverse<true>, dim3(z), dim3(1024), cinv_lds, st, sc, h->gj_a,
+                       h->ainv64, h->ainv32, h->gj_f, h->astride);
+  else
+    hipLaunchKernelGGL(k_coarse_inverse<false>, dim3(z), dim3(1024), 0, st, sc, h->gj_a,
+                       h->ainv64, h->ainv32, h->gj_f, h->astride);
   HIPCK(hipGetLastError());
   h->nset = nset;
   h->sigma_set = true;
