@@ -150,7 +150,8 @@ struct Level {
 // doubles taken by the int part of a control block
 constexpr int kCtlInts = SDC_CTL_N * (int)sizeof(int) / (int)sizeof(double);
 
-// dynamic LDS of the coarsest inverse when it runs in LDS (the part's 160 KB per work-group)
+// dynamic LDS of the coarsest inverse when it runs in LDS (160 KB: all a gfx950 work-group
+// may use)
 constexpr size_t kCoarseInvLds = 160 * 1024;
 
 template <typename T>
