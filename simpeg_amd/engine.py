@@ -351,6 +351,17 @@ class NodalEngine:
         return out
 
 
+def download(x):
+    """A device tensor as a host array: through page-locked memory (a DMA copy on the current
+    stream; torch's host allocator keeps the block for the next call) when x is on a GPU."""
+    if not x.is_cuda:
+        return x.numpy()
+    host = torch.empty(x.shape, dtype=x.dtype, pin_memory=True)
+    host.copy_(x, non_blocking=True)
+    torch.cuda.current_stream(x.device).synchronize()
+    return host.numpy()
+
+
 def coo_to_gather(rows, flat, w, nrow):
     """COO (row, flat index, weight) -> CSR arrays (duplicates kept)."""
     order = np.lexsort((flat, rows))

@@ -22,6 +22,7 @@ import numpy as np
 import torch
 
 from .engine import NodalEngine
+from .engine import download as _download
 from .maps import ExpMap, IdentityMap, Projection, ReciprocalMap
 
 # elementwise maps whose device evaluation has no per-instance state
@@ -95,17 +96,6 @@ class _Single:
 
     def jtvec(self, v, f):
         return [self.sim.Jtvec(self.sim.model, v[0], f=f[0])]
-
-
-def _download(x):
-    """A device tensor as a host array: through page-locked memory (a DMA copy on the current
-    stream; torch's host allocator keeps the block for the next call) when x is on a GPU."""
-    if not x.is_cuda:
-        return x.numpy()
-    host = torch.empty(x.shape, dtype=x.dtype, pin_memory=True)
-    host.copy_(x, non_blocking=True)
-    torch.cuda.current_stream(x.device).synchronize()
-    return host.numpy()
 
 
 def _host_buffer(shape):

@@ -24,13 +24,15 @@ Results are host numpy arrays, as in the reference.
 
 from __future__ import annotations
 
+import warnings
+
 import numpy as np
 import scipy.sparse as sp
 import torch
 
 from .. import maps as _maps
 from ..data import Data, SyntheticData
-from ..engine import NodalEngine, coo_to_gather, coo_to_scatter
+from ..engine import NodalEngine, coo_to_gather, coo_to_scatter, download
 from ..mesh import edge_mass, mesh_widths, node_interpolation, robin_diagonal
 from .fields import Fields3DNodal
 from .survey import Survey
@@ -124,6 +126,7 @@ class Simulation3DNodal:
         self._model = None
         self._model_version = 0  # counts model changes (operators built elsewhere check it)
         self._sigma_on_gpu = None
+        self._m_dev = None  # the model on the device when sigma was evaluated there
         self._Jmatrix = None
         self._gtgdiag = None
         self._Av = None
@@ -285,6 +288,20 @@ class Simulation3DNodal:
 
     def _ensure_sigma(self):
         if self._sigma_on_gpu is None or self.sigmaMap is None and self.rhoMap is None:
+            self._m_dev = None
+            fm = self._device_map() if self._model is not None else None
+            if fm is not None and self._model.size == self._nC:
+                # an elementwise map: the model goes to the device once and sigma (and later
+                # its derivative) is evaluated there
+                with warnings.catch_warnings():
+                    # a model held as a read-only view (MetaSimulation's copy) is only read
+                    warnings.simplefilter("ignore", UserWarning)
+                    m_t = torch.from_numpy(np.ascontiguousarray(self._model, dtype=float))
+                m_t = m_t.to(self.engine.device)
+                self.engine.set_sigma(fm[0](m_t).contiguous())
+                self._m_dev = m_t
+                self._sigma_on_gpu = True
+                return
             sig = self.sigma
             if sig.size != self._nC:
                 # the reference raises this for Robin (resistivity/simulation.py:546-551); its
@@ -478,8 +495,11 @@ class Simulation3DNodal:
         v = self._mini_dataT(v)
         rhs = eng.scatter(pr["scatter"], v, nS)
         lam = self._solve(rhs, nS)
-        g = eng.contract(f.u_block, lam, nS).cpu().numpy()
-        return -self._sigma_deriv_T(g)
+        g = eng.contract(f.u_block, lam, nS)
+        if self._m_dev is not None:
+            # -dsigma/dm^T g with the diagonal derivative evaluated on the device model
+            return download(-(self._device_map()[1](self._m_dev) * g))
+        return -self._sigma_deriv_T(download(g))
 
     def getJ(self, m, f=None):
         """Dense sensitivity (nD, nP) (``getJ``, simulation.py:115-121), one adjoint column per datum."""
